@@ -1,0 +1,166 @@
+"""CPU oracle for NetworKit's PageRank -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this package, and only as the checker (or, for the baseline, as the thing timed on the
+host cores).  The product (``networkit_amd``) never imports it and has no CPU fallback.
+
+The arithmetic lives in ``pagerank_oracle.c`` (a restatement of
+networkit/cpp/centrality/PageRank.cpp:20-104); this module loads it with ctypes and adds the
+NetworKit-semantics edge-list -> in-list conversion (Graph.cpp:549-597, Graph.hpp:1918-1938)
+that the C code consumes.
+
+Parity status of each function is pinned in tests/test_oracle.py:
+  pagerank        reference known answers (CentralityGTest.cpp:484-664), sparse direct solve,
+                  dense power iteration, invariants
+  ranking         brute-force sort
+  in_lists        hand-built NetworKit adjacency of small graphs (directed, undirected,
+                  self-loops, multi-edges)
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_pagerank.so")
+_lib = None
+
+L1 = 0
+L2 = 1
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    p = ctypes.c_void_p
+    u64 = ctypes.c_uint64
+    lib.oracle_pagerank.argtypes = [u64, p, p, p, p, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, u64, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, p, ctypes.POINTER(u64),
+                                    ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_pagerank.restype = ctypes.c_int
+    lib.oracle_ranking.argtypes = [u64, p, p, p]
+    lib.oracle_ranking.restype = u64
+    lib.oracle_out_degrees.argtypes = [u64, p, p, p, p, p]
+    lib.oracle_out_degrees.restype = None
+    lib.oracle_time_sweeps.argtypes = [u64, p, p, p, ctypes.c_double, u64, ctypes.c_int, p]
+    lib.oracle_time_sweeps.restype = ctypes.c_double
+    lib.oracle_num_threads.argtypes = []
+    lib.oracle_num_threads.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class InLists:
+    """Flattened NetworKit in-lists: in_ptr[z+1], in_src[m] (uint64), in_w[m] (float64 or
+    None), exists[z] (uint8 or None)."""
+
+    def __init__(self, z, in_ptr, in_src, in_w=None, exists=None, directed=True):
+        self.z = int(z)
+        self.in_ptr = np.ascontiguousarray(in_ptr, dtype=np.uint64)
+        self.in_src = np.ascontiguousarray(in_src, dtype=np.uint64)
+        self.in_w = None if in_w is None else np.ascontiguousarray(in_w, dtype=np.float64)
+        self.exists = None if exists is None else np.ascontiguousarray(exists, dtype=np.uint8)
+        self.directed = directed
+
+    @property
+    def m(self):
+        return int(self.in_ptr[-1])
+
+
+def in_lists(z, u, v, w=None, directed=True, exists=None):
+    """Build the in-lists NetworKit's Graph would hold after ``addEdge(u[i], v[i], w[i])`` in
+    order i = 0, 1, ...  (Graph.cpp:549-597).  Directed: inEdges[v] gets u.  Undirected:
+    outEdges[u] gets v and, unless u == v, outEdges[v] gets u (a self-loop is stored once).
+    The relative order of entries within a list is insertion order, as in NetworKit."""
+    u = np.asarray(u, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64)
+    wt = None if w is None else np.asarray(w, dtype=np.float64)
+    if directed:
+        rows, srcs, seq = v, u, np.arange(len(u), dtype=np.int64)
+        ws = wt
+    else:
+        loop = u == v
+        k = len(u)
+        # event order: edge i contributes (row u, src v) then (row v, src u) if not a loop
+        rows = np.concatenate([u, v[~loop]])
+        srcs = np.concatenate([v, u[~loop]])
+        seq = np.concatenate([2 * np.arange(k), 2 * np.arange(k)[~loop] + 1])
+        ws = None if wt is None else np.concatenate([wt, wt[~loop]])
+    order = np.lexsort((seq, rows))
+    rows_s = rows[order]
+    counts = np.bincount(rows_s, minlength=z) if len(rows_s) else np.zeros(z, np.int64)
+    ptr = np.zeros(z + 1, dtype=np.uint64)
+    ptr[1:] = np.cumsum(counts)
+    src = srcs[order].astype(np.uint64)
+    wout = None if ws is None else ws[order]
+    return InLists(z, ptr, src, wout, exists, directed)
+
+
+def pagerank(g, damp=0.85, tol=1e-8, norm=L2, max_iterations=2**64 - 1, normalized=False,
+             sink_mode=0, start=None, threads=0):
+    """Run the restated PageRank::run.  ``start`` is the start vector (default 1/n in every
+    slot, as a freshly constructed reference object).  Returns (scores, iterations, maximum)."""
+    lib = _load()
+    z = g.z
+    n = z if g.exists is None else int(g.exists.sum())
+    if start is None:
+        scores = np.full(z, 1.0 / max(n, 1), dtype=np.float64)
+    else:
+        scores = np.array(start, dtype=np.float64, copy=True)
+    it = ctypes.c_uint64(0)
+    mx = ctypes.c_double(0.0)
+    rc = lib.oracle_pagerank(z, _ptr(g.exists), _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w),
+                             damp, tol, norm, int(max_iterations), int(bool(normalized)),
+                             int(sink_mode), int(threads), _ptr(scores), ctypes.byref(it),
+                             ctypes.byref(mx))
+    if rc != 0:
+        raise RuntimeError("oracle_pagerank failed")
+    return scores, int(it.value), float(mx.value)
+
+
+def ranking(scores, exists=None):
+    """Existing node ids in Centrality::ranking order (score desc, id asc)."""
+    lib = _load()
+    scores = np.ascontiguousarray(scores, dtype=np.float64)
+    z = len(scores)
+    ids = np.zeros(max(z, 1), dtype=np.uint64)
+    ex = None if exists is None else np.ascontiguousarray(exists, dtype=np.uint8)
+    k = lib.oracle_ranking(z, _ptr(ex), _ptr(scores), _ptr(ids))
+    return ids[:k].astype(np.int64)
+
+
+def out_degrees(g):
+    """(weighted out-degree float64[z], unweighted out-count uint64[z])."""
+    lib = _load()
+    deg = np.zeros(g.z, dtype=np.float64)
+    cnt = np.zeros(g.z, dtype=np.uint64)
+    lib.oracle_out_degrees(g.z, _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w), _ptr(deg),
+                           _ptr(cnt))
+    return deg, cnt
+
+
+def time_sweeps(g, damp=0.85, sweeps=3, threads=0):
+    """Wall seconds of ``sweeps`` reference sweeps on the host cores (CPU baseline)."""
+    lib = _load()
+    x = np.full(g.z, 1.0 / max(g.z, 1), dtype=np.float64)
+    return lib.oracle_time_sweeps(g.z, _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w), damp,
+                                  int(sweeps), int(threads), _ptr(x))
+
+
+def num_threads():
+    return int(_load().oracle_num_threads())

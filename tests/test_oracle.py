@@ -1,0 +1,183 @@
+"""Pins the CPU oracle (oracle/pagerank_oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  - reference known answers (8-node Matlab values, wiki-Vote top-1 in four variants) catch a
+    wrong in-edge direction, a missing damping/teleport term or a wrong normalisation;
+  - the sparse direct solve catches a dropped or transposed term at 1e-9 instead of 1e-4;
+  - dense power iteration pins the sink-distribution mode, which the reference lacks;
+  - invariants (uniform scores on cycles and regular graphs, automorphic pairs, iteration cap,
+    warm start) pin the loop control.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from closed_forms import dense_power_iteration, pagerank_closed_form
+from graphs import (EIGHT_PR, EIGHT_PR_NORMALIZED, WIKI_TOP, WIKI_TOP_NORMALIZED, eight_node,
+                    random_graph, skewed_graph, wiki_vote)
+
+
+def test_in_lists_directed_order():
+    g = oracle.in_lists(3, [0, 1, 0, 2], [1, 2, 2, 2], directed=True)
+    assert g.in_ptr.tolist() == [0, 0, 1, 4]
+    assert g.in_src.tolist() == [0, 1, 0, 2]
+
+
+def test_in_lists_undirected_selfloop_once():
+    g = oracle.in_lists(3, [0, 1, 1], [1, 1, 2], [2.0, 5.0, 7.0], directed=False)
+    # node 0: {1}; node 1: {0, 1 (loop once), 2}; node 2: {1}
+    assert g.in_ptr.tolist() == [0, 1, 4, 5]
+    assert g.in_src.tolist() == [1, 0, 1, 2, 1]
+    assert g.in_w.tolist() == [2.0, 2.0, 5.0, 7.0, 7.0]
+    deg, cnt = oracle.out_degrees(g)
+    assert deg.tolist() == [2.0, 14.0, 7.0]
+    assert cnt.tolist() == [1, 3, 1]
+
+
+def test_eight_node_known_answers():
+    z, u, v, w = eight_node()
+    g = oracle.in_lists(z, u, v, w, directed=False)
+    s, it, mx = oracle.pagerank(g, 0.85)
+    assert np.allclose(s, EIGHT_PR, atol=1e-4)
+    assert mx == 1.0
+    s, it, mx = oracle.pagerank(g, 0.85, normalized=True)
+    assert np.allclose(s, EIGHT_PR_NORMALIZED, atol=1e-4)
+    assert mx == pytest.approx(s.max())
+
+
+@pytest.mark.parametrize("directed", [True, False])
+@pytest.mark.parametrize("norm", [oracle.L1, oracle.L2])
+def test_wiki_vote_known_answers(directed, norm):
+    z, u, v = wiki_vote(directed)
+    g = oracle.in_lists(z, u, v, None, directed)
+    s, it, _ = oracle.pagerank(g, norm=norm)
+    r = oracle.ranking(s)
+    top, val = WIKI_TOP[directed]
+    assert r[0] == top and abs(s[top] - val) <= 1e-3
+    s2, it2, _ = oracle.pagerank(g, norm=norm, max_iterations=2, start=s)
+    assert it2 <= 2
+    sn, _, mx = oracle.pagerank(g, norm=norm, normalized=True)
+    rn = oracle.ranking(sn)
+    topn, valn = WIKI_TOP_NORMALIZED[directed]
+    assert rn[0] == topn and abs(sn[topn] - valn) <= 1e-4
+    assert mx == sn.max()
+
+
+@pytest.mark.parametrize("case", ["er_dir", "er_undir_w", "skew_dir_w", "wiki_dir"])
+@pytest.mark.parametrize("sink_mode", [0, 1])
+def test_oracle_matches_direct_solve(case, sink_mode):
+    if case == "er_dir":
+        z, u, v, w = random_graph(3000, 9000, 1, directed=True)
+        g = oracle.in_lists(z, u, v, w, True)
+    elif case == "er_undir_w":
+        z, u, v, w = random_graph(2000, 6000, 2, directed=False, weighted=True)
+        g = oracle.in_lists(z, u, v, w, False)
+    elif case == "skew_dir_w":
+        z, u, v, w = skewed_graph(4000, 30000, 3)
+        g = oracle.in_lists(z, u, v, w, True)
+    else:
+        z, u, v = wiki_vote(True)
+        g = oracle.in_lists(z, u, v, None, True)
+    s, it, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=5000,
+                               sink_mode=sink_mode)
+    ref = pagerank_closed_form(g, 0.85, sink_mode)
+    assert np.abs(s - ref).sum() < 1e-11
+    # a wrong edge direction (transposed operator) is caught
+    gt = oracle.in_lists(z, g_src_rows(g)[1], g_src_rows(g)[0], g.in_w, True)
+    st, _, _ = oracle.pagerank(gt, tol=1e-15, norm=oracle.L1, max_iterations=5000,
+                               sink_mode=sink_mode)
+    if case != "er_undir_w":
+        assert np.abs(st - ref).sum() > 1e-3
+
+
+def g_src_rows(g):
+    rows = np.repeat(np.arange(g.z), np.diff(g.in_ptr.astype(np.int64)))
+    return g.in_src.astype(np.int64), rows
+
+
+@pytest.mark.parametrize("sink_mode", [0, 1])
+def test_oracle_matches_dense_power(sink_mode):
+    z, u, v, w = random_graph(60, 150, 7, directed=True, weighted=True)
+    g = oracle.in_lists(z, u, v, w, True)
+    s, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=10000,
+                              sink_mode=sink_mode)
+    ref = dense_power_iteration(g, 0.85, sink_mode)
+    assert np.abs(s - ref).sum() < 1e-12
+
+
+def test_distribute_sinks_keeps_mass():
+    z, u, v, w = random_graph(500, 700, 11, directed=True)
+    g = oracle.in_lists(z, u, v, w, True)
+    deg, cnt = oracle.out_degrees(g)
+    assert (cnt == 0).sum() > 10
+    s, _, _ = oracle.pagerank(g, tol=1e-14, norm=oracle.L1, max_iterations=1,
+                              sink_mode=1)
+    # one sweep from 1/n keeps total mass 1 exactly (up to rounding) before normalisation
+    x0 = np.full(z, 1.0 / z)
+    sinks = cnt == 0
+    import closed_forms
+    P, idx, exists, snk, n = closed_forms._operator(g)
+    y = 0.85 * (P @ x0 + x0[sinks].sum() / n) + 0.15 / n
+    assert abs(y.sum() - 1.0) < 1e-12
+    assert np.allclose(s, y / y.sum(), rtol=1e-12, atol=0)
+
+
+def test_uniform_on_cycle_and_regular():
+    n = 97
+    u = np.arange(n)
+    g = oracle.in_lists(n, u, (u + 1) % n, None, True)
+    s, _, _ = oracle.pagerank(g)
+    assert np.allclose(s, 1.0 / n, rtol=0, atol=1e-15)
+    g = oracle.in_lists(n, np.concatenate([u, u]), np.concatenate([(u + 1) % n, (u + 5) % n]),
+                        None, False)
+    s, _, _ = oracle.pagerank(g)
+    assert np.allclose(s, 1.0 / n, rtol=0, atol=1e-15)
+
+
+def test_automorphic_pairs_equal():
+    z, u, v, w = eight_node()
+    g = oracle.in_lists(z, u, v, w, directed=False)
+    s, _, _ = oracle.pagerank(g, tol=1e-14)
+    for a, b in [(0, 6), (1, 7), (2, 5)]:
+        assert s[a] == pytest.approx(s[b], rel=1e-12)
+
+
+def test_no_in_edges_gets_teleport():
+    # node 0 has no in-edges: after any sweep its unnormalised score is (1-d)/n exactly
+    g = oracle.in_lists(4, [0, 1, 2, 3], [1, 2, 3, 1], None, True)
+    s, it, _ = oracle.pagerank(g, max_iterations=1)
+    # single sweep from uniform: node 0 -> 0.15/4, normalised by the sum
+    y = np.array([0.0375, 0.85 * (0.25 + 0.25) + 0.0375, 0.85 * 0.25 + 0.0375,
+                  0.85 * 0.25 + 0.0375])
+    assert np.allclose(s, y / y.sum(), rtol=1e-15)
+    assert it == 1
+
+
+def test_max_iterations_zero_still_sweeps_once():
+    z, u, v, w = random_graph(50, 100, 3)
+    g = oracle.in_lists(z, u, v, w, True)
+    _, it, _ = oracle.pagerank(g, max_iterations=0)
+    assert it == 1
+
+
+def test_deleted_slots_keep_start_value():
+    z, u, v, w = random_graph(40, 120, 4, directed=True)
+    keep = (u != 7) & (v != 7) & (u != 13) & (v != 13)
+    exists = np.ones(z, np.uint8)
+    exists[[7, 13]] = 0
+    g = oracle.in_lists(z, u[keep], v[keep], None, True, exists=exists)
+    s, _, mx = oracle.pagerank(g, normalized=True, tol=1e-15, norm=oracle.L1,
+                               max_iterations=5000)
+    assert s[7] == 1.0 / 38 and s[13] == 1.0 / 38
+    ref = pagerank_closed_form(g, 0.85, 0, normalized=True)
+    assert np.abs(s[exists == 1] - ref[exists == 1]).sum() < 1e-11
+
+
+def test_ranking_bruteforce():
+    rng = np.random.default_rng(5)
+    s = rng.integers(0, 20, 500).astype(np.float64) / 7.0
+    exists = (rng.random(500) > 0.1).astype(np.uint8)
+    r = oracle.ranking(s, exists)
+    ids = [i for i in range(500) if exists[i]]
+    ids.sort(key=lambda i: (-s[i], i))
+    assert r.tolist() == ids
