@@ -1,0 +1,211 @@
+"""PageRank throughput on one MI355X: mixed precision (FP32 sweeps + FP64 refinement) against
+plain FP64 at equal certified L1 error, on a directed R-MAT graph.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26] [--no-cpu]
+
+One step = one complete PageRank solve (damp 0.85, certified L1 error <= 1e-8 on the
+sum-normalised scores) of the graph already resident in HBM, from the uniform start vector.
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(m, n, weighted, kind):
+    """Bytes one sweep must move (each array element once, one gather per in-edge).
+    fp32 plain:  per edge src 4 + coef 4 + gather 4;  per node colptr 8 + x 4 + x' 4
+    fp32 corr:   as plain + right-hand side 4 per node
+    fp64:        per edge src 4 + coef 8 + gather 8;  per node colptr 8 + x 8 + x' 8 (+ r 4)
+    unweighted (prescaled gather, no coefficient stream): drop coef, add 1/deg read and the
+    prescaled write per node."""
+    if kind == "fp32":
+        e = 12 if weighted else 8
+        v = 16 if weighted else 24
+    elif kind == "fp32corr":
+        e = 12 if weighted else 8
+        v = 20 if weighted else 28
+    else:  # fp64 residual sweep (writes the FP32 residual too)
+        e = 20 if weighted else 12
+        v = 28 if weighted else 44
+    return e * m + v * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.time()
+    g = gen.rmat(args.scale, args.edge_factor, seed=2, device="cuda")
+    torch.cuda.synchronize()
+    z = g["z"]
+    m = g["src"].numel()
+    log("R-MAT scale %d: z=%d m=%d generated in %.1f s" % (args.scale, z, m, time.time() - t0))
+    t0 = time.time()
+    dg = capi.DeviceGraph(z, g["colptr"], g["src"], g["weight"], device=True)
+    info = dg.info()
+    n = info["n"]
+    log("device graph: n=%d m=%d sinks=%d max_in_degree=%d items=%d prep %.0f ms (wall %.1f s)"
+        % (n, info["m"], info["sinks"], info["max_in_degree"], info["work_items"],
+           info["prep_ms"], time.time() - t0))
+    stream = torch.cuda.ExternalStream(dg.stream())
+    x = torch.empty(z, dtype=torch.float64, device=dev)
+
+    def solve(precision, tol, timing=1):
+        with torch.cuda.stream(stream):
+            x.fill_(1.0 / z)
+        p = capi.default_params(scores_on_device=1, precision=precision, tol=tol,
+                                stop=capi.NKPR_STOP_L1_ERROR, timing=timing)
+        return dg.run(x, p)
+
+    # tightly converged FP64 reference (not timed)
+    st = solve(capi.NKPR_FP64, 1e-13)
+    ref = x.clone()
+    log("reference: %d FP64 sweeps, certified L1 bound %.2e" % (st["iterations"],
+                                                              st["l1_error_bound"]))
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            solve(precision, args.tol)
+        torch.cuda.synchronize()
+        stats = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            stats.append(solve(precision, args.tol))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps, stats
+
+    sec_mixed, st_mixed = timed(capi.NKPR_MIXED, args.steps, args.warmup)
+    err_mixed = float((x - ref).abs().sum())
+    x_mixed = x.clone()
+    sec_fp64, st_fp64 = timed(capi.NKPR_FP64, args.steps, args.warmup)
+    err_fp64 = float((x - ref).abs().sum())
+    # top-1000 ranking agreement of the mixed result with the reference
+    top_ref = torch.argsort(ref, descending=True, stable=True)[:1000]
+    top_mix = torch.argsort(x_mixed, descending=True, stable=True)[:1000]
+    mism = (top_ref != top_mix).nonzero().flatten()
+    tie_ok = bool(all(abs(float(ref[top_ref[i]] - ref[top_mix[i]])) <= 1e-12 for i in mism))
+
+    s0 = st_mixed[-1]
+    sweeps_mixed = s0["sweeps_fp32"] + s0["sweeps_fp64"]
+    sweeps_fp64 = st_fp64[-1]["sweeps_fp64"]
+    value = m * sweeps_mixed / sec_mixed
+    # roofline of the dominant kernel: the FP32 sweep, timed with HIP events on its stream
+    n_corr = sum(s["sweeps_fp32_correction"] for s in st_mixed)
+    n_f32 = sum(s["sweeps_fp32"] for s in st_mixed)
+    ms_f32 = sum(s["ms_sweeps_fp32"] for s in st_mixed)
+    bytes_f32 = ((n_f32 - n_corr) * algorithmic_bytes(m, n, True, "fp32") +
+                 n_corr * algorithmic_bytes(m, n, True, "fp32corr"))
+    achieved = bytes_f32 / (ms_f32 / 1e3) / 1e9
+    ms64 = sum(s["ms_sweeps_fp64"] for s in st_fp64)
+    n64 = sum(s["sweeps_fp64"] for s in st_fp64)
+    achieved64 = n64 * (20 * m + 24 * n) / (ms64 / 1e3) / 1e9
+    log("mixed: %.1f ms/solve, %d fp32 (%d corr) + %d fp64 sweeps, L1 err %.2e; "
+        "fp32 sweep %.3f ms = %.0f GB/s" % (sec_mixed * 1e3, s0["sweeps_fp32"],
+                                            s0["sweeps_fp32_correction"], s0["sweeps_fp64"],
+                                            err_mixed, ms_f32 / n_f32, achieved))
+    log("fp64: %.1f ms/solve, %d sweeps, L1 err %.2e; fp64 sweep %.3f ms = %.0f GB/s" % (
+        sec_fp64 * 1e3, sweeps_fp64, err_fp64, ms64 / n64, achieved64))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(g, z, m, args.cpu_seconds)
+    out = {
+        "metric": "PageRank edge-updates/s & %HBM roofline, mixed vs FP64 at equal L1 error",
+        "value": value,
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "PageRank solve to certified L1 error <= %g (damp 0.85, no sink "
+                        "handling) of a directed R-MAT graph, scale %d, edge factor %d, "
+                        "a/b/c/d=.57/.19/.19/.05, duplicates and self-loops removed, "
+                        "FP32 weights U(0.5,1.5), seed 2" % (args.tol, args.scale,
+                                                              args.edge_factor),
+            "n": z, "m": m, "sinks": info["sinks"], "parallelism": "single GPU",
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "FP32 pull sweep", "bytes_per_sweep": algorithmic_bytes(
+                         m, n, True, "fp32"),
+                     "ms_per_sweep": ms_f32 / n_f32},
+        "cpu_baseline": cpu,
+        "mixed": {"ms_per_solve": sec_mixed * 1e3, "sweeps_fp32": s0["sweeps_fp32"],
+                  "sweeps_fp32_correction": s0["sweeps_fp32_correction"],
+                  "sweeps_fp64": s0["sweeps_fp64"], "l1_error": err_mixed,
+                  "l1_error_bound": s0["l1_error_bound"], "top1000_same": tie_ok},
+        "fp64": {"ms_per_solve": sec_fp64 * 1e3, "sweeps": sweeps_fp64, "l1_error": err_fp64,
+                 "edge_updates_per_s": m * sweeps_fp64 / sec_fp64,
+                 "ms_per_sweep": ms64 / n64, "achieved_GBs": achieved64},
+        "speedup_vs_fp64": sec_fp64 / sec_mixed,
+        "prep_ms": info["prep_ms"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(g, z, m, seconds):
+    """The oracle (a restatement of NetworKit's PageRank loop with NetworKit's element types:
+    uint64 ids, FP64 weights, FP64 scores, guided OpenMP schedule) timed on the host cores on a
+    bounded sample: whole reference sweeps over the same graph until ~`seconds` of CPU work."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    t0 = time.time()
+    colptr = g["colptr"].cpu().numpy().astype(np.uint64)
+    src = g["src"].cpu().numpy().astype(np.uint64)
+    w = g["weight"].cpu().numpy().astype(np.float64)
+    og = oracle.InLists(z, colptr, src, w)
+    log("cpu baseline: graph on host in %.1f s, %d threads" % (time.time() - t0, threads))
+    t1 = oracle.time_sweeps(og, 0.85, 1, threads)
+    sweeps = max(1, min(20, int(seconds / max(t1, 1e-3))))
+    if sweeps > 1:
+        tt = oracle.time_sweeps(og, 0.85, sweeps, threads)
+    else:
+        tt = t1
+    log("cpu baseline: %d sweeps in %.2f s" % (sweeps, tt))
+    return {"value": m * sweeps / tt, "unit": "edge-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d FP64 reference sweeps (NetworKit element types) over the full graph "
+                      "on %d host threads; %.2f s" % (sweeps, threads, tt)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * nkpr.h -- C ABI of the MI355X PageRank library (libnkpr.so).
+ *
+ * This is the boundary a NetworKit build would bind instead of running the OpenMP loop in
+ * networkit/cpp/centrality/PageRank.cpp.  Every entry point takes plain pointers and sizes,
+ * returns an int status (NKPR_OK = 0) and never throws; the message of the last failure on
+ * the calling thread is available from nkpr_last_error().
+ *
+ * Which reference interface each entry point replaces (paths in kit-parco/networkit v9.0):
+ *   nkpr_graph_create    the Graph state PageRank::run reads: in-lists (Graph.hpp:1918-1938),
+ *                        exists[] (Graph.hpp:1802-1809), weightedDegree (Graph.cpp:427-455)
+ *   nkpr_run             PageRank::run (PageRank.cpp:20-104) incl. numberOfIterations
+ *                        (PageRank.hpp:58-61) and maximum() (PageRank.cpp:95-108)
+ *   nkpr_ranking         Centrality::ranking (Centrality.cpp:29-41)
+ *   nkpr_params_default  the constructor defaults PageRank(G, 0.85, 1e-8, false), the field
+ *                        defaults maxIterations = UINT64_MAX, norm = L2Norm (PageRank.hpp:40-67)
+ *
+ * Device placement: a graph lives on the HIP device that was current when it was created and
+ * owns one HIP stream (default priority); all of its work is ordered on that stream.  Calls
+ * on one graph handle must not overlap (the reference does not support concurrent run() on
+ * one object either).  Distinct handles may be used from distinct threads.
+ */
+#ifndef NKPR_H
+#define NKPR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NKPR_ABI_VERSION 1
+
+/* status codes */
+#define NKPR_OK 0
+#define NKPR_ERR_INVALID 1     /* bad argument or unsupported input                      */
+#define NKPR_ERR_HIP 2         /* a HIP runtime call failed (message has the HIP error)   */
+#define NKPR_ERR_NOMEM 3       /* device or host allocation failed                        */
+#define NKPR_ERR_INTERRUPTED 4 /* *interrupt became nonzero (Aux::SignalHandler analogue) */
+#define NKPR_ERR_NODEVICE 5    /* no usable gfx950 device                                 */
+
+/* PageRank::Norm (PageRank.hpp:31): L1Norm = 0, L2Norm = 1 */
+#define NKPR_L1 0
+#define NKPR_L2 1
+
+/* sink handling.  NONE is the reference behaviour: rank sent to a node without out-edges is
+ * lost every sweep and only the final normalisation rescales it (PageRank.cpp:56-92).
+ * DISTRIBUTE adds damp * (sum of sink scores) / n to every node each sweep (Google matrix). */
+#define NKPR_SINK_NONE 0
+#define NKPR_SINK_DISTRIBUTE 1
+
+/* arithmetic of the sweeps.
+ * FP64  every sweep in FP64 (folded FP64 edge coefficients); with NKPR_STOP_STEP it follows
+ *       the reference loop exactly (same stopping rule and iteration count).
+ * MIXED FP32 sweeps until the step stalls, then FP64 residual sweeps with FP32 correction
+ *       solves (iterative refinement) until the certified L1 error of the sum-normalised
+ *       scores is <= tol.  Always uses NKPR_STOP_L1_ERROR.
+ * FP32  every sweep in FP32 (stops at the FP32 noise floor if tol is below it). */
+#define NKPR_FP64 0
+#define NKPR_MIXED 1
+#define NKPR_FP32 2
+
+/* stopping rule.
+ * STEP      the reference rule: stop when ||x' - x|| <= tol in `norm` (PageRank.cpp:34-51)
+ *           or the iteration count reaches max_iterations.
+ * L1_ERROR  stop when the certified bound on the L1 error of the sum-normalised scores,
+ *           2 * d/(1-d) * ||x' - x||_1 / sum(x'), is <= tol. */
+#define NKPR_STOP_STEP 0
+#define NKPR_STOP_L1_ERROR 1
+
+typedef struct nkpr_graph nkpr_graph;
+
+/* In-lists of a graph in compressed form (CSC of the pull operator).  Row u lists the sources
+ * of u's in-edges: for a directed graph the inEdges[u] of NetworKit, for an undirected graph
+ * its adjacency outEdges[u] (both endpoints list an edge, a self-loop is listed once). */
+typedef struct {
+    uint64_t z;               /* node-id bound (Graph::upperNodeIdBound); z < 2^31        */
+    const uint64_t *colptr;   /* [z+1] offsets into src/weight; colptr[z] = m < 2^32     */
+    const int32_t *src;       /* [m] source node of each in-edge                           */
+    const float *weight;      /* [m] edge weights, or NULL: unweighted (every weight 1.0)  */
+    const double *weight64;   /* [m] optional exact FP64 weights (NetworKit edgeweight); when
+                                 given they define the FP64 coefficients, `weight` must then
+                                 hold the same values rounded to FP32                      */
+    const double *out_wdeg;   /* [z] weighted out-degree per node, or NULL: summed from the
+                                 lists (Graph::weightedDegree(u) semantics)               */
+    const uint8_t *exists;    /* [z] 1 if the slot holds a node, or NULL: all exist        */
+    int32_t device_ptrs;      /* 1: the arrays are device pointers on the current device  */
+} nkpr_csc;
+
+typedef struct {
+    uint64_t z;               /* node-id bound                                             */
+    uint64_t n;               /* existing nodes (Graph::numberOfNodes)                     */
+    uint64_t m;               /* in-list entries                                           */
+    uint64_t sinks;           /* existing nodes without out-edges                          */
+    uint64_t max_in_degree;
+    uint64_t work_items;      /* scheduled row blocks + split-row chunks per sweep         */
+    uint64_t device_bytes;    /* device memory held by the handle                          */
+    int32_t weighted;
+    int32_t device;           /* HIP device ordinal                                        */
+    double prep_ms;           /* wall time of nkpr_graph_create                            */
+} nkpr_graph_info;
+
+typedef struct {
+    double damp;              /* damping factor d (default 0.85)                           */
+    double tol;               /* tolerance (default 1e-8), meaning set by `stop`           */
+    uint64_t max_iterations;  /* default UINT64_MAX                                        */
+    int32_t norm;             /* NKPR_L1 / NKPR_L2 (default L2)                            */
+    int32_t sink_handling;    /* NKPR_SINK_* (default NONE)                                */
+    int32_t normalized;       /* PageRank(..., normalized) (default 0)                     */
+    int32_t precision;        /* NKPR_FP64 / NKPR_MIXED / NKPR_FP32 (default FP64)         */
+    int32_t stop;             /* NKPR_STOP_* (default STEP; MIXED always uses L1_ERROR)    */
+    int32_t scores_on_device; /* 1: `scores` of nkpr_run is a device pointer               */
+    int32_t timing;           /* 1: time every sweep kernel with HIP events                */
+    int32_t reserved;
+    const volatile int32_t *interrupt; /* optional; checked between batches of sweeps      */
+} nkpr_params;
+
+typedef struct {
+    uint64_t iterations;      /* PageRank::numberOfIterations(): all sweeps of the run     */
+    uint64_t sweeps_fp32;
+    uint64_t sweeps_fp64;
+    uint64_t refinements;     /* FP64 residual sweeps followed by an FP32 correction solve */
+    uint64_t sweeps_fp32_correction; /* FP32 sweeps of correction solves (part of sweeps_fp32) */
+    double last_step;         /* ||x' - x|| of the last sweep in params->norm              */
+    double last_step_l1;
+    double l1_error_bound;    /* certified bound on the L1 error of the normalised scores  */
+    double maximum;           /* PageRank::maximum()                                       */
+    double score_sum;         /* sum of the unnormalised scores before normalisation       */
+    double ms_sweeps_fp32;    /* summed sweep-kernel time (params->timing = 1)             */
+    double ms_sweeps_fp64;
+    double ms_total;          /* wall time of nkpr_run                                     */
+} nkpr_stats;
+
+/* Fills the reference defaults. */
+int nkpr_params_default(nkpr_params *p);
+
+/* Uploads (or, with device_ptrs, copies on the device) and preprocesses a graph: relabels the
+ * existing nodes by out-degree (sinks last), sorts every in-list by the new source id, folds
+ * w / deg(src) into the edge coefficients and builds the sweep schedule. */
+int nkpr_graph_create(const nkpr_csc *csc, nkpr_graph **out);
+int nkpr_graph_destroy(nkpr_graph *g);
+int nkpr_graph_get_info(const nkpr_graph *g, nkpr_graph_info *info);
+/* The graph's HIP stream (hipStream_t), e.g. to order caller work or events with it. */
+void *nkpr_graph_stream(const nkpr_graph *g);
+
+/* PageRank::run.  scores[z] holds the start vector on entry (a fresh reference object starts
+ * at 1/n everywhere; a re-run starts from the previous scores, PageRank.cpp:26) and the
+ * scores on return; slots that hold no node are left untouched.  stats may be NULL. */
+int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats);
+
+/* Centrality::ranking: ids of the existing nodes ordered by score descending, ties by id
+ * ascending; ids_out and scores_out (nullable) receive *count = number of existing nodes
+ * entries.  All arrays are host pointers. */
+int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_t *ids_out,
+                 double *scores_out, uint64_t *count);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nkpr_last_error(void);
+
+int nkpr_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NKPR_H */

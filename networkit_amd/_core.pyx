@@ -1,0 +1,356 @@
+# distutils: language = c++
+# cython: language_level=3, boundscheck=False, wraparound=False
+"""Cython bindings of the MI355X PageRank drop-in.
+
+Mirrors the reference's Python surface for this path:
+  networkit.graph.Graph            (networkit/graph.pyx:29-547, subset)
+  networkit.base.Algorithm         (networkit/base.pyx:3-69)
+  networkit.centrality.Centrality  (networkit/centrality.pyx:24-100)
+  networkit.centrality.PageRank    (networkit/centrality.pyx:1989-2037) and Norm (:1970-1979)
+run() releases the GIL around the C++ call, as base.pyx:28-29 does.
+"""
+from libc.stdint cimport uint64_t, int64_t, int32_t
+from libcpp cimport bool as bool_t
+from libcpp.vector cimport vector
+from libcpp.utility cimport pair
+from libcpp.string cimport string
+
+import numpy as np
+cimport numpy as cnp
+
+cnp.import_array()
+
+ctypedef uint64_t node
+ctypedef uint64_t count
+ctypedef double edgeweight
+
+cdef extern from "nkpr.h":
+    ctypedef struct nkpr_stats:
+        uint64_t iterations
+        uint64_t sweeps_fp32
+        uint64_t sweeps_fp64
+        uint64_t refinements
+        uint64_t sweeps_fp32_correction
+        double last_step
+        double last_step_l1
+        double l1_error_bound
+        double maximum
+        double score_sum
+        double ms_sweeps_fp32
+        double ms_sweeps_fp64
+        double ms_total
+    ctypedef struct nkpr_graph_info:
+        uint64_t z
+        uint64_t n
+        uint64_t m
+        uint64_t sinks
+        uint64_t max_in_degree
+        uint64_t work_items
+        uint64_t device_bytes
+        int32_t weighted
+        int32_t device
+        double prep_ms
+
+cdef extern from "Graph.hpp" namespace "NetworKit":
+    cdef cppclass _Graph "NetworKit::Graph":
+        _Graph(count, bool_t, bool_t) except +
+        node addNode() except +
+        node addNodes(count) except +
+        void removeNode(node) except +
+        void restoreNode(node) except +
+        bool_t hasNode(node)
+        void addEdge(node, node, edgeweight) except +
+        bool_t hasEdge(node, node)
+        void removeEdge(node, node) except +
+        edgeweight weight(node, node) except +
+        count numberOfNodes()
+        count numberOfEdges()
+        count upperNodeIdBound()
+        count numberOfSelfLoops()
+        bool_t isWeighted()
+        bool_t isDirected()
+        count degree(node) except +
+        count degreeIn(node) except +
+        count degreeOut(node) except +
+        edgeweight weightedDegree(node, bool_t) except +
+
+cdef extern from "PageRank.hpp" namespace "NetworKit":
+    cdef cppclass _Algorithm "NetworKit::Algorithm":
+        void run() except + nogil
+        bool_t hasFinished()
+    cdef cppclass _Centrality "NetworKit::Centrality"(_Algorithm):
+        vector[double] scores() except +
+        vector[pair[node, double]] ranking() except +
+        double score(node) except +
+        double maximum() except +
+        double centralization() except +
+    cdef enum _Norm "NetworKit::PageRank::Norm":
+        _L1Norm "NetworKit::PageRank::L1Norm"
+        _L2Norm "NetworKit::PageRank::L2Norm"
+    cdef enum _SinkHandling "NetworKit::PageRank::SinkHandling":
+        _NoSinkHandling "NetworKit::PageRank::NoSinkHandling"
+        _DistributeSinks "NetworKit::PageRank::DistributeSinks"
+    cdef enum _Precision "NetworKit::PageRank::Precision":
+        _FP64 "NetworKit::PageRank::FP64"
+        _Mixed "NetworKit::PageRank::Mixed"
+        _FP32 "NetworKit::PageRank::FP32"
+    cdef cppclass _PageRank "NetworKit::PageRank"(_Centrality):
+        _PageRank(_Graph, double, double, bool_t, _SinkHandling) except +
+        count numberOfIterations() except +
+        count maxIterations
+        _Norm norm
+        _Precision precision
+        bool_t certifiedStop
+        bool_t timeSweeps
+        const nkpr_stats& lastStats()
+        const nkpr_graph_info& deviceGraphInfo() except +
+
+
+cdef class Graph:
+    """Graph(n=0, weighted=False, directed=False): NetworKit-compatible graph (subset)."""
+    cdef _Graph* _this
+
+    def __cinit__(self, n=0, bool_t weighted=False, bool_t directed=False):
+        self._this = new _Graph(n, weighted, directed)
+
+    def __dealloc__(self):
+        if self._this != NULL:
+            del self._this
+            self._this = NULL
+
+    def addNode(self):
+        return self._this.addNode()
+
+    def addNodes(self, count k):
+        return self._this.addNodes(k)
+
+    def removeNode(self, node u):
+        self._this.removeNode(u)
+
+    def restoreNode(self, node u):
+        self._this.restoreNode(u)
+
+    def hasNode(self, node u):
+        return self._this.hasNode(u)
+
+    def addEdge(self, node u, node v, edgeweight w=1.0):
+        self._this.addEdge(u, v, w)
+        return self
+
+    def addEdges(self, u, v, w=None):
+        """Bulk addEdge(u[i], v[i], w[i]) in order (arrays of equal length)."""
+        cdef cnp.ndarray[cnp.uint64_t, ndim=1] uu = np.ascontiguousarray(u, dtype=np.uint64)
+        cdef cnp.ndarray[cnp.uint64_t, ndim=1] vv = np.ascontiguousarray(v, dtype=np.uint64)
+        cdef cnp.ndarray[cnp.float64_t, ndim=1] ww
+        cdef Py_ssize_t i, k = uu.shape[0]
+        if vv.shape[0] != k:
+            raise ValueError("u and v differ in length")
+        if w is None:
+            for i in range(k):
+                self._this.addEdge(uu[i], vv[i], 1.0)
+        else:
+            ww = np.ascontiguousarray(w, dtype=np.float64)
+            if ww.shape[0] != k:
+                raise ValueError("w differs in length")
+            for i in range(k):
+                self._this.addEdge(uu[i], vv[i], ww[i])
+        return self
+
+    def hasEdge(self, node u, node v):
+        return self._this.hasEdge(u, v)
+
+    def removeEdge(self, node u, node v):
+        self._this.removeEdge(u, v)
+        return self
+
+    def weight(self, node u, node v):
+        return self._this.weight(u, v)
+
+    def numberOfNodes(self):
+        return self._this.numberOfNodes()
+
+    def numberOfEdges(self):
+        return self._this.numberOfEdges()
+
+    def upperNodeIdBound(self):
+        return self._this.upperNodeIdBound()
+
+    def numberOfSelfLoops(self):
+        return self._this.numberOfSelfLoops()
+
+    def isWeighted(self):
+        return self._this.isWeighted()
+
+    def isDirected(self):
+        return self._this.isDirected()
+
+    def degree(self, node u):
+        return self._this.degree(u)
+
+    def degreeIn(self, node u):
+        return self._this.degreeIn(u)
+
+    def degreeOut(self, node u):
+        return self._this.degreeOut(u)
+
+    def weightedDegree(self, node u, bool_t countSelfLoopsTwice=False):
+        return self._this.weightedDegree(u, countSelfLoopsTwice)
+
+
+cdef class Algorithm:
+    """Abstract base class for algorithms (networkit/base.pyx:3-69)."""
+    cdef _Algorithm* _this
+
+    def __init__(self, *args, **namedargs):
+        if type(self) == Algorithm:
+            raise RuntimeError("Error, you may not use Algorithm directly, use a sub-class instead")
+
+    def __cinit__(self, *args, **namedargs):
+        self._this = NULL
+
+    def __dealloc__(self):
+        if self._this != NULL:
+            del self._this
+        self._this = NULL
+
+    def run(self):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        with nogil:
+            self._this.run()
+        return self
+
+    def hasFinished(self):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        return self._this.hasFinished()
+
+
+cdef class Centrality(Algorithm):
+    """Abstract base class for centrality measures (networkit/centrality.pyx:24-100)."""
+    cdef Graph _G
+
+    def __init__(self, *args, **kwargs):
+        if type(self) == Centrality:
+            raise RuntimeError("Error, you may not use Centrality directly, use a sub-class instead")
+
+    def __dealloc__(self):
+        self._G = None
+
+    def scores(self):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        return (<_Centrality*>(self._this)).scores()
+
+    def score(self, v):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        return (<_Centrality*>(self._this)).score(v)
+
+    def ranking(self):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        return (<_Centrality*>(self._this)).ranking()
+
+    def maximum(self):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        return (<_Centrality*>(self._this)).maximum()
+
+    def centralization(self):
+        if self._this == NULL:
+            raise RuntimeError("Error, object not properly initialized")
+        return (<_Centrality*>(self._this)).centralization()
+
+
+class Norm(object):
+    """Stopping-criterion norm (PageRank::Norm)."""
+    l1norm = 0
+    l2norm = 1
+    L1Norm = 0
+    L2Norm = 1
+
+
+class SinkHandling(object):
+    """What happens to the rank of nodes without out-edges."""
+    NoSinkHandling = 0
+    DistributeSinks = 1
+
+
+class Precision(object):
+    """Arithmetic of the device sweeps."""
+    FP64 = 0
+    Mixed = 1
+    FP32 = 2
+
+
+cdef class PageRank(Centrality):
+    """PageRank(G, damp=0.85, tol=1e-8, normalized=False, distributeSinks=SinkHandling.NoSinkHandling)
+
+    Compute PageRank as node centrality measure on the MI355X.  Parameters and results follow
+    networkit.centrality.PageRank.  Extra properties: `precision` (Precision.FP64 by default,
+    Precision.Mixed for FP32 sweeps with FP64 refinement to a certified L1 error <= tol),
+    `certifiedStop`, `timeSweeps` and `stats()`.
+    """
+
+    def __cinit__(self, Graph G, double damp=0.85, double tol=1e-8, bool_t normalized=False,
+                  distributeSinks=SinkHandling.NoSinkHandling):
+        self._G = G
+        self._this = new _PageRank(G._this[0], damp, tol, normalized,
+                                   <_SinkHandling>(<int>distributeSinks))
+
+    def numberOfIterations(self):
+        return (<_PageRank*>(self._this)).numberOfIterations()
+
+    property norm:
+        def __get__(self):
+            return <int>(<_PageRank*>(self._this)).norm
+        def __set__(self, norm):
+            if int(norm) not in (0, 1):
+                raise ValueError("norm must be Norm.l1norm or Norm.l2norm")
+            (<_PageRank*>(self._this)).norm = <_Norm>(<int>norm)
+
+    property maxIterations:
+        def __get__(self):
+            return (<_PageRank*>(self._this)).maxIterations
+        def __set__(self, maxIterations):
+            if maxIterations < 0:
+                raise Exception("Max iterations cannot be a negative number.")
+            (<_PageRank*>(self._this)).maxIterations = maxIterations
+
+    property precision:
+        def __get__(self):
+            return <int>(<_PageRank*>(self._this)).precision
+        def __set__(self, p):
+            if int(p) not in (0, 1, 2):
+                raise ValueError("precision must be Precision.FP64, .Mixed or .FP32")
+            (<_PageRank*>(self._this)).precision = <_Precision>(<int>p)
+
+    property certifiedStop:
+        def __get__(self):
+            return (<_PageRank*>(self._this)).certifiedStop
+        def __set__(self, bool_t v):
+            (<_PageRank*>(self._this)).certifiedStop = v
+
+    property timeSweeps:
+        def __get__(self):
+            return (<_PageRank*>(self._this)).timeSweeps
+        def __set__(self, bool_t v):
+            (<_PageRank*>(self._this)).timeSweeps = v
+
+    def stats(self):
+        """Statistics of the last run (sweeps per precision, error bound, timings)."""
+        cdef nkpr_stats s = (<_PageRank*>(self._this)).lastStats()
+        return dict(iterations=s.iterations, sweeps_fp32=s.sweeps_fp32,
+                    sweeps_fp64=s.sweeps_fp64, refinements=s.refinements,
+                    sweeps_fp32_correction=s.sweeps_fp32_correction,
+                    last_step=s.last_step, last_step_l1=s.last_step_l1,
+                    l1_error_bound=s.l1_error_bound, maximum=s.maximum,
+                    score_sum=s.score_sum, ms_sweeps_fp32=s.ms_sweeps_fp32,
+                    ms_sweeps_fp64=s.ms_sweeps_fp64, ms_total=s.ms_total)
+
+    def deviceGraphInfo(self):
+        cdef nkpr_graph_info i = (<_PageRank*>(self._this)).deviceGraphInfo()
+        return dict(z=i.z, n=i.n, m=i.m, sinks=i.sinks, max_in_degree=i.max_in_degree,
+                    work_items=i.work_items, device_bytes=i.device_bytes,
+                    weighted=bool(i.weighted), device=i.device, prep_ms=i.prep_ms)

@@ -1,0 +1,196 @@
+"""ctypes binding of the C ABI in include/nkpr.h (libnkpr.so).
+
+This is the thinnest Python route to the device library: tests use it to call every entry
+point directly, and the benchmark uses it to hand device-resident arrays (torch tensors) to
+nkpr_graph_create / nkpr_run without a host round trip.  The Cython module `_core` is the
+NetworKit-style API on top of the C++ drop-in; both end in the same library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnkpr.so")
+
+NKPR_OK = 0
+NKPR_ERR_INVALID = 1
+NKPR_ERR_HIP = 2
+NKPR_ERR_NOMEM = 3
+NKPR_ERR_INTERRUPTED = 4
+NKPR_ERR_NODEVICE = 5
+NKPR_L1, NKPR_L2 = 0, 1
+NKPR_SINK_NONE, NKPR_SINK_DISTRIBUTE = 0, 1
+NKPR_FP64, NKPR_MIXED, NKPR_FP32 = 0, 1, 2
+NKPR_STOP_STEP, NKPR_STOP_L1_ERROR = 0, 1
+
+# every symbol include/nkpr.h declares
+EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_destroy",
+            "nkpr_graph_get_info", "nkpr_graph_stream", "nkpr_run", "nkpr_ranking",
+            "nkpr_last_error", "nkpr_abi_version"]
+
+
+class Csc(ctypes.Structure):
+    _fields_ = [("z", ctypes.c_uint64), ("colptr", ctypes.c_void_p), ("src", ctypes.c_void_p),
+                ("weight", ctypes.c_void_p), ("weight64", ctypes.c_void_p),
+                ("out_wdeg", ctypes.c_void_p), ("exists", ctypes.c_void_p),
+                ("device_ptrs", ctypes.c_int32)]
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [("z", ctypes.c_uint64), ("n", ctypes.c_uint64), ("m", ctypes.c_uint64),
+                ("sinks", ctypes.c_uint64), ("max_in_degree", ctypes.c_uint64),
+                ("work_items", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("weighted", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("prep_ms", ctypes.c_double)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("damp", ctypes.c_double), ("tol", ctypes.c_double),
+                ("max_iterations", ctypes.c_uint64), ("norm", ctypes.c_int32),
+                ("sink_handling", ctypes.c_int32), ("normalized", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("stop", ctypes.c_int32),
+                ("scores_on_device", ctypes.c_int32), ("timing", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("interrupt", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("sweeps_fp32", ctypes.c_uint64),
+                ("sweeps_fp64", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
+                ("sweeps_fp32_correction", ctypes.c_uint64), ("last_step", ctypes.c_double), ("last_step_l1", ctypes.c_double),
+                ("l1_error_bound", ctypes.c_double), ("maximum", ctypes.c_double),
+                ("score_sum", ctypes.c_double), ("ms_sweeps_fp32", ctypes.c_double),
+                ("ms_sweeps_fp64", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libnkpr.so (raises OSError if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libnkpr.so is not built: run __graft_entry__.build()")
+    from ._runtime import bind_hip_runtime
+    bind_hip_runtime()
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.nkpr_params_default.argtypes = [ctypes.POINTER(Params)]
+    L.nkpr_graph_create.argtypes = [ctypes.POINTER(Csc), ctypes.POINTER(vp)]
+    L.nkpr_graph_destroy.argtypes = [vp]
+    L.nkpr_graph_get_info.argtypes = [vp, ctypes.POINTER(GraphInfo)]
+    L.nkpr_graph_stream.argtypes = [vp]
+    L.nkpr_graph_stream.restype = vp
+    L.nkpr_run.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.POINTER(Stats)]
+    L.nkpr_ranking.argtypes = [ctypes.c_uint64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.nkpr_last_error.restype = ctypes.c_char_p
+    L.nkpr_abi_version.restype = ctypes.c_int
+    for name in EXPORTED:
+        if name not in ("nkpr_graph_stream", "nkpr_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+class NkprError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nkpr error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != NKPR_OK:
+        raise NkprError(rc, lib().nkpr_last_error().decode())
+
+
+def default_params(**kw):
+    p = Params()
+    check(lib().nkpr_params_default(ctypes.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _addr(a):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+class DeviceGraph:
+    """RAII wrapper of an nkpr_graph handle.
+
+    Host arrays (numpy) are uploaded; device arrays (torch CUDA tensors) are read in place
+    (set ``device=True``).  The arrays only have to live until the constructor returns."""
+
+    def __init__(self, z, colptr, src, weight=None, out_wdeg=None, exists=None, weight64=None,
+                 device=False):
+        self._keep = (colptr, src, weight, out_wdeg, exists, weight64)
+        if not device:
+            colptr = np.ascontiguousarray(colptr, dtype=np.uint64)
+            src = np.ascontiguousarray(src, dtype=np.int32)
+            weight = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+            weight64 = None if weight64 is None else np.ascontiguousarray(weight64, dtype=np.float64)
+            out_wdeg = None if out_wdeg is None else np.ascontiguousarray(out_wdeg, dtype=np.float64)
+            exists = None if exists is None else np.ascontiguousarray(exists, dtype=np.uint8)
+            self._keep = (colptr, src, weight, out_wdeg, exists, weight64)
+        c = Csc(z=int(z), colptr=_addr(colptr), src=_addr(src), weight=_addr(weight),
+                weight64=_addr(weight64), out_wdeg=_addr(out_wdeg), exists=_addr(exists),
+                device_ptrs=1 if device else 0)
+        h = ctypes.c_void_p()
+        check(lib().nkpr_graph_create(ctypes.byref(c), ctypes.byref(h)))
+        self.handle = h
+        self.z = int(z)
+        self._keep = None
+
+    def info(self):
+        i = GraphInfo()
+        check(lib().nkpr_graph_get_info(self.handle, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def stream(self):
+        return lib().nkpr_graph_stream(self.handle)
+
+    def run(self, scores, params=None, **kw):
+        """nkpr_run on ``scores`` (numpy float64[z] on the host, updated in place, or a torch
+        float64 CUDA tensor with params.scores_on_device = 1).  Returns the stats dict."""
+        p = params if params is not None else default_params(**kw)
+        if isinstance(scores, np.ndarray):
+            assert scores.dtype == np.float64 and scores.flags.c_contiguous
+            assert len(scores) == self.z
+        st = Stats()
+        check(lib().nkpr_run(self.handle, ctypes.byref(p), _addr(scores), ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().nkpr_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ranking(scores, exists=None):
+    scores = np.ascontiguousarray(scores, dtype=np.float64)
+    z = len(scores)
+    ids = np.zeros(max(z, 1), dtype=np.int64)
+    vals = np.zeros(max(z, 1), dtype=np.float64)
+    k = ctypes.c_uint64(0)
+    ex = None if exists is None else np.ascontiguousarray(exists, dtype=np.uint8)
+    check(lib().nkpr_ranking(z, _addr(scores), _addr(ex), _addr(ids), _addr(vals),
+                             ctypes.byref(k)))
+    return ids[:k.value], vals[:k.value]

@@ -1,0 +1,76 @@
+// NetworKit-compatible Graph (the subset PageRank needs), written for this library.
+//
+// Storage follows NetworKit's semantics (include/networkit/graph/Graph.hpp:97-146): per-node
+// adjacency vectors, weights in the same order, an exists[] flag per slot.  Directed graphs
+// keep out- and in-lists; undirected graphs keep one list per node in which an edge appears
+// at both endpoints and a self-loop once (networkit/cpp/graph/Graph.cpp:549-597).  Multi-
+// edges are kept.  exportInLists() flattens the lists PageRank pulls over
+// (Graph::forInEdgesOf, Graph.hpp:1918-1938) into the CSC the device library consumes.
+#pragma once
+
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+namespace NetworKit {
+
+using node = uint64_t;
+using index = uint64_t;
+using count = uint64_t;
+using edgeweight = double;
+constexpr edgeweight defaultEdgeWeight = 1.0;
+constexpr node none = std::numeric_limits<node>::max();
+
+class Graph {
+public:
+    explicit Graph(count n = 0, bool weighted = false, bool directed = false);
+
+    node addNode();
+    node addNodes(count k);                    // returns the last new id
+    void removeNode(node v);                   // removes incident edges first (Graph.cpp:514-527)
+    void restoreNode(node v);
+    bool hasNode(node v) const noexcept { return v < z && exists[v]; }
+
+    void addEdge(node u, node v, edgeweight ew = defaultEdgeWeight);
+    bool hasEdge(node u, node v) const noexcept;
+    void removeEdge(node u, node v);           // removes one occurrence
+    edgeweight weight(node u, node v) const;
+
+    count numberOfNodes() const noexcept { return n; }
+    count numberOfEdges() const noexcept { return m; }
+    count upperNodeIdBound() const noexcept { return z; }
+    count numberOfSelfLoops() const noexcept { return selfLoops; }
+    bool isWeighted() const noexcept { return weighted; }
+    bool isDirected() const noexcept { return directed; }
+
+    count degree(node v) const { return outEdges[v].size(); }
+    count degreeOut(node v) const { return outEdges[v].size(); }
+    count degreeIn(node v) const { return directed ? inEdges[v].size() : outEdges[v].size(); }
+    // Graph::computeWeightedDegree(u, inDegree = false, countSelfLoopsTwice), Graph.cpp:427-455
+    edgeweight weightedDegree(node u, bool countSelfLoopsTwice = false) const;
+
+    template <typename L> void forNodes(L handle) const {
+        for (node v = 0; v < z; ++v)
+            if (exists[v]) handle(v);
+    }
+
+    // Flattened pull lists: row u holds the in-neighbours of u (directed) or the adjacency of
+    // u (undirected), in insertion order.  Weights are filled only for weighted graphs.
+    void exportInLists(std::vector<uint64_t> &colptr, std::vector<int32_t> &src,
+                       std::vector<float> *w32, std::vector<double> *w64) const;
+    const std::vector<bool> &existsFlags() const noexcept { return exists; }
+
+    // Increments on every structural or weight change; consumers cache device copies by it.
+    uint64_t version() const noexcept { return ver; }
+
+private:
+    count n, m, z, selfLoops;
+    bool weighted, directed;
+    uint64_t ver = 0;
+    std::vector<bool> exists;
+    std::vector<std::vector<node>> outEdges, inEdges;
+    std::vector<std::vector<edgeweight>> outEdgeWeights, inEdgeWeights;
+};
+
+}  // namespace NetworKit

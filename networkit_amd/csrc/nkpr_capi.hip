@@ -1,0 +1,508 @@
+// C ABI of libnkpr.so and the host orchestration of a PageRank run.
+//
+// nkpr_run restates the control flow of networkit/cpp/centrality/PageRank.cpp:20-104 on the
+// device: start vector (:22-27, warm start included), sweeps until the stopping rule holds
+// (:53-71), normalisation (:73-92) and maximum() (:95-102).  The sweeps are issued in
+// batches; a device-side flag set by the finalize kernel turns the sweeps after convergence
+// into no-ops, so the host synchronises once per batch, not once per sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "nkpr_internal.hpp"
+
+namespace nkpr {
+void create_graph(const nkpr_csc *c, nkpr_graph *g);
+}
+
+using namespace nkpr;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F> int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return NKPR_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return NKPR_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return NKPR_ERR_INVALID;
+    }
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        NKPR_HIP(hipGetDevice(&prev));
+        if (prev != dev) NKPR_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Buffers of one precision for a ping-pong phase.
+template <typename T> struct Vecs {
+    T *x[2];
+    T *xh[2];
+};
+
+struct RunCtx {
+    nkpr_graph *g;
+    const nkpr_params *p;
+    nkpr_stats *st;
+    uint64_t sweeps_left;
+    int grid_cache[2][3][2];  // [dbl][kind][folded]
+};
+
+// Launch sweeps of one phase until `done` or `max_launch` sweeps.  Returns completed sweeps.
+template <typename T>
+uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule rule,
+                   uint64_t max_launch, bool timing, double *ms_acc) {
+    nkpr_graph *g = rc.g;
+    const bool folded = g->weighted;
+    const bool dbl = sizeof(T) == 8;
+    int &grid = rc.grid_cache[dbl][kind][folded];
+    if (grid <= 0) grid = sweep_grid_size(g->device, dbl, folded, kind, g->n_items);
+    FinalizeArgs f{};
+    f.split_rows = g->split_rows.as<uint32_t>();
+    f.split_first = g->split_first.as<uint32_t>();
+    f.split_count = g->split_count.as<uint32_t>();
+    f.n_split_rows = g->n_split_rows;
+    f.n_blocks = (uint32_t)grid;
+    f.rule = rule;
+    a.partials = g->partials.as<double>();
+    a.items = g->items.as<uint2>();
+    a.n_items = g->n_items;
+    a.ctl = g->ctl.as<PhaseCtl>();
+
+    uint64_t launched = 0, done_sweeps = 0;
+    uint64_t batch = (max_launch <= 2) ? max_launch : 4;
+    const size_t need_ev = 2 * 64;
+    if (timing && g->events.size() < need_ev) {
+        while (g->events.size() < need_ev) {
+            hipEvent_t e;
+            NKPR_HIP(hipEventCreate(&e));
+            g->events.push_back(e);
+        }
+    }
+    while (launched < max_launch) {
+        if (rc.p->interrupt && *rc.p->interrupt)
+            throw Error(NKPR_ERR_INTERRUPTED, "interrupted");
+        const uint64_t nb = std::min<uint64_t>(std::min<uint64_t>(batch, 64), max_launch - launched);
+        for (uint64_t j = 0; j < nb; ++j) {
+            const uint64_t k = launched + j;
+            const int cur = (int)(k & 1), nxt = cur ^ 1;
+            a.xold = v.x[cur];
+            a.xnew = v.x[nxt];
+            a.xg = folded ? v.x[cur] : v.xh[cur];
+            a.xhat_new = folded ? nullptr : v.xh[nxt];
+            hipEvent_t e0 = timing ? g->events[2 * j] : nullptr;
+            hipEvent_t e1 = timing ? g->events[2 * j + 1] : nullptr;
+            launch_sweep<T>(a, f, kind, folded, grid, g->stream, e0, e1);
+        }
+        NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,
+                                g->stream));
+        NKPR_HIP(hipStreamSynchronize(g->stream));
+        const uint64_t it = g->ctl_host->iterations;
+        const uint64_t executed = it - done_sweeps;
+        if (timing) {
+            for (uint64_t j = 0; j < executed && j < nb; ++j) {
+                float ms = 0.f;
+                NKPR_HIP(hipEventElapsedTime(&ms, g->events[2 * j], g->events[2 * j + 1]));
+                *ms_acc += ms;
+            }
+        }
+        done_sweeps = it;
+        launched += nb;
+        if (g->ctl_host->done) break;
+        batch *= 2;
+    }
+    return done_sweeps;
+}
+
+double rel_error_bound(double damp, double l1, double sum) {
+    return sum > 0.0 ? 2.0 * damp / (1.0 - damp) * l1 / sum : INFINITY;
+}
+
+void ensure(DevBuf &b, size_t bytes) { b.alloc(bytes); }
+
+void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    NKPR_CHECK(g != nullptr && p != nullptr, "graph or params is NULL");
+    NKPR_CHECK(g->z == 0 || scores != nullptr, "scores is NULL");
+    NKPR_CHECK(p->damp >= 0.0 && p->damp < 1.0, "damp must be in [0, 1)");
+    NKPR_CHECK(p->norm == NKPR_L1 || p->norm == NKPR_L2, "norm must be NKPR_L1 or NKPR_L2");
+    NKPR_CHECK(p->sink_handling == NKPR_SINK_NONE || p->sink_handling == NKPR_SINK_DISTRIBUTE,
+               "unknown sink handling");
+    NKPR_CHECK(p->precision >= NKPR_FP64 && p->precision <= NKPR_FP32, "unknown precision");
+    NKPR_CHECK(p->stop == NKPR_STOP_STEP || p->stop == NKPR_STOP_L1_ERROR, "unknown stop rule");
+    NKPR_CHECK(!(p->tol < 0.0) && !std::isnan(p->tol), "tol must be >= 0");
+    DeviceGuard dg(g->device);
+    std::memset(st, 0, sizeof(*st));
+    const uint64_t z = g->z;
+    const uint32_t n = g->n;
+    const hipStream_t s = g->stream;
+    if (z == 0) {
+        st->maximum = p->normalized ? 0.0 : 1.0;
+        return;
+    }
+
+    // start vector in slot order on the device
+    double *sz;
+    if (p->scores_on_device) {
+        sz = scores;
+    } else {
+        ensure(g->zbuf, 8 * z);
+        sz = g->zbuf.as<double>();
+        NKPR_HIP(hipMemcpyAsync(sz, scores, 8 * z, hipMemcpyHostToDevice, s));
+    }
+    if (n == 0) {
+        // no node: nothing to iterate; the reference's loop would not run meaningfully
+        st->maximum = p->normalized ? (p->scores_on_device ? 0.0 : scores[0]) : 1.0;
+        return;
+    }
+
+    const double damp = p->damp;
+    const double tele = (1.0 - damp) / (double)n;
+    const bool folded = g->weighted;
+    const uint64_t max_it = p->max_iterations;
+    ensure(g->ctl, sizeof(PhaseCtl));
+    if (!g->ctl_host) NKPR_HIP(hipHostMalloc((void **)&g->ctl_host, sizeof(PhaseCtl), hipHostMallocDefault));
+    ensure(g->red_partials, 8 * 1024);
+    ensure(g->red_out, 8 * 4);
+    {
+        int cus = 0;
+        NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+        ensure(g->partials, 8 * 4 * (size_t)std::max(cus, 1) * 64);
+    }
+    ensure(g->split_part32, 4 * std::max<size_t>(g->n_split_items, 1));
+    ensure(g->split_part64, 8 * std::max<size_t>(g->n_split_items, 1));
+
+    RunCtx rc{g, p, st, max_it, {}};
+    const bool dist = p->sink_handling == NKPR_SINK_DISTRIBUTE;
+    PhaseCtl *ctl = g->ctl.as<PhaseCtl>();
+
+    auto init_phase = [&](const void *x, bool dbl) {
+        const double *sink_sum = nullptr;
+        if (dist && g->sink_begin < n) {
+            if (dbl) launch_range_sum<double>((const double *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+            else launch_range_sum<float>((const float *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+            sink_sum = g->red_out.as<double>();
+        }
+        launch_init_ctl(ctl, sink_sum, s);
+    };
+
+    auto base_args64 = [&]() {
+        SweepArgs<double> a{};
+        a.colptr = g->colptr.as<uint64_t>();
+        a.src = g->src.as<int32_t>();
+        a.coef = folded ? g->coef64.as<double>() : nullptr;
+        a.inv = folded ? nullptr : g->inv64.as<double>();
+        a.split_partial = g->split_part64.as<double>();
+        a.damp = damp;
+        a.tele = tele;
+        a.n = n;
+        a.sink_begin = g->sink_begin;
+        a.distribute = dist ? 1 : 0;
+        return a;
+    };
+    auto base_args32 = [&]() {
+        SweepArgs<float> a{};
+        a.colptr = g->colptr.as<uint64_t>();
+        a.src = g->src.as<int32_t>();
+        a.coef = folded ? g->coef32.as<float>() : nullptr;
+        a.inv = folded ? nullptr : g->inv32.as<float>();
+        a.split_partial = g->split_part32.as<float>();
+        a.damp = damp;
+        a.tele = tele;
+        a.n = n;
+        a.sink_begin = g->sink_begin;
+        a.distribute = dist ? 1 : 0;
+        return a;
+    };
+
+    const double *result = nullptr;  // compact FP64 result
+    double res_sum = 0.0, res_sink = 0.0;
+    uint64_t total = 0;
+    const bool timing = p->timing != 0;
+
+    auto alloc64 = [&]() {
+        ensure(g->x64a, 8 * (size_t)n);
+        ensure(g->x64b, 8 * (size_t)n);
+        if (!folded) {
+            ensure(g->xh64a, 8 * (size_t)n);
+            ensure(g->xh64b, 8 * (size_t)n);
+        }
+    };
+    auto alloc32 = [&]() {
+        ensure(g->x32a, 4 * (size_t)n);
+        ensure(g->x32b, 4 * (size_t)n);
+        if (!folded) {
+            ensure(g->xh32a, 4 * (size_t)n);
+            ensure(g->xh32b, 4 * (size_t)n);
+        }
+    };
+    Vecs<double> v64{};
+    Vecs<float> v32{};
+
+    if (p->precision == NKPR_FP64 || p->precision == NKPR_FP32) {
+        const bool dbl = p->precision == NKPR_FP64;
+        PhaseRule rule{};
+        rule.kind = (p->stop == NKPR_STOP_STEP) ? kRuleStep : kRuleErr;
+        rule.norm = p->norm;
+        rule.tol = p->tol;
+        rule.max_iterations = max_it;
+        rule.damp = damp;
+        if (dbl) {
+            alloc64();
+            v64 = {{g->x64a.as<double>(), g->x64b.as<double>()}, {g->xh64a.as<double>(), g->xh64b.as<double>()}};
+            launch_gather_start(sz, g->perm.as<int32_t>(), n, v64.x[0], nullptr, s);
+            if (!folded) launch_prescale(v64.x[0], g->inv64.as<double>(), v64.xh[0], nullptr, nullptr, nullptr, n, g->sink_begin, s);
+            init_phase(v64.x[0], true);
+            total = run_phase<double>(rc, base_args64(), v64, kSweepPlain, rule, std::max<uint64_t>(max_it, 1), timing, &st->ms_sweeps_fp64);
+            st->sweeps_fp64 = total;
+            result = v64.x[total & 1];
+        } else {
+            alloc32();
+            ensure(g->x64a, 8 * (size_t)n);
+            v32 = {{g->x32a.as<float>(), g->x32b.as<float>()}, {g->xh32a.as<float>(), g->xh32b.as<float>()}};
+            launch_gather_start(sz, g->perm.as<int32_t>(), n, nullptr, v32.x[0], s);
+            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
+            init_phase(v32.x[0], false);
+            total = run_phase<float>(rc, base_args32(), v32, kSweepPlain, rule, std::max<uint64_t>(max_it, 1), timing, &st->ms_sweeps_fp32);
+            st->sweeps_fp32 = total;
+            launch_to_double(v32.x[total & 1], g->x64a.as<double>(), n, s);
+            result = g->x64a.as<double>();
+        }
+        res_sum = g->ctl_host->sum;
+        res_sink = g->ctl_host->sink_out;
+        st->last_step_l1 = g->ctl_host->l1;
+        st->last_step = (p->norm == NKPR_L2) ? std::sqrt(g->ctl_host->l2) : g->ctl_host->l1;
+        st->l1_error_bound = rel_error_bound(damp, g->ctl_host->l1, g->ctl_host->sum);
+    } else {
+        // ---- mixed precision: FP32 sweeps, then FP64 residual sweeps + FP32 corrections ----
+        alloc32();
+        alloc64();
+        ensure(g->r32, 4 * (size_t)n);
+        v32 = {{g->x32a.as<float>(), g->x32b.as<float>()}, {g->xh32a.as<float>(), g->xh32b.as<float>()}};
+        double *xcur = g->x64a.as<double>();   // current FP64 iterate
+        double *ynew = g->x64b.as<double>();   // FP64 sweep output
+        double *xh = g->xh64a.as<double>();
+        launch_gather_start(sz, g->perm.as<int32_t>(), n, nullptr, v32.x[0], s);
+        if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
+        init_phase(v32.x[0], false);
+        PhaseRule r1{};
+        r1.kind = kRuleAbsStall;
+        r1.norm = NKPR_L1;
+        r1.tol = std::max(p->tol, 1e-4);  // stop the plain FP32 phase at a 1e-4 error bound
+        r1.tol_abs = 0.0;
+        r1.stall_ratio = 0.95;
+        r1.damp = damp;
+        const uint64_t cap1 = (max_it > 1) ? max_it - 1 : 0;  // keep one sweep for FP64
+        uint64_t k1 = 0;
+        if (cap1 > 0) {
+            r1.max_iterations = cap1;
+            k1 = run_phase<float>(rc, base_args32(), v32, kSweepPlain, r1, cap1, timing, &st->ms_sweeps_fp32);
+        }
+        st->sweeps_fp32 += k1;
+        total += k1;
+        launch_to_double(v32.x[k1 & 1], xcur, n, s);
+        for (int round = 0;; ++round) {
+            // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
+            if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
+            init_phase(xcur, true);
+            PhaseRule rr{};
+            rr.kind = kRuleErr;
+            rr.norm = NKPR_L1;
+            rr.tol = p->tol;
+            rr.max_iterations = 1;
+            rr.damp = damp;
+            SweepArgs<double> a64 = base_args64();
+            a64.r32 = g->r32.as<float>();
+            Vecs<double> vr{{xcur, ynew}, {xh, g->xh64b.as<double>()}};
+            const uint64_t k = run_phase<double>(rc, a64, vr, kSweepResid, rr, 1, timing, &st->ms_sweeps_fp64);
+            st->sweeps_fp64 += k;
+            total += k;
+            const PhaseCtl c = *g->ctl_host;
+            st->last_step_l1 = c.l1;
+            st->last_step = c.l1;
+            st->l1_error_bound = c.bound;
+            res_sum = c.sum;
+            res_sink = c.sink_out;
+            result = ynew;
+            if (c.bound <= p->tol || total >= max_it || round >= 8) break;
+            // FP32 correction solve: e = d P e + r  (e_0 = r)
+            ++st->refinements;
+            launch_copy_f32(g->r32.as<float>(), v32.x[0], n, s);
+            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
+            init_phase(v32.x[0], false);
+            PhaseRule rc2{};
+            rc2.kind = kRuleAbsStall;
+            rc2.norm = NKPR_L1;
+            rc2.tol = -1.0;
+            rc2.tol_abs = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
+            rc2.stall_ratio = 0.95;
+            rc2.damp = damp;
+            const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
+            const uint64_t cap = std::min<uint64_t>(left, 4000);
+            if (cap == 0) break;
+            rc2.max_iterations = cap;
+            SweepArgs<float> a32 = base_args32();
+            a32.tele = 0.0;
+            a32.base_vec = g->r32.as<float>();
+            const uint64_t kc = run_phase<float>(rc, a32, v32, kSweepBaseVec, rc2, cap, timing, &st->ms_sweeps_fp32);
+            st->sweeps_fp32 += kc;
+            st->sweeps_fp32_correction += kc;
+            total += kc;
+            launch_add_correction(xcur, v32.x[kc & 1], n, s);
+        }
+    }
+    st->iterations = total;
+    st->score_sum = res_sum;
+
+    // ---- normalisation (PageRank.cpp:73-92) and scatter to slot order -----------------------
+    double factor;
+    if (!p->normalized) factor = res_sum;
+    else factor = (1.0 / (double)n) * ((1.0 - damp) + damp * res_sink);
+    launch_scatter_result(result, g->perm.as<int32_t>(), n, factor, sz, s);
+    double maximum = 1.0;
+    if (p->normalized) {
+        launch_range_max(result, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+        double mx = 0.0, s0 = 0.0;
+        NKPR_HIP(hipMemcpyAsync(&mx, g->red_out.p, 8, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipMemcpyAsync(&s0, sz, 8, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+        maximum = std::max(mx / factor, s0);
+    }
+    st->maximum = maximum;
+    if (!p->scores_on_device) NKPR_HIP(hipMemcpyAsync(scores, sz, 8 * z, hipMemcpyDeviceToHost, s));
+    NKPR_HIP(hipStreamSynchronize(s));
+    st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int nkpr_abi_version(void) { return NKPR_ABI_VERSION; }
+
+const char *nkpr_last_error(void) { return g_last_error.c_str(); }
+
+int nkpr_params_default(nkpr_params *p) {
+    return guarded([&] {
+        NKPR_CHECK(p != nullptr, "params is NULL");
+        std::memset(p, 0, sizeof(*p));
+        p->damp = 0.85;
+        p->tol = 1e-8;
+        p->max_iterations = std::numeric_limits<uint64_t>::max();
+        p->norm = NKPR_L2;
+        p->sink_handling = NKPR_SINK_NONE;
+        p->normalized = 0;
+        p->precision = NKPR_FP64;
+        p->stop = NKPR_STOP_STEP;
+    });
+}
+
+int nkpr_graph_create(const nkpr_csc *csc, nkpr_graph **out) {
+    return guarded([&] {
+        NKPR_CHECK(out != nullptr, "out is NULL");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error(NKPR_ERR_NODEVICE, "no HIP device is visible");
+        int dev = 0;
+        NKPR_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        NKPR_HIP(hipGetDeviceProperties(&prop, dev));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw Error(NKPR_ERR_NODEVICE, std::string("libnkpr is built for gfx950, device is ") + prop.gcnArchName);
+        auto *g = new nkpr_graph();
+        g->device = dev;
+        try {
+            NKPR_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+            create_graph(csc, g);
+        } catch (...) {
+            nkpr_graph_destroy(g);
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int nkpr_graph_destroy(nkpr_graph *g) {
+    if (!g) return NKPR_OK;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
+    if (g->ctl_host) (void)hipHostFree(g->ctl_host);
+    hipStream_t s = g->stream;
+    delete g;  // frees device buffers
+    if (s) (void)hipStreamDestroy(s);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return NKPR_OK;
+}
+
+int nkpr_graph_get_info(const nkpr_graph *g, nkpr_graph_info *info) {
+    return guarded([&] {
+        NKPR_CHECK(g != nullptr && info != nullptr, "graph or info is NULL");
+        std::memset(info, 0, sizeof(*info));
+        info->z = g->z;
+        info->n = g->n;
+        info->m = g->m;
+        info->sinks = g->n - g->sink_begin;
+        info->max_in_degree = g->max_in_degree;
+        info->work_items = g->n_items;
+        info->device_bytes = g->perm.bytes + g->colptr.bytes + g->src.bytes + g->coef32.bytes +
+                             g->coef64.bytes + g->inv32.bytes + g->inv64.bytes + g->items.bytes;
+        info->weighted = g->weighted ? 1 : 0;
+        info->device = g->device;
+        info->prep_ms = g->prep_ms;
+    });
+}
+
+void *nkpr_graph_stream(const nkpr_graph *g) { return g ? (void *)g->stream : nullptr; }
+
+int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats) {
+    nkpr_stats local;
+    return guarded([&] { run_impl(g, p, scores, stats ? stats : &local); });
+}
+
+int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_t *ids_out,
+                 double *scores_out, uint64_t *count) {
+    return guarded([&] {
+        NKPR_CHECK(z == 0 || (scores != nullptr && ids_out != nullptr), "NULL argument");
+        std::vector<int64_t> ids;
+        ids.reserve(z);
+        for (uint64_t u = 0; u < z; ++u)
+            if (!exists || exists[u]) ids.push_back((int64_t)u);
+        std::stable_sort(ids.begin(), ids.end(), [&](int64_t a, int64_t b) {
+            if (scores[a] == scores[b]) return a < b;
+            return scores[a] > scores[b];
+        });
+        std::copy(ids.begin(), ids.end(), ids_out);
+        if (scores_out)
+            for (size_t i = 0; i < ids.size(); ++i) scores_out[i] = scores[ids[i]];
+        if (count) *count = ids.size();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Internal declarations shared by the translation units of libnkpr.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nkpr.h"
+
+namespace nkpr {
+
+// ---------------------------------------------------------------------------------------
+// Sweep schedule constants.  A sweep block is 256 threads (4 wave64s).  A "stream" work item
+// is a run of consecutive rows whose in-edges (at most kCap) are staged as products in LDS;
+// a row with more than kCap in-edges is a "vector" item reduced by the whole block; a row
+// with more than kVectorMax in-edges is cut into kChunk-sized "split" items whose partial
+// sums the finalize kernel adds in chunk order (deterministic).
+// ---------------------------------------------------------------------------------------
+constexpr int kBlock = 256;
+constexpr int kCap = 2048;
+constexpr int kRowsMax = 512;
+constexpr uint32_t kVectorMax = 16384;
+constexpr uint32_t kChunk = 8192;
+constexpr uint32_t kSplitFlag = 0x80000000u;
+constexpr int kFinalizeThreads = 1024;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define NKPR_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t nkpr_e_ = (call);                                                         \
+        if (nkpr_e_ != hipSuccess)                                                           \
+            throw ::nkpr::Error(NKPR_ERR_HIP, std::string(#call) + " failed: " +              \
+                                                  hipGetErrorString(nkpr_e_));               \
+    } while (0)
+
+#define NKPR_CHECK(cond, msg)                                                                \
+    do {                                                                                     \
+        if (!(cond)) throw ::nkpr::Error(NKPR_ERR_INVALID, (msg));                           \
+    } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b <= bytes && p) return;
+        reset();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(NKPR_ERR_NOMEM, "hipMalloc(" + std::to_string(b) +
+                                            " bytes) failed: " + hipGetErrorString(e));
+        }
+        bytes = b;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Device-resident control block of one sweep phase; written by the finalize kernel.
+struct PhaseCtl {
+    double sink_in;   // sum of sink scores of the vector the next sweep reads
+    double l1;        // ||x' - x||_1 of the last sweep
+    double l2;        // ||x' - x||_2^2 of the last sweep
+    double sum;       // sum of x' over existing nodes
+    double sink_out;  // sum of x' over sinks
+    double prev_l1;   // l1 of the sweep before
+    double bound;     // certified L1 error bound (ERR rule)
+    uint64_t iterations;
+    uint32_t done;
+    uint32_t pad;
+};
+
+// Stopping rules evaluated by the finalize kernel.
+enum RuleKind : int32_t {
+    kRuleStep = 0,       // reference: step in `norm` <= tol
+    kRuleErr = 1,        // 2 d/(1-d) l1 / sum <= tol
+    kRuleAbsStall = 2,   // l1 <= tol_abs, or the step stopped shrinking (rounding floor)
+};
+
+struct PhaseRule {
+    int32_t kind;
+    int32_t norm;
+    double tol;
+    double tol_abs;
+    double stall_ratio;
+    uint64_t max_iterations;
+    double damp;
+};
+
+template <typename T> struct SweepArgs {
+    const uint64_t *colptr;  // [n+1]
+    const int32_t *src;      // [m]
+    const T *coef;           // [m] folded w/deg(src) (weighted) or nullptr
+    const T *inv;            // [n] 1/deg (unweighted: prescale the new vector) or nullptr
+    const T *xg;             // gather vector (x, or x*inv when prescaled)
+    const T *xold;           // [n] previous iterate
+    T *xnew;                 // [n] new iterate
+    T *xhat_new;             // [n] prescaled new iterate (unweighted) or nullptr
+    const T *base_vec;       // [n] per-node right-hand side (correction solves) or nullptr
+    float *r32;              // [n] FP32 copy of x' - x (FP64 residual sweeps) or nullptr
+    const uint2 *items;      // work items (split chunks, vector rows, stream blocks)
+    uint32_t n_items;
+    T *split_partial;        // [n_split_items]
+    double *partials;        // [gridDim.x * 4]  (l1, l2, sum, sink) per block
+    PhaseCtl *ctl;
+    double damp;
+    double tele;             // constant right-hand side per node ((1-d)/n, or 0)
+    uint32_t n;
+    uint32_t sink_begin;     // nodes [sink_begin, n) have no out-edges
+    int32_t distribute;      // add damp * sink_in / n to every node
+};
+
+struct FinalizeArgs {
+    const uint32_t *split_rows;
+    const uint32_t *split_first;
+    const uint32_t *split_count;
+    uint32_t n_split_rows;
+    uint32_t n_blocks;
+    PhaseRule rule;
+};
+
+// Kernel launchers (nkpr_sweep.hip).
+enum SweepKind : int {
+    kSweepPlain = 0,      // x' = d P x + tele (+ sinks)
+    kSweepBaseVec = 1,    // x' = d P x + base_vec (+ sinks): correction solve
+    kSweepResid = 2,      // plain + r32 = float(x' - x)  (FP64 only)
+};
+
+template <typename T>
+void launch_sweep(const SweepArgs<T> &a, const FinalizeArgs &f, int kind, bool folded,
+                  int grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+int sweep_grid_size(int device, bool dbl, bool folded, int kind, uint32_t n_items);
+
+// Small vector kernels (nkpr_sweep.hip).
+void launch_gather_start(const double *scores_z, const int32_t *perm, uint32_t n,
+                         double *x64, float *x32, hipStream_t s);
+void launch_prescale(const double *x64, const double *inv64, double *xhat64, const float *x32,
+                     const float *inv32, float *xhat32, uint32_t n, uint32_t sink_begin,
+                     hipStream_t s);
+void launch_to_double(const float *x32, double *x64, uint32_t n, hipStream_t s);
+void launch_add_correction(double *x64, const float *e32, uint32_t n, hipStream_t s);
+void launch_copy_f32(const float *in, float *out, uint32_t n, hipStream_t s);
+void launch_scatter_result(const double *y, const int32_t *perm, uint32_t n, double factor,
+                           double *scores_z, hipStream_t s);
+// Deterministic two-stage sums / max over x[begin, end).
+template <typename T>
+void launch_range_sum(const T *x, uint32_t begin, uint32_t end, double *partials,
+                      double *out, hipStream_t s);
+void launch_range_max(const double *x, uint32_t n, double *partials, double *out,
+                      hipStream_t s);
+void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, hipStream_t s);
+
+}  // namespace nkpr
+
+// The graph handle.
+struct nkpr_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t z = 0, m = 0;
+    uint32_t n = 0, sink_begin = 0;
+    uint64_t max_in_degree = 0;
+    bool weighted = false;
+    double prep_ms = 0.0;
+
+    nkpr::DevBuf perm;      // int32 [n]   compact id -> slot
+    nkpr::DevBuf colptr;    // uint64 [n+1]
+    nkpr::DevBuf src;       // int32 [m]
+    nkpr::DevBuf coef32;    // float [m]   (weighted)
+    nkpr::DevBuf coef64;    // double [m]  (weighted)
+    nkpr::DevBuf inv32;     // float [n]   (unweighted)
+    nkpr::DevBuf inv64;     // double [n]  (unweighted)
+    std::vector<uint8_t> exists_host;  // empty: every slot exists
+
+    // schedule
+    nkpr::DevBuf items;     // uint2 [n_items]
+    uint32_t n_items = 0, n_split_items = 0;
+    nkpr::DevBuf split_rows, split_first, split_count;  // uint32 [n_split_rows]
+    uint32_t n_split_rows = 0;
+
+    // run workspace (lazily sized)
+    nkpr::DevBuf zbuf;                  // double [z] staging of host scores
+    nkpr::DevBuf x64a, x64b, xh64a, xh64b;
+    nkpr::DevBuf x32a, x32b, xh32a, xh32b;
+    nkpr::DevBuf r32, base64;
+    nkpr::DevBuf partials;              // double [grid*4]
+    nkpr::DevBuf red_partials;          // double [1024]
+    nkpr::DevBuf red_out;               // double [4]
+    nkpr::DevBuf split_part32, split_part64;
+    nkpr::DevBuf ctl;                   // PhaseCtl
+    nkpr::PhaseCtl *ctl_host = nullptr; // pinned mirror
+    std::vector<hipEvent_t> events;
+};

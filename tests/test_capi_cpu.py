@@ -1,0 +1,79 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol include/nkpr.h
+declares, and its host-only entry points behave; no device compute is attempted here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from networkit_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "nkpr.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nkpr_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_expected_symbols():
+    assert declared_symbols() == sorted(capi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    out = os.popen("nm -D --defined-only %s" % capi.LIB_PATH).read()
+    for name in declared_symbols():
+        assert re.search(r"\bT %s\b" % name, out), name
+
+
+def test_struct_sizes_match_header_layout():
+    # sizes of the C structs (x86-64 SysV): keep ctypes mirrors in sync with nkpr.h
+    assert ctypes.sizeof(capi.Csc) == 8 * 7 + 8
+    assert ctypes.sizeof(capi.Params) == 8 * 3 + 4 * 8 + 8
+    assert ctypes.sizeof(capi.Stats) == 8 * 13
+    assert ctypes.sizeof(capi.GraphInfo) == 8 * 7 + 8 + 8
+
+
+def test_params_default_is_reference_default():
+    p = capi.default_params()
+    assert p.damp == 0.85 and p.tol == 1e-8
+    assert p.max_iterations == 2 ** 64 - 1
+    assert p.norm == capi.NKPR_L2
+    assert p.sink_handling == capi.NKPR_SINK_NONE
+    assert p.normalized == 0 and p.precision == capi.NKPR_FP64
+    assert p.stop == capi.NKPR_STOP_STEP
+    assert capi.lib().nkpr_abi_version() == 1
+
+
+def test_graph_create_without_device_reports_nodevice():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(capi.NkprError) as e:
+        capi.DeviceGraph(3, np.array([0, 1, 2, 3]), np.array([1, 2, 0]))
+    assert e.value.code == capi.NKPR_ERR_NODEVICE
+    assert "device" in capi.lib().nkpr_last_error().decode()
+
+
+def test_null_arguments_are_errors_not_crashes():
+    L = capi.lib()
+    assert L.nkpr_graph_create(None, None) == capi.NKPR_ERR_INVALID or \
+        L.nkpr_graph_create(None, None) == capi.NKPR_ERR_NODEVICE
+    assert L.nkpr_run(None, None, None, None) != capi.NKPR_OK
+    assert L.nkpr_graph_destroy(None) == capi.NKPR_OK
+    assert L.nkpr_params_default(None) == capi.NKPR_ERR_INVALID
+
+
+def test_ranking_order_score_desc_id_asc():
+    rng = np.random.default_rng(1)
+    s = rng.integers(0, 10, 1000) / 3.0
+    ex = (rng.random(1000) > 0.05).astype(np.uint8)
+    ids, vals = capi.ranking(s, ex)
+    ref = sorted([i for i in range(1000) if ex[i]], key=lambda i: (-s[i], i))
+    assert ids.tolist() == ref
+    assert np.array_equal(vals, s[ids])

@@ -1,0 +1,281 @@
+"""GPU parity of the PageRank drop-in against the oracle and the reference's known answers.
+
+Bars (written into each assertion):
+  FP64 mode, reference stopping rule: the same iteration count as the oracle and an L1
+  difference <= 1e-12 (rounding of a different summation order only); the same ranking.
+  Mixed mode: L1 error <= tol (1e-8) against a tightly converged FP64 oracle and the same
+  top-1000 ranking up to ties within 1e-12.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import networkit_amd as nk
+import oracle
+from graphs import (EIGHT_PR, EIGHT_PR_NORMALIZED, WIKI_TOP, WIKI_TOP_NORMALIZED, eight_node,
+                    random_graph, skewed_graph, wiki_vote)
+from networkit_amd import capi
+from networkit_amd import generators as gen
+
+pytestmark = pytest.mark.gpu
+
+NORMS = {0: nk.centrality.Norm.l1norm, 1: nk.centrality.Norm.l2norm}
+
+
+def build_graph(z, u, v, w, directed, remove=()):
+    G = nk.Graph(z, w is not None, directed)
+    G.addEdges(u, v, w)
+    for x in remove:
+        G.removeNode(x)
+    return G
+
+
+def oracle_graph(z, u, v, w, directed, remove=()):
+    keep = np.ones(len(u), bool)
+    for x in remove:
+        keep &= (u != x) & (v != x)
+    exists = None
+    if remove:
+        exists = np.ones(z, np.uint8)
+        exists[list(remove)] = 0
+    return oracle.in_lists(z, u[keep], v[keep], None if w is None else w[keep], directed,
+                           exists=exists)
+
+
+def run_api(G, damp=0.85, tol=1e-8, norm=1, normalized=False, sinks=0, max_it=None,
+            precision=0, pr=None):
+    if pr is None:
+        pr = nk.centrality.PageRank(G, damp, tol, normalized, sinks)
+    pr.norm = NORMS[norm]
+    if max_it is not None:
+        pr.maxIterations = max_it
+    pr.precision = precision
+    pr.run()
+    return pr, np.array(pr.scores())
+
+
+def assert_same_ranking(a, b, exists=None, ties=1e-12, top=None):
+    ra = oracle.ranking(a, exists)
+    rb = oracle.ranking(b, exists)
+    if top is not None:
+        ra, rb = ra[:top], rb[:top]
+    diff = np.nonzero(ra != rb)[0]
+    for i in diff:
+        # positions may swap only between nodes whose scores tie within `ties`
+        assert abs(a[ra[i]] - a[rb[i]]) <= ties, (i, ra[i], rb[i])
+
+
+def check_fp64_parity(z, u, v, w, directed, remove=(), **kw):
+    G = build_graph(z, u, v, w, directed, remove)
+    g = oracle_graph(z, u, v, w, directed, remove)
+    pr, got = run_api(G, **kw)
+    ref, it, mx = oracle.pagerank(g, kw.get("damp", 0.85), kw.get("tol", 1e-8),
+                                  kw.get("norm", 1), kw.get("max_it", 2 ** 64 - 1),
+                                  kw.get("normalized", False), kw.get("sinks", 0))
+    assert pr.numberOfIterations() == it
+    scale = 1.0 if not kw.get("normalized") else max(1.0, np.abs(ref).max())
+    assert np.abs(got - ref).sum() <= 1e-12 * scale * max(1, z / 1000)
+    assert pr.maximum() == pytest.approx(mx, rel=1e-12)
+    assert_same_ranking(ref, got, g.exists)
+    return pr, got, ref
+
+
+# ---------------------------------------------------------------- reference known answers
+@pytest.mark.parametrize("normalized", [False, True])
+def test_eight_node_known_answers(normalized):
+    z, u, v, w = eight_node()
+    pr, got, ref = check_fp64_parity(z, u, v, w, False, normalized=normalized)
+    expect = EIGHT_PR_NORMALIZED if normalized else EIGHT_PR
+    assert np.allclose(got, expect, atol=1e-4)
+
+
+@pytest.mark.parametrize("directed", [True, False])
+@pytest.mark.parametrize("norm", [0, 1])
+@pytest.mark.parametrize("normalized", [False, True])
+def test_wiki_vote_known_answers(directed, norm, normalized):
+    z, u, v = wiki_vote(directed)
+    pr, got, ref = check_fp64_parity(z, u, v, None, directed, norm=norm,
+                                     normalized=normalized)
+    r = pr.ranking()
+    top, val = (WIKI_TOP_NORMALIZED if normalized else WIKI_TOP)[directed]
+    assert r[0][0] == top
+    assert abs(r[0][1] - val) <= (1e-4 if normalized else 1e-3)
+    # re-run with maxIterations = 2 starts from the previous scores (PageRank.cpp:26)
+    pr.maxIterations = 2
+    pr.run()
+    assert pr.numberOfIterations() <= 2
+    g = oracle_graph(z, u, v, None, directed)
+    ref2, it2, _ = oracle.pagerank(g, norm=norm, max_iterations=2, normalized=normalized,
+                                   start=got)
+    assert pr.numberOfIterations() == it2
+    assert np.abs(np.array(pr.scores()) - ref2).sum() <= 1e-12 * max(1, np.abs(ref2).max())
+
+
+# ---------------------------------------------------------------- oracle parity, FP64 mode
+def test_config1_er_undirected_unweighted():
+    u, v = gen.erdos_renyi(10000, 1e-3, seed=1)
+    for sinks in (0, 1):
+        check_fp64_parity(10000, u, v, None, False, sinks=sinks)
+
+
+@pytest.mark.parametrize("sinks", [0, 1])
+@pytest.mark.parametrize("normalized", [False, True])
+def test_rmat_weighted_directed(sinks, normalized):
+    g = gen.rmat(13, 8, seed=2)
+    z = g["z"]
+    rows = np.repeat(np.arange(z), np.diff(g["colptr"].astype(np.int64)))
+    check_fp64_parity(z, g["src"], rows, g["weight"].astype(np.float64), True, sinks=sinks,
+                      normalized=normalized)
+
+
+def test_heavy_rows_split_and_vector_items():
+    # one row with 40000 in-edges (split across blocks), one with 5000 (one block), the rest
+    # a sparse random graph with self-loops and multi-edges
+    rng = np.random.default_rng(3)
+    z = 50000
+    u = np.concatenate([np.arange(1, 40001), rng.integers(0, z, 5000), rng.integers(0, z, 60000)])
+    v = np.concatenate([np.zeros(40000, np.int64), np.full(5000, 7), rng.integers(0, z, 60000)])
+    w = rng.uniform(0.5, 1.5, len(u))
+    for directed in (True, False):
+        check_fp64_parity(z, u, v, w, directed, norm=0)
+        check_fp64_parity(z, u, v, None, directed, sinks=1, normalized=True)
+
+
+def test_skewed_directed_weighted():
+    z, u, v, w = skewed_graph(30000, 400000, 5)
+    check_fp64_parity(z, u, v, w, True)
+
+
+def test_deleted_nodes_self_loops_multi_edges():
+    z, u, v, w = random_graph(3000, 12000, 9, directed=True, weighted=True)
+    for directed in (True, False):
+        pr, got, ref = check_fp64_parity(z, u, v, w, directed, remove=(0, 17, 2999),
+                                         normalized=True)
+        assert got[17] == 1.0 / 2997  # deleted slots keep the start value
+
+
+def test_max_iterations_zero_and_one():
+    z, u, v, w = random_graph(500, 2000, 2)
+    check_fp64_parity(z, u, v, w, True, max_it=0)
+    check_fp64_parity(z, u, v, w, True, max_it=1)
+
+
+def test_edgeless_and_tiny_graphs():
+    G = nk.Graph(5, False, True)
+    pr, got = run_api(G)
+    assert np.allclose(got, 0.2, rtol=0, atol=1e-16)
+    # sweep 1 moves every score to (1-d)/n, sweep 2 repeats it: the step is 0
+    _, it, _ = oracle.pagerank(oracle.in_lists(5, [], [], None, True))
+    assert pr.numberOfIterations() == it == 2
+    G = nk.Graph(1, False, False)
+    G.addEdge(0, 0)
+    pr, got = run_api(G)
+    assert got.tolist() == [1.0]
+
+
+def test_centralization_and_score():
+    z, u, v, w = random_graph(400, 1600, 4)
+    G = build_graph(z, u, v, None, True)
+    pr, got = run_api(G, normalized=True)
+    assert pr.score(5) == got[5]
+    c = pr.centralization()
+    mx = pr.maximum()
+    center = got.max()
+    assert c == pytest.approx((center - got).sum() / (mx - got).sum())
+
+
+def test_graph_change_reexports():
+    z, u, v, w = random_graph(300, 900, 6)
+    G = build_graph(z, u, v, None, True)
+    pr, a = run_api(G)
+    G.addEdge(0, 1)
+    pr2, b = run_api(G)
+    g = oracle.in_lists(z, np.append(u, 0), np.append(v, 1), None, True)
+    ref, _, _ = oracle.pagerank(g)
+    assert np.abs(b - ref).sum() <= 1e-12
+
+
+# ---------------------------------------------------------------- mixed precision
+@pytest.mark.parametrize("sinks", [0, 1])
+def test_mixed_precision_reaches_tol(sinks):
+    g = gen.rmat(15, 16, seed=2)
+    z = g["z"]
+    rows = np.repeat(np.arange(z), np.diff(g["colptr"].astype(np.int64)))
+    w = g["weight"].astype(np.float64)
+    G = build_graph(z, g["src"], rows, w, True)
+    og = oracle_graph(z, g["src"], rows, w, True)
+    ref, _, _ = oracle.pagerank(og, tol=1e-15, norm=0, max_iterations=3000, sink_mode=sinks)
+    pr = nk.centrality.PageRank(G, 0.85, 1e-8, False, sinks)
+    pr.precision = nk.centrality.Precision.Mixed
+    pr.run()
+    got = np.array(pr.scores())
+    st = pr.stats()
+    err = np.abs(got - ref).sum()
+    print("mixed: L1 %.3e bound %.3e sweeps fp32 %d fp64 %d" % (
+        err, st["l1_error_bound"], st["sweeps_fp32"], st["sweeps_fp64"]))
+    assert err <= 1e-8
+    assert st["l1_error_bound"] <= 1e-8 and err <= st["l1_error_bound"]
+    assert st["sweeps_fp64"] <= 4 and st["sweeps_fp32"] > 0
+    assert pr.numberOfIterations() == st["sweeps_fp32"] + st["sweeps_fp64"]
+    assert_same_ranking(ref, got, top=1000)
+
+
+def test_fp32_mode_is_close():
+    z, u, v, w = random_graph(5000, 40000, 12, weighted=True)
+    G = build_graph(z, u, v, w, True)
+    pr = nk.centrality.PageRank(G, 0.85, 1e-5)
+    pr.precision = nk.centrality.Precision.FP32
+    pr.certifiedStop = True
+    pr.run()
+    got = np.array(pr.scores())
+    ref, _, _ = oracle.pagerank(oracle_graph(z, u, v, w, True), tol=1e-14, norm=0)
+    # certified bound 1e-5 on the FP32 iterate; FP32 rounding adds ~1e-6 at this size
+    assert np.abs(got - ref).sum() < 1e-5 + 2e-6
+    assert pr.stats()["sweeps_fp64"] == 0
+
+
+# ---------------------------------------------------------------- the C ABI directly
+def test_capi_device_pointers_and_device_scores():
+    import torch
+    g = gen.rmat(14, 16, seed=5, device="cuda")
+    dg = capi.DeviceGraph(g["z"], g["colptr"], g["src"], g["weight"], device=True)
+    info = dg.info()
+    assert info["n"] == g["z"] and info["m"] == g["src"].numel()
+    x = torch.full((g["z"],), 1.0 / g["z"], dtype=torch.float64, device="cuda")
+    st = dg.run(x, capi.default_params(scores_on_device=1, tol=1e-10))
+    torch.cuda.synchronize()
+    got = x.cpu().numpy()
+    h = gen.rmat(14, 16, seed=5, device="cpu")
+    assert np.array_equal(h["src"], g["src"].cpu().numpy())
+    z = h["z"]
+    rows = np.repeat(np.arange(z), np.diff(h["colptr"].astype(np.int64)))
+    ref, it, _ = oracle.pagerank(oracle.in_lists(z, h["src"], rows,
+                                                 h["weight"].astype(np.float64), True),
+                                 tol=1e-10)
+    assert st["iterations"] == it
+    assert np.abs(got - ref).sum() <= 1e-12
+
+
+def test_capi_interrupt_flag():
+    z, u, v, w = random_graph(1000, 5000, 1)
+    c = gen.csc_from_edges(z, u, v, None, True)
+    dg = capi.DeviceGraph(z, c["colptr"], c["src"])
+    flag = ctypes.c_int32(1)
+    p = capi.default_params(interrupt=ctypes.cast(ctypes.pointer(flag), ctypes.c_void_p))
+    x = np.full(z, 1.0 / z)
+    with pytest.raises(capi.NkprError) as e:
+        dg.run(x, p)
+    assert e.value.code == capi.NKPR_ERR_INTERRUPTED
+    assert np.all(x == 1.0 / z)  # untouched
+
+
+def test_capi_rejects_bad_input():
+    with pytest.raises(capi.NkprError) as e:
+        capi.DeviceGraph(3, np.array([0, 1, 2, 3]), np.array([1, 5, 0]))
+    assert e.value.code == capi.NKPR_ERR_INVALID
+    with pytest.raises(capi.NkprError):
+        capi.DeviceGraph(3, np.array([0, 2, 1, 3]), np.array([1, 2, 0]))
+    dg = capi.DeviceGraph(3, np.array([0, 1, 2, 3]), np.array([1, 2, 0]))
+    with pytest.raises(capi.NkprError):
+        dg.run(np.full(3, 1 / 3), capi.default_params(damp=1.5))
