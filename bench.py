@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--sweeps-only", type=int, default=0,
+                    help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -84,15 +86,41 @@ def main():
     x = torch.empty(z, dtype=torch.float64, device=dev)
 
     def solve(precision, tol, timing=1):
+        # torch's own stream may still read x (clones, error sums): order it before the refill
+        torch.cuda.current_stream().synchronize()
         with torch.cuda.stream(stream):
             x.fill_(1.0 / z)
         p = capi.default_params(scores_on_device=1, precision=precision, tol=tol,
                                 stop=capi.NKPR_STOP_L1_ERROR, timing=timing)
         return dg.run(x, p)
 
+    if args.sweeps_only:
+        # reference copy kernel of known size (1 GiB read + 1 GiB written) for scaling the
+        # FETCH_SIZE / WRITE_SIZE counters of a profiled run
+        a_src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        a_dst = torch.empty_like(a_src)
+        a_dst.copy_(a_src)
+        torch.cuda.synchronize()
+        del a_src, a_dst
+        for name, prec, per_edge, per_node in (("fp32", capi.NKPR_FP32, 12, 16),
+                                                ("fp64", capi.NKPR_FP64, 20, 24)):
+            torch.cuda.current_stream().synchronize()
+            with torch.cuda.stream(stream):
+                x.fill_(1.0 / z)
+            p = capi.default_params(scores_on_device=1, precision=prec, tol=0.0,
+                                    max_iterations=args.sweeps_only, timing=1)
+            st = dg.run(x, p)
+            k = st["sweeps_fp32"] + st["sweeps_fp64"]
+            ms = (st["ms_sweeps_fp32"] + st["ms_sweeps_fp64"]) / k
+            gbs = (per_edge * m + per_node * n) / (ms / 1e3) / 1e9
+            log("%s: %d sweeps, %.3f ms/sweep, %.0f GB/s algorithmic, %.3g edge-updates/s" % (
+                name, k, ms, gbs, m / (ms / 1e3)))
+        return
+
     # tightly converged FP64 reference (not timed)
     st = solve(capi.NKPR_FP64, 1e-13)
     ref = x.clone()
+    torch.cuda.synchronize()
     log("reference: %d FP64 sweeps, certified L1 bound %.2e" % (st["iterations"],
                                                               st["l1_error_bound"]))
 
@@ -110,6 +138,7 @@ def main():
     sec_mixed, st_mixed = timed(capi.NKPR_MIXED, args.steps, args.warmup)
     err_mixed = float((x - ref).abs().sum())
     x_mixed = x.clone()
+    torch.cuda.synchronize()
     sec_fp64, st_fp64 = timed(capi.NKPR_FP64, args.steps, args.warmup)
     err_fp64 = float((x - ref).abs().sum())
     # top-1000 ranking agreement of the mixed result with the reference

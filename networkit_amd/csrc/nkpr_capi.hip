@@ -63,13 +63,36 @@ template <typename T> struct Vecs {
     T *xh[2];
 };
 
+void ensure(DevBuf &b, size_t bytes) { b.alloc(bytes); }
+
 struct RunCtx {
     nkpr_graph *g;
     const nkpr_params *p;
     nkpr_stats *st;
     uint64_t sweeps_left;
-    int grid_cache[2][3][2];  // [dbl][kind][folded]
+    int workers[2][3][2];  // [dbl][kind][folded] workers per slice
+    int combine_grid;
 };
+
+Schedule make_schedule(nkpr_graph *g, uint32_t workers) {
+    Schedule sc{};
+    sc.src = g->src.as<int32_t>();
+    sc.n_slices = g->n_slices;
+    sc.drow = g->drow.as<uint32_t>();
+    sc.dptr = g->dptr.as<uint32_t>();
+    sc.ditems = g->ditems.as<uint4>();
+    sc.n_d = g->n_d;
+    sc.n_ditems = g->n_ditems;
+    sc.srow = g->srow.as<uint32_t>();
+    sc.sptr = g->sptr.as<uint32_t>();
+    sc.sitems = g->sitems.as<uint4>();
+    for (int q = 0; q <= kSlices; ++q) sc.sitem_off[q] = g->sitem_off[q];
+    sc.n_s = g->n_s;
+    sc.split_flag = g->split_flag.as<uint8_t>();
+    sc.claim = g->claim.as<uint32_t>();
+    sc.workers = workers;
+    return sc;
+}
 
 // Launch sweeps of one phase until `done` or `max_launch` sweeps.  Returns completed sweeps.
 template <typename T>
@@ -78,18 +101,19 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
     nkpr_graph *g = rc.g;
     const bool folded = g->weighted;
     const bool dbl = sizeof(T) == 8;
-    int &grid = rc.grid_cache[dbl][kind][folded];
-    if (grid <= 0) grid = sweep_grid_size(g->device, dbl, folded, kind, g->n_items);
+    int &workers = rc.workers[dbl][kind][folded];
+    if (workers <= 0)
+        workers = std::max(1, sweep_resident_blocks(g->device, dbl, folded, kind) / (int)g->n_slices);
     FinalizeArgs f{};
     f.split_rows = g->split_rows.as<uint32_t>();
     f.split_first = g->split_first.as<uint32_t>();
     f.split_count = g->split_count.as<uint32_t>();
     f.n_split_rows = g->n_split_rows;
-    f.n_blocks = (uint32_t)grid;
+    f.n_blocks = (uint32_t)(workers * g->n_slices + (g->n_s ? rc.combine_grid : 0));
     f.rule = rule;
+    ensure(g->partials, 8 * 4 * (size_t)f.n_blocks);
+    a.sc = make_schedule(g, (uint32_t)workers);
     a.partials = g->partials.as<double>();
-    a.items = g->items.as<uint2>();
-    a.n_items = g->n_items;
     a.ctl = g->ctl.as<PhaseCtl>();
 
     uint64_t launched = 0, done_sweeps = 0;
@@ -115,7 +139,7 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
             a.xhat_new = folded ? nullptr : v.xh[nxt];
             hipEvent_t e0 = timing ? g->events[2 * j] : nullptr;
             hipEvent_t e1 = timing ? g->events[2 * j + 1] : nullptr;
-            launch_sweep<T>(a, f, kind, folded, grid, g->stream, e0, e1);
+            launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
         }
         NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,
                                 g->stream));
@@ -140,8 +164,6 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
 double rel_error_bound(double damp, double l1, double sum) {
     return sum > 0.0 ? 2.0 * damp / (1.0 - damp) * l1 / sum : INFINITY;
 }
-
-void ensure(DevBuf &b, size_t bytes) { b.alloc(bytes); }
 
 void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *st) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -187,15 +209,16 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
     if (!g->ctl_host) NKPR_HIP(hipHostMalloc((void **)&g->ctl_host, sizeof(PhaseCtl), hipHostMallocDefault));
     ensure(g->red_partials, 8 * 1024);
     ensure(g->red_out, 8 * 4);
-    {
-        int cus = 0;
-        NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-        ensure(g->partials, 8 * 4 * (size_t)std::max(cus, 1) * 64);
-    }
+    int cus = 0;
+    NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
     ensure(g->split_part32, 4 * std::max<size_t>(g->n_split_items, 1));
     ensure(g->split_part64, 8 * std::max<size_t>(g->n_split_items, 1));
 
-    RunCtx rc{g, p, st, max_it, {}};
+    ensure(g->part32, 4 * std::max<size_t>((size_t)g->n_slices * g->n_s, 1));
+    ensure(g->part64, 8 * std::max<size_t>((size_t)g->n_slices * g->n_s, 1));
+    RunCtx rc{g, p, st, max_it, {}, 0};
+    rc.combine_grid = (int)std::min<uint64_t>(((uint64_t)g->n_s + kCombineThreads - 1) / kCombineThreads,
+                                              (uint64_t)std::max(cus, 1) * 8);
     const bool dist = p->sink_handling == NKPR_SINK_DISTRIBUTE;
     PhaseCtl *ctl = g->ctl.as<PhaseCtl>();
 
@@ -206,14 +229,13 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
             else launch_range_sum<float>((const float *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
             sink_sum = g->red_out.as<double>();
         }
-        launch_init_ctl(ctl, sink_sum, s);
+        launch_init_ctl(ctl, sink_sum, g->claim.as<uint32_t>(), s);
     };
 
     auto base_args64 = [&]() {
         SweepArgs<double> a{};
-        a.colptr = g->colptr.as<uint64_t>();
-        a.src = g->src.as<int32_t>();
         a.coef = folded ? g->coef64.as<double>() : nullptr;
+        a.partial = g->part64.as<double>();
         a.inv = folded ? nullptr : g->inv64.as<double>();
         a.split_partial = g->split_part64.as<double>();
         a.damp = damp;
@@ -225,9 +247,8 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
     };
     auto base_args32 = [&]() {
         SweepArgs<float> a{};
-        a.colptr = g->colptr.as<uint64_t>();
-        a.src = g->src.as<int32_t>();
         a.coef = folded ? g->coef32.as<float>() : nullptr;
+        a.partial = g->part32.as<float>();
         a.inv = folded ? nullptr : g->inv32.as<float>();
         a.split_partial = g->split_part32.as<float>();
         a.damp = damp;
@@ -472,7 +493,9 @@ int nkpr_graph_get_info(const nkpr_graph *g, nkpr_graph_info *info) {
         info->max_in_degree = g->max_in_degree;
         info->work_items = g->n_items;
         info->device_bytes = g->perm.bytes + g->colptr.bytes + g->src.bytes + g->coef32.bytes +
-                             g->coef64.bytes + g->inv32.bytes + g->inv64.bytes + g->items.bytes;
+                             g->coef64.bytes + g->inv32.bytes + g->inv64.bytes + g->ditems.bytes +
+                             g->sitems.bytes + g->sptr.bytes + g->drow.bytes + g->dptr.bytes +
+                             g->srow.bytes;
         info->weighted = g->weighted ? 1 : 0;
         info->device = g->device;
         info->prep_ms = g->prep_ms;

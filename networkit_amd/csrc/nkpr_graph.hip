@@ -19,7 +19,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 
 #include "nkpr_internal.hpp"
 
@@ -135,65 +138,309 @@ __global__ void inv_degree_kernel(const int32_t *perm, uint32_t n, uint32_t sink
     inv32[r] = (float)v;
 }
 
-void build_schedule(nkpr_graph *g, const std::vector<uint64_t> &cp) {
-    std::vector<uint2> split, vec, stream;
-    std::vector<uint32_t> srow, sfirst, scount;
-    const uint32_t n = g->n;
-    uint64_t maxdeg = 0;
+// Per sliced row j: number of in-edges whose source falls in each slice (slice-major output).
+// Slice of an in-edge of sliced row j from source v.  Sources ranked below `rep` (the most
+// gathered ones; relabelled ids are out-degree ranks) are cached by every XCD and their edges
+// are spread over the slices by row, which keeps the slices balanced and keeps the requests
+// for the hottest lines off a single L2 channel; every other source belongs to one slice by
+// its 32-id line group.
+__device__ __forceinline__ uint32_t slice_of(uint32_t v, uint64_t j, uint32_t rep, uint32_t S) {
+    return (v < rep) ? (uint32_t)(j % S) : ((v >> kSliceShift) % S);
+}
+
+__global__ void slice_count_kernel(const uint64_t *colptr, const int32_t *src,
+                                   const uint32_t *srow, uint32_t n_s, uint32_t S, uint32_t rep,
+                                   uint32_t *counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t j = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < n_s;
+         j += nwaves) {
+        const uint32_t r = srow[j];
+        const uint64_t b = colptr[r], e = colptr[r + 1];
+        uint32_t cnt[kSlices] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint64_t i = b; i < e; i += 64) {
+            const bool valid = i + lane < e;
+            const uint32_t sl = valid ? slice_of((uint32_t)src[i + lane], j, rep, S) : 0u;
+            for (uint32_t q = 0; q < S; ++q) cnt[q] += __popcll(__ballot(valid && sl == q));
+        }
+        if (lane == 0)
+            for (uint32_t q = 0; q < S; ++q) counts[(uint64_t)q * n_s + j] = cnt[q];
+    }
+}
+
+__global__ void add_base_kernel(uint32_t *p, uint64_t n, uint32_t base) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] += base;
+}
+
+// Copy the in-edges of every row to its place in the sweep order: direct rows keep their
+// (source-sorted) order; sliced rows are split slice by slice, source order kept within a
+// slice (stable: ballot ranks).
+__global__ void reorder_edges_kernel(const uint64_t *colptr, const int32_t *src,
+                                     const float *c32, const double *c64, const uint32_t *rows,
+                                     uint32_t nrows, const uint32_t *dptr, const uint32_t *sptr,
+                                     uint32_t n_s, uint32_t S, uint32_t rep, int32_t *src2,
+                                     float *c32_2, double *c64_2) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t j = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nrows;
+         j += nwaves) {
+        const uint32_t r = rows[j];
+        const uint64_t b = colptr[r], e = colptr[r + 1];
+        if (dptr) {
+            const uint32_t d = dptr[j];
+            for (uint64_t i = b + lane; i < e; i += 64) {
+                src2[d + (i - b)] = src[i];
+                if (c32) {
+                    c32_2[d + (i - b)] = c32[i];
+                    c64_2[d + (i - b)] = c64[i];
+                }
+            }
+            continue;
+        }
+        uint32_t run[kSlices] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint64_t i = b; i < e; i += 64) {
+            const bool valid = i + lane < e;
+            const int32_t v = valid ? src[i + lane] : 0;
+            const uint32_t sl = valid ? slice_of((uint32_t)v, j, rep, S) : 0u;
+            uint32_t dest = 0;
+            for (uint32_t q = 0; q < S; ++q) {
+                const uint64_t mask = __ballot(valid && sl == q);
+                if (valid && sl == q)
+                    dest = sptr[(uint64_t)q * n_s + j] + run[q] + (uint32_t)__popcll(mask & below);
+                run[q] += (uint32_t)__popcll(mask);
+            }
+            if (valid) {
+                src2[dest] = v;
+                if (c32) {
+                    c32_2[dest] = c32[i + lane];
+                    c64_2[dest] = c64[i + lane];
+                }
+            }
+        }
+    }
+}
+
+struct SplitRec {
+    uint32_t first[kSlices];
+    uint32_t count[kSlices];
+};
+
+// Greedy work items over rows [0, nrows) of a list with absolute in-edge offsets ptr[0..nrows]:
+// stream blocks of consecutive rows with at most kCap in-edges and kRowsMax rows; a longer
+// row segment becomes a vector item, or split chunks (when allow_split) recorded in `splits`.
+void build_items(const uint32_t *ptr, uint32_t nrows, uint32_t slice, std::vector<uint4> &out,
+                 uint32_t &split_slots, std::map<uint32_t, SplitRec> *splits,
+                 uint64_t &maxlen) {
+    std::vector<uint4> split, vec, stream;
     uint32_t r = 0;
-    while (r < n) {
-        const uint64_t len = cp[r + 1] - cp[r];
-        maxdeg = std::max(maxdeg, len);
+    while (r < nrows) {
+        const uint64_t len = (uint64_t)ptr[r + 1] - ptr[r];
+        maxlen = std::max(maxlen, len);
         if (len > (uint64_t)kCap) {
-            if (len > kVectorMax) {
+            if (len > kVectorMax && splits) {
                 const uint32_t chunks = (uint32_t)((len + kChunk - 1) / kChunk);
-                srow.push_back(r);
-                sfirst.push_back((uint32_t)split.size());
-                scount.push_back(chunks);
-                for (uint32_t c = 0; c < chunks; ++c) split.push_back(make_uint2(r, c | kSplitFlag));
+                SplitRec &rec = (*splits)[r];
+                rec.first[slice] = split_slots;
+                rec.count[slice] = chunks;
+                for (uint32_t c = 0; c < chunks; ++c) {
+                    const uint64_t b = ptr[r] + (uint64_t)c * kChunk;
+                    const uint64_t e = std::min<uint64_t>(b + kChunk, ptr[r + 1]);
+                    split.push_back(make_uint4(r, split_slots++ | kSplitFlag, (uint32_t)b,
+                                               (uint32_t)(e - b)));
+                }
             } else {
-                vec.push_back(make_uint2(r, r + 1));
+                vec.push_back(make_uint4(r, r + 1, ptr[r], (uint32_t)len));
             }
             ++r;
             continue;
         }
         const uint32_t r0 = r;
         uint64_t nnz = 0;
-        while (r < n && r - r0 < (uint32_t)kRowsMax) {
-            const uint64_t l = cp[r + 1] - cp[r];
+        while (r < nrows && r - r0 < (uint32_t)kRowsMax) {
+            const uint64_t l = (uint64_t)ptr[r + 1] - ptr[r];
             if (l > (uint64_t)kCap || nnz + l > (uint64_t)kCap) break;
-            maxdeg = std::max(maxdeg, l);
+            maxlen = std::max(maxlen, l);
             nnz += l;
             ++r;
         }
-        stream.push_back(make_uint2(r0, r));
+        stream.push_back(make_uint4(r0, r, ptr[r0], (uint32_t)nnz));
     }
-    std::vector<uint2> items;
-    items.reserve(split.size() + vec.size() + stream.size());
-    items.insert(items.end(), split.begin(), split.end());
-    items.insert(items.end(), vec.begin(), vec.end());
-    items.insert(items.end(), stream.begin(), stream.end());
-    NKPR_CHECK(items.size() < 0x7FFFFFFFull, "too many work items");
-    g->n_items = (uint32_t)items.size();
-    g->n_split_items = (uint32_t)split.size();
-    g->n_split_rows = (uint32_t)srow.size();
+    out.insert(out.end(), split.begin(), split.end());
+    out.insert(out.end(), vec.begin(), vec.end());
+    out.insert(out.end(), stream.begin(), stream.end());
+}
+
+template <typename V> void upload(nkpr_graph *g, nkpr::DevBuf &b, const std::vector<V> &h) {
+    b.alloc(sizeof(V) * std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+        NKPR_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice,
+                                g->stream));
+}
+
+uint32_t env_u32(const char *name, uint32_t dflt) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return (uint32_t)std::strtoul(v, nullptr, 10);
+}
+
+// Splits rows into direct and sliced ones, reorders the in-edges into sweep order and builds
+// the work items.  Input: g->colptr / g->src / g->coef* in (row, source) order.
+void build_schedule(nkpr_graph *g, const std::vector<uint64_t> &cp) {
+    const hipStream_t s = g->stream;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    // slicing pays off once the gathered vector outgrows one L2; below that everything is
+    // direct.  NKPR_SLICES (1 or 8) and NKPR_DIRECT_MAX_DEG override the defaults.
+    uint32_t S = (n >= (1u << 20)) ? (uint32_t)kSlices : 1u;
+    S = env_u32("NKPR_SLICES", S);
+    NKPR_CHECK(S == 1 || S == (uint32_t)kSlices, "NKPR_SLICES must be 1 or 8");
+    uint32_t dmax = (S == 1) ? (uint32_t)kCap : kDefaultDirectMaxDeg;
+    dmax = std::min<uint32_t>(env_u32("NKPR_DIRECT_MAX_DEG", dmax), (uint32_t)kCap);
+    const uint32_t rep = env_u32("NKPR_SLICE_REPLICATE", kDefaultSliceReplicate);
+    g->n_slices = S;
+    g->direct_max_deg = dmax;
+
+    std::vector<uint32_t> drow, srow, dptr;
+    drow.reserve(n);
+    uint64_t md = 0, maxdeg = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint64_t len = cp[r + 1] - cp[r];
+        maxdeg = std::max(maxdeg, len);
+        if (len <= dmax) drow.push_back(r);
+        else srow.push_back(r);
+    }
     g->max_in_degree = maxdeg;
-    g->items.alloc(sizeof(uint2) * std::max<size_t>(items.size(), 1));
-    if (!items.empty())
-        NKPR_HIP(hipMemcpyAsync(g->items.p, items.data(), sizeof(uint2) * items.size(),
-                                hipMemcpyHostToDevice, g->stream));
-    g->split_rows.alloc(sizeof(uint32_t) * std::max<size_t>(srow.size(), 1));
-    g->split_first.alloc(sizeof(uint32_t) * std::max<size_t>(srow.size(), 1));
-    g->split_count.alloc(sizeof(uint32_t) * std::max<size_t>(srow.size(), 1));
-    if (!srow.empty()) {
-        NKPR_HIP(hipMemcpyAsync(g->split_rows.p, srow.data(), 4 * srow.size(),
-                                hipMemcpyHostToDevice, g->stream));
-        NKPR_HIP(hipMemcpyAsync(g->split_first.p, sfirst.data(), 4 * srow.size(),
-                                hipMemcpyHostToDevice, g->stream));
-        NKPR_HIP(hipMemcpyAsync(g->split_count.p, scount.data(), 4 * srow.size(),
-                                hipMemcpyHostToDevice, g->stream));
+    dptr.resize(drow.size() + 1);
+    for (size_t k = 0; k < drow.size(); ++k) {
+        dptr[k] = (uint32_t)md;
+        md += cp[drow[k] + 1] - cp[drow[k]];
     }
-    NKPR_HIP(hipStreamSynchronize(g->stream));
+    dptr[drow.size()] = (uint32_t)md;
+    g->n_d = (uint32_t)drow.size();
+    g->n_s = (uint32_t)srow.size();
+    const uint32_t n_s = g->n_s;
+    upload(g, g->drow, drow);
+    upload(g, g->dptr, dptr);
+    upload(g, g->srow, srow);
+
+    // per-slice offsets of the sliced rows: exclusive scan of the slice-major counts
+    const uint64_t ns_tot = (uint64_t)S * n_s;
+    NKPR_CHECK(ns_tot + 1 < 0xFFFFFFFFull, "too many sliced rows");
+    g->sptr.alloc(4 * (ns_tot + 1));
+    {
+        DevBuf counts, tmp;
+        counts.alloc(4 * (ns_tot + 1));
+        NKPR_HIP(hipMemsetAsync(counts.p, 0, 4 * (ns_tot + 1), s));
+        if (n_s) {
+            const unsigned blocks = (unsigned)std::min<uint64_t>(blocks_for((uint64_t)n_s * 64), 1u << 18);
+            hipLaunchKernelGGL(slice_count_kernel, dim3(blocks), dim3(256), 0, s,
+                               g->colptr.as<uint64_t>(), g->src.as<int32_t>(),
+                               g->srow.as<uint32_t>(), n_s, S, rep, counts.as<uint32_t>());
+            NKPR_HIP(hipGetLastError());
+        }
+        size_t tb = 0;
+        NKPR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts.as<uint32_t>(),
+                                                  g->sptr.as<uint32_t>(), (size_t)ns_tot + 1, s));
+        tmp.alloc(tb);
+        NKPR_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, counts.as<uint32_t>(),
+                                                  g->sptr.as<uint32_t>(), (size_t)ns_tot + 1, s));
+        hipLaunchKernelGGL(add_base_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(ns_tot + 1), 65536)),
+                           dim3(256), 0, s, g->sptr.as<uint32_t>(), ns_tot + 1, (uint32_t)md);
+        NKPR_HIP(hipGetLastError());
+        NKPR_HIP(hipStreamSynchronize(s));
+    }
+
+    // edges into sweep order (kCap zero entries of padding after the last one)
+    {
+        DevBuf src2, c32, c64;
+        src2.alloc(4 * (m + kCap));
+        NKPR_HIP(hipMemsetAsync(src2.as<int32_t>() + m, 0, 4 * kCap, s));
+        if (g->weighted) {
+            c32.alloc(4 * (m + kCap));
+            c64.alloc(8 * (m + kCap));
+            NKPR_HIP(hipMemsetAsync(c32.as<float>() + m, 0, 4 * kCap, s));
+            NKPR_HIP(hipMemsetAsync(c64.as<double>() + m, 0, 8 * kCap, s));
+        }
+        const float *o32 = g->weighted ? g->coef32.as<float>() : nullptr;
+        const double *o64 = g->weighted ? g->coef64.as<double>() : nullptr;
+        float *n32 = g->weighted ? c32.as<float>() : nullptr;
+        double *n64 = g->weighted ? c64.as<double>() : nullptr;
+        if (g->n_d) {
+            const unsigned blocks = (unsigned)std::min<uint64_t>(blocks_for((uint64_t)g->n_d * 64), 1u << 18);
+            hipLaunchKernelGGL(reorder_edges_kernel, dim3(blocks), dim3(256), 0, s,
+                               g->colptr.as<uint64_t>(), g->src.as<int32_t>(), o32, o64,
+                               g->drow.as<uint32_t>(), g->n_d, g->dptr.as<uint32_t>(), nullptr,
+                               0u, S, rep, src2.as<int32_t>(), n32, n64);
+            NKPR_HIP(hipGetLastError());
+        }
+        if (n_s) {
+            const unsigned blocks = (unsigned)std::min<uint64_t>(blocks_for((uint64_t)n_s * 64), 1u << 18);
+            hipLaunchKernelGGL(reorder_edges_kernel, dim3(blocks), dim3(256), 0, s,
+                               g->colptr.as<uint64_t>(), g->src.as<int32_t>(), o32, o64,
+                               g->srow.as<uint32_t>(), n_s, nullptr, g->sptr.as<uint32_t>(),
+                               n_s, S, rep, src2.as<int32_t>(), n32, n64);
+            NKPR_HIP(hipGetLastError());
+        }
+        NKPR_HIP(hipStreamSynchronize(s));
+        std::swap(g->src.p, src2.p);
+        std::swap(g->src.bytes, src2.bytes);
+        if (g->weighted) {
+            std::swap(g->coef32.p, c32.p);
+            std::swap(g->coef32.bytes, c32.bytes);
+            std::swap(g->coef64.p, c64.p);
+            std::swap(g->coef64.bytes, c64.bytes);
+        }
+    }
+
+    // work items
+    std::vector<uint4> ditems, sitems;
+    uint32_t split_slots = 0;
+    uint64_t maxlen = 0;
+    build_items(dptr.data(), g->n_d, 0, ditems, split_slots, nullptr, maxlen);
+    std::map<uint32_t, SplitRec> splits;
+    std::vector<uint32_t> hs((size_t)ns_tot + 1);
+    NKPR_HIP(hipMemcpy(hs.data(), g->sptr.p, 4 * (ns_tot + 1), hipMemcpyDeviceToHost));
+    for (uint32_t q = 0; q < S; ++q) {
+        g->sitem_off[q] = (uint32_t)sitems.size();
+        build_items(hs.data() + (size_t)q * n_s, n_s, q, sitems, split_slots, &splits, maxlen);
+    }
+    for (uint32_t q = S; q <= (uint32_t)kSlices; ++q) g->sitem_off[q] = (uint32_t)sitems.size();
+    NKPR_CHECK(sitems.size() + ditems.size() < 0x7FFFFFFFull, "too many work items");
+    g->n_ditems = (uint32_t)ditems.size();
+    g->n_items = (uint32_t)(ditems.size() + sitems.size());
+    g->n_split_items = split_slots;
+    upload(g, g->ditems, ditems);
+    upload(g, g->sitems, sitems);
+    std::vector<uint8_t> flag(std::max<uint32_t>(n_s, 1), 0);
+    std::vector<uint32_t> srows, sfirst, scount;
+    for (auto &kv : splits) {
+        flag[kv.first] = 1;
+        srows.push_back(kv.first);
+        for (uint32_t q = 0; q < S; ++q) {
+            sfirst.push_back(kv.second.first[q]);
+            scount.push_back(kv.second.count[q]);
+        }
+    }
+    g->n_split_rows = (uint32_t)srows.size();
+    upload(g, g->split_flag, flag);
+    upload(g, g->split_rows, srows);
+    upload(g, g->split_first, sfirst);
+    upload(g, g->split_count, scount);
+    g->claim.alloc(4 * kSlices);
+    NKPR_HIP(hipMemsetAsync(g->claim.p, 0, 4 * kSlices, s));
+    NKPR_HIP(hipStreamSynchronize(s));
+    if (env_u32("NKPR_DEBUG", 0)) {
+        std::fprintf(stderr, "[nkpr] n=%u direct rows %u (%llu edges, %u items), sliced rows %u, "
+                             "slices %u, split rows %u\n", n, g->n_d, (unsigned long long)md,
+                     g->n_ditems, n_s, S, g->n_split_rows);
+        for (uint32_t q = 0; q < S; ++q)
+            std::fprintf(stderr, "[nkpr]   slice %u: %llu edges, %u items\n", q,
+                         (unsigned long long)(hs[(size_t)(q + 1) * n_s] - hs[(size_t)q * n_s]),
+                         g->sitem_off[q + 1] - g->sitem_off[q]);
+    }
 }
 
 int bits_for(uint64_t v) {

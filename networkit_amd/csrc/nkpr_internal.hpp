@@ -13,11 +13,22 @@
 namespace nkpr {
 
 // ---------------------------------------------------------------------------------------
-// Sweep schedule constants.  A sweep block is 256 threads (4 wave64s).  A "stream" work item
-// is a run of consecutive rows whose in-edges (at most kCap) are staged as products in LDS;
-// a row with more than kCap in-edges is a "vector" item reduced by the whole block; a row
+// Sweep schedule.  A sweep block is 256 threads (4 wave64s).  A "stream" work item is a run
+// of consecutive rows whose in-edges (at most kCap) are staged as products in LDS; a row
+// segment with more than kCap in-edges is a "vector" item reduced by the whole block; one
 // with more than kVectorMax in-edges is cut into kChunk-sized "split" items whose partial
 // sums the finalize kernel adds in chunk order (deterministic).
+//
+// Source slicing.  Random gathers of the iterate are the cost of a sweep: an L2 miss moves a
+// 128-byte line for a 4- or 8-byte value.  The gathered node ids are therefore cut into
+// kSlices slices, interleaved in groups of 32 ids (one 128-byte line of FP32 values), and the
+// in-edges of every row with more than `direct_max_deg` in-edges are stored slice by slice.
+// Work on slice s is done by blocks running on XCD s (read from the hardware register
+// XCC_ID), so each XCD's L2 caches only the lines of its own slice: the eight L2s hold
+// distinct data instead of eight copies of the same hot lines.  The per-slice row sums are
+// added in slice order by a combine kernel, which also runs the row epilogue.  Rows with few
+// in-edges are processed unsliced ("direct") with the epilogue fused.  Which block serves
+// which slice affects only speed: any assignment gives the same result.
 // ---------------------------------------------------------------------------------------
 constexpr int kBlock = 256;
 constexpr int kCap = 2048;
@@ -26,6 +37,11 @@ constexpr uint32_t kVectorMax = 16384;
 constexpr uint32_t kChunk = 8192;
 constexpr uint32_t kSplitFlag = 0x80000000u;
 constexpr int kFinalizeThreads = 1024;
+constexpr int kSlices = 8;
+constexpr int kSliceShift = 5;
+constexpr uint32_t kDefaultDirectMaxDeg = 8;
+constexpr uint32_t kDefaultSliceReplicate = 32768;  // top-ranked sources cached by every XCD
+constexpr int kCombineThreads = 256;
 
 struct Error : std::runtime_error {
     int code;
@@ -104,10 +120,29 @@ struct PhaseRule {
     double damp;
 };
 
+// Device view of the sweep schedule (built once per graph).
+struct Schedule {
+    const int32_t *src;          // [m + kCap] sources: direct region, then slice regions
+    uint32_t n_slices;           // 1 or kSlices
+    // direct rows (in-degree <= direct_max_deg): epilogue fused into the sweep
+    const uint32_t *drow;        // [n_d] row ids
+    const uint32_t *dptr;        // [n_d + 1] absolute in-edge offsets
+    const uint4 *ditems;         // {k0, k1, e0, cnt} over direct index k
+    uint32_t n_d, n_ditems;
+    // sliced rows: per-slice partial sums, epilogue in the combine kernel
+    const uint32_t *srow;        // [n_s] row ids
+    const uint32_t *sptr;        // [n_slices * n_s + 1] slice-major in-edge offsets
+    const uint4 *sitems;         // per slice: split chunks, vector rows, stream blocks
+    uint32_t sitem_off[kSlices + 1];
+    uint32_t n_s;
+    const uint8_t *split_flag;   // [n_s] 1: the finalize kernel finishes this row
+    uint32_t *claim;             // [kSlices] worker claim counters (reset every sweep)
+    uint32_t workers;            // workers per slice; the sweep grid is n_slices * workers
+};
+
 template <typename T> struct SweepArgs {
-    const uint64_t *colptr;  // [n+1]
-    const int32_t *src;      // [m]
-    const T *coef;           // [m] folded w/deg(src) (weighted) or nullptr
+    Schedule sc;
+    const T *coef;           // [m + kCap] folded w/deg(src) (weighted) or nullptr
     const T *inv;            // [n] 1/deg (unweighted: prescale the new vector) or nullptr
     const T *xg;             // gather vector (x, or x*inv when prescaled)
     const T *xold;           // [n] previous iterate
@@ -115,10 +150,9 @@ template <typename T> struct SweepArgs {
     T *xhat_new;             // [n] prescaled new iterate (unweighted) or nullptr
     const T *base_vec;       // [n] per-node right-hand side (correction solves) or nullptr
     float *r32;              // [n] FP32 copy of x' - x (FP64 residual sweeps) or nullptr
-    const uint2 *items;      // work items (split chunks, vector rows, stream blocks)
-    uint32_t n_items;
-    T *split_partial;        // [n_split_items]
-    double *partials;        // [gridDim.x * 4]  (l1, l2, sum, sink) per block
+    T *partial;              // [n_slices * n_s] per-slice row sums of sliced rows
+    T *split_partial;        // [split chunks]
+    double *partials;        // [(sweep grid + combine grid) * 4]  (l1, l2, sum, sink)
     PhaseCtl *ctl;
     double damp;
     double tele;             // constant right-hand side per node ((1-d)/n, or 0)
@@ -128,11 +162,11 @@ template <typename T> struct SweepArgs {
 };
 
 struct FinalizeArgs {
-    const uint32_t *split_rows;
-    const uint32_t *split_first;
-    const uint32_t *split_count;
+    const uint32_t *split_rows;   // [n_split_rows] sliced-row index j
+    const uint32_t *split_first;  // [n_split_rows * n_slices] first chunk (count 0: partial)
+    const uint32_t *split_count;  // [n_split_rows * n_slices]
     uint32_t n_split_rows;
-    uint32_t n_blocks;
+    uint32_t n_blocks;            // sweep grid + combine grid
     PhaseRule rule;
 };
 
@@ -143,10 +177,12 @@ enum SweepKind : int {
     kSweepResid = 2,      // plain + r32 = float(x' - x)  (FP64 only)
 };
 
+// sweep (grid = n_slices * workers) + combine (combine_grid blocks, if n_s > 0) + finalize
 template <typename T>
 void launch_sweep(const SweepArgs<T> &a, const FinalizeArgs &f, int kind, bool folded,
-                  int grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
-int sweep_grid_size(int device, bool dbl, bool folded, int kind, uint32_t n_items);
+                  int combine_grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+// blocks of the sweep kernel that fit the device at once
+int sweep_resident_blocks(int device, bool dbl, bool folded, int kind);
 
 // Small vector kernels (nkpr_sweep.hip).
 void launch_gather_start(const double *scores_z, const int32_t *perm, uint32_t n,
@@ -165,7 +201,7 @@ void launch_range_sum(const T *x, uint32_t begin, uint32_t end, double *partials
                       double *out, hipStream_t s);
 void launch_range_max(const double *x, uint32_t n, double *partials, double *out,
                       hipStream_t s);
-void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, hipStream_t s);
+void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, uint32_t *claim, hipStream_t s);
 
 }  // namespace nkpr
 
@@ -180,18 +216,23 @@ struct nkpr_graph {
     double prep_ms = 0.0;
 
     nkpr::DevBuf perm;      // int32 [n]   compact id -> slot
-    nkpr::DevBuf colptr;    // uint64 [n+1]
-    nkpr::DevBuf src;       // int32 [m]
-    nkpr::DevBuf coef32;    // float [m]   (weighted)
-    nkpr::DevBuf coef64;    // double [m]  (weighted)
+    nkpr::DevBuf colptr;    // uint64 [n+1] in-lists in (row, source) order (before slicing)
+    nkpr::DevBuf src;       // int32 [m + kCap]   sweep order (direct region, slice regions)
+    nkpr::DevBuf coef32;    // float [m + kCap]   (weighted)
+    nkpr::DevBuf coef64;    // double [m + kCap]  (weighted)
     nkpr::DevBuf inv32;     // float [n]   (unweighted)
     nkpr::DevBuf inv64;     // double [n]  (unweighted)
     std::vector<uint8_t> exists_host;  // empty: every slot exists
 
     // schedule
-    nkpr::DevBuf items;     // uint2 [n_items]
+    uint32_t n_slices = 1, direct_max_deg = 0;
+    nkpr::DevBuf drow, dptr, ditems;
+    uint32_t n_d = 0, n_ditems = 0;
+    nkpr::DevBuf srow, sptr, sitems, split_flag, claim;
+    uint32_t sitem_off[nkpr::kSlices + 1] = {0};
+    uint32_t n_s = 0;
     uint32_t n_items = 0, n_split_items = 0;
-    nkpr::DevBuf split_rows, split_first, split_count;  // uint32 [n_split_rows]
+    nkpr::DevBuf split_rows, split_first, split_count;
     uint32_t n_split_rows = 0;
 
     // run workspace (lazily sized)
@@ -203,6 +244,7 @@ struct nkpr_graph {
     nkpr::DevBuf red_partials;          // double [1024]
     nkpr::DevBuf red_out;               // double [4]
     nkpr::DevBuf split_part32, split_part64;
+    nkpr::DevBuf part32, part64;        // per-slice row sums of sliced rows
     nkpr::DevBuf ctl;                   // PhaseCtl
     nkpr::PhaseCtl *ctl_host = nullptr; // pinned mirror
     std::vector<hipEvent_t> events;

@@ -18,6 +18,7 @@
 // into no-ops, so the host reads the state back only once per batch of sweeps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "nkpr_internal.hpp"
@@ -51,22 +52,29 @@ struct Acc4 {
     double l1 = 0.0, l2 = 0.0, sum = 0.0, sink = 0.0;
 };
 
-// Row epilogue: the fused "pr[u] = pr[u]*damp + teleport" plus everything the loop needs
-// from the new value (residual terms, score sum, sink mass, prescaled copy, FP32 residual).
+// Stores of a finished row value y (old value xo already loaded) and its reductions: the
+// fused "pr[u] = pr[u]*damp + teleport" result plus everything the loop needs from it
+// (residual terms, score sum, sink mass, prescaled copy, FP32 residual).
+template <typename T, bool RESID>
+__device__ __forceinline__ void epilogue_store(const SweepArgs<T> &a, uint32_t u, T y, T xo,
+                                               Acc4 &s) {
+    __builtin_nontemporal_store(y, a.xnew + u);
+    if (a.xhat_new != nullptr && u < a.sink_begin)
+        __builtin_nontemporal_store(y * __builtin_nontemporal_load(a.inv + u), a.xhat_new + u);
+    const double dif = (double)y - (double)xo;
+    if (RESID) __builtin_nontemporal_store((float)dif, a.r32 + u);
+    s.l1 += fabs(dif);
+    s.l2 += dif * dif;
+    s.sum += (double)y;
+    if (u >= a.sink_begin) s.sink += (double)y;
+}
+
 template <typename T, bool BASEVEC, bool RESID>
 __device__ __forceinline__ void row_epilogue(const SweepArgs<T> &a, uint32_t u, T acc, T damp,
                                              T c, Acc4 &s) {
     T y = damp * acc + c;
     if (BASEVEC) y += a.base_vec[u];
-    const T xo = a.xold[u];
-    a.xnew[u] = y;
-    if (a.xhat_new != nullptr && u < a.sink_begin) a.xhat_new[u] = y * a.inv[u];
-    const double dif = (double)y - (double)xo;
-    if (RESID) a.r32[u] = (float)dif;
-    s.l1 += fabs(dif);
-    s.l2 += dif * dif;
-    s.sum += (double)y;
-    if (u >= a.sink_begin) s.sink += (double)y;
+    epilogue_store<T, RESID>(a, u, y, a.xold[u], s);
 }
 
 template <typename T> __device__ __forceinline__ T rhs_const(const SweepArgs<T> &a) {
@@ -77,36 +85,146 @@ template <typename T> __device__ __forceinline__ T rhs_const(const SweepArgs<T> 
 
 constexpr int kPer = kCap / kBlock;  // staged in-edges per thread in a stream item
 
+// Streaming accesses (read once per sweep) are non-temporal so that they do not evict the
+// cache lines of the gathered vector from L2; gathers use ordinary cached loads.
+template <typename T> __device__ __forceinline__ T ld_stream(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// XCD (accelerator complex die) the calling wave runs on: hardware register XCC_ID, bits 3:0
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;
+}
+
 // Sum of the products of in-edges [e0, e0+cnt) by the whole block (vector rows and split
 // chunks).  Four independent gathers per thread per step keep enough loads in flight.
 template <typename T, bool FOLDED>
-__device__ __forceinline__ T block_row_sum(const SweepArgs<T> &a, uint64_t e0, uint32_t cnt,
+__device__ __forceinline__ T block_row_sum(const SweepArgs<T> &a, uint32_t e0, uint32_t cnt,
                                            T *red) {
     T acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
     const uint32_t tid = threadIdx.x;
+    const int32_t *__restrict__ srcb = a.sc.src + e0;
+    const T *__restrict__ cfb = FOLDED ? a.coef + e0 : nullptr;
     uint32_t i = tid;
     for (; i + 3 * kBlock < cnt; i += 4 * kBlock) {
-        const int32_t s0 = a.src[e0 + i], s1 = a.src[e0 + i + kBlock];
-        const int32_t s2 = a.src[e0 + i + 2 * kBlock], s3 = a.src[e0 + i + 3 * kBlock];
-        T v0 = a.xg[s0], v1 = a.xg[s1], v2 = a.xg[s2], v3 = a.xg[s3];
+        const int32_t s0 = ld_stream(srcb + i), s1 = ld_stream(srcb + i + kBlock);
+        const int32_t s2 = ld_stream(srcb + i + 2 * kBlock), s3 = ld_stream(srcb + i + 3 * kBlock);
+        T c0 = 1, c1 = 1, c2 = 1, c3 = 1;
         if (FOLDED) {
-            v0 *= a.coef[e0 + i];
-            v1 *= a.coef[e0 + i + kBlock];
-            v2 *= a.coef[e0 + i + 2 * kBlock];
-            v3 *= a.coef[e0 + i + 3 * kBlock];
+            c0 = ld_stream(cfb + i);
+            c1 = ld_stream(cfb + i + kBlock);
+            c2 = ld_stream(cfb + i + 2 * kBlock);
+            c3 = ld_stream(cfb + i + 3 * kBlock);
         }
-        acc0 += v0;
-        acc1 += v1;
-        acc2 += v2;
-        acc3 += v3;
+        acc0 += a.xg[s0] * c0;
+        acc1 += a.xg[s1] * c1;
+        acc2 += a.xg[s2] * c2;
+        acc3 += a.xg[s3] * c3;
     }
     for (; i < cnt; i += kBlock) {
-        T v = a.xg[a.src[e0 + i]];
-        if (FOLDED) v *= a.coef[e0 + i];
+        T v = a.xg[ld_stream(srcb + i)];
+        if (FOLDED) v *= ld_stream(cfb + i);
         acc0 += v;
     }
     T acc = (acc0 + acc1) + (acc2 + acc3);
     return block_sum<kBlock>(acc, red);
+}
+
+// One stream item: rows [r0, r0+R) of a compressed list with absolute in-edge offsets `ptr`,
+// in-edges [e0, e0+cnt) (cnt <= kCap).  DIRECT rows finish with the epilogue (row ids from
+// sc.drow); sliced rows store their row sums into `out` (one slice's partial vector).
+template <typename T, bool FOLDED, bool BASEVEC, bool RESID, bool DIRECT>
+__device__ __forceinline__ void stream_item(const SweepArgs<T> &a, const uint4 item,
+                                            const uint32_t *__restrict__ ptr, T *__restrict__ out,
+                                            T *s_prod, uint32_t *s_ptr, T *s_sum, T damp, T c,
+                                            Acc4 &st) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r0 = item.x, R = item.y - item.x, e0 = item.z, cnt = item.w;
+    // issue every load that does not depend on another load first
+    const uint32_t *__restrict__ pb = ptr + r0;
+    uint32_t p0 = 0, p1 = 0, p2 = 0;
+    if (tid <= R) p0 = ld_stream(pb + tid) - e0;
+    if (tid + kBlock <= R) p1 = ld_stream(pb + tid + kBlock) - e0;
+    if (tid == 0 && R == 2 * kBlock) p2 = ld_stream(pb + R) - e0;
+    uint32_t u0 = 0, u1 = 0;
+    if (DIRECT) {
+        if (tid < R) u0 = ld_stream(a.sc.drow + r0 + tid);
+        if (tid + kBlock < R) u1 = ld_stream(a.sc.drow + r0 + tid + kBlock);
+    }
+    int32_t sidx[kPer];
+    T cf[kPer];
+    {
+        // lane-predicated loads share one base address (immediate chunk offsets)
+        const int32_t *__restrict__ srcb = a.sc.src + e0;
+        const T *__restrict__ cfb = FOLDED ? a.coef + e0 : nullptr;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            sidx[k] = 0;
+            cf[k] = 0;
+            if (tid + k * kBlock < cnt) {
+                sidx[k] = ld_stream(srcb + tid + k * kBlock);
+                cf[k] = FOLDED ? ld_stream(cfb + tid + k * kBlock) : (T)1;
+            }
+        }
+    }
+    T xo0 = 0, xo1 = 0, bv0 = 0, bv1 = 0;
+    if (DIRECT) {
+        // folded: xold is also the gathered vector, keep its lines cacheable
+        if (tid < R) {
+            xo0 = FOLDED ? a.xold[u0] : ld_stream(a.xold + u0);
+            if (BASEVEC) bv0 = ld_stream(a.base_vec + u0);
+        }
+        if (tid + kBlock < R) {
+            xo1 = FOLDED ? a.xold[u1] : ld_stream(a.xold + u1);
+            if (BASEVEC) bv1 = ld_stream(a.base_vec + u1);
+        }
+    }
+    T v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) v[k] = a.xg[sidx[k]] * cf[k];
+    if (tid <= R) s_ptr[tid] = p0;
+    if (tid + kBlock <= R) s_ptr[tid + kBlock] = p1;
+    if (tid == 0 && R == 2 * kBlock) s_ptr[R] = p2;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = tid + k * kBlock;
+        if (i < cnt) s_prod[i] = v[k];
+    }
+    __syncthreads();
+    // row sums: G consecutive lanes per row, G ~ average row length (power of two)
+    const uint32_t avg = cnt / R;
+    const int lg = avg >= 64 ? 6 : (avg >= 32 ? 5 : (avg >= 16 ? 4 : (avg >= 8 ? 3 : (avg >= 4 ? 2 : (avg >= 2 ? 1 : 0)))));
+    const uint32_t G = 1u << lg;
+    const uint32_t per_pass = kBlock >> lg;
+    const uint32_t lane = tid & (G - 1);
+    for (uint32_t base = 0; base < R; base += per_pass) {
+        const uint32_t r = base + (tid >> lg);
+        T acc = 0;
+        if (r < R) {
+            const uint32_t b = s_ptr[r], e = s_ptr[r + 1];
+            for (uint32_t i = b + lane; i < e; i += G) acc += s_prod[i];
+        }
+        for (uint32_t off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (r < R && lane == 0) s_sum[r] = acc;
+    }
+    __syncthreads();
+    // one thread per row; the LDS buffers are rewritten only after the next item's first
+    // barrier, which every thread reaches after this point
+    if (DIRECT) {
+        if (tid < R) {
+            T y = damp * s_sum[tid] + c;
+            if (BASEVEC) y += bv0;
+            epilogue_store<T, RESID>(a, u0, y, xo0, st);
+        }
+        if (tid + kBlock < R) {
+            T y = damp * s_sum[tid + kBlock] + c;
+            if (BASEVEC) y += bv1;
+            epilogue_store<T, RESID>(a, u1, y, xo1, st);
+        }
+    } else {
+        if (tid < R) __builtin_nontemporal_store(s_sum[tid], out + r0 + tid);
+        if (tid + kBlock < R) __builtin_nontemporal_store(s_sum[tid + kBlock], out + r0 + tid + kBlock);
+    }
 }
 
 template <typename T, bool FOLDED, bool BASEVEC, bool RESID>
@@ -118,79 +236,74 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
     __shared__ T s_sum[kRowsMax];
     __shared__ T s_redT[kBlock / 64];
     __shared__ double s_red[kBlock / 64];
+    __shared__ uint32_t s_worker;
 
     const T damp = (T)a.damp;
     const T c = rhs_const(a);
     const uint32_t tid = threadIdx.x;
+    const uint32_t S = a.sc.n_slices, K = a.sc.workers;
     Acc4 st;
 
-    for (uint32_t it = blockIdx.x; it < a.n_items; it += gridDim.x) {
-        const uint2 item = a.items[it];
-        if (item.y & kSplitFlag) {
-            // one chunk of a row that is too long for one block
-            const uint32_t row = item.x, chunk = item.y & ~kSplitFlag;
-            const uint64_t rb = a.colptr[row], re = a.colptr[row + 1];
-            const uint64_t e0 = rb + (uint64_t)chunk * kChunk;
-            const uint64_t e1 = (e0 + kChunk < re) ? e0 + kChunk : re;
-            const T part = block_row_sum<T, FOLDED>(a, e0, (uint32_t)(e1 - e0), s_redT);
-            if (tid == 0) a.split_partial[it] = part;
-            continue;
-        }
-        const uint32_t r0 = item.x, r1 = item.y, R = r1 - r0;
-        const uint64_t e0 = a.colptr[r0];
-        const uint32_t cnt = (uint32_t)(a.colptr[r1] - e0);
-        if (R == 1 && cnt > (uint32_t)kCap) {
-            // a heavy row: the whole block reduces it
-            const T acc = block_row_sum<T, FOLDED>(a, e0, cnt, s_redT);
-            if (tid == 0) row_epilogue<T, BASEVEC, RESID>(a, r0, acc, damp, c, st);
-            continue;
-        }
-        // stream item: stage the products of all in-edges of rows [r0, r1) in LDS
-        for (uint32_t i = tid; i <= R; i += kBlock) s_ptr[i] = (uint32_t)(a.colptr[r0 + i] - e0);
-        if (cnt > 0) {
-            int32_t sidx[kPer];
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const uint32_t i = tid + k * kBlock;
-                sidx[k] = a.src[e0 + (i < cnt ? i : cnt - 1)];
-            }
-            T v[kPer];
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) v[k] = a.xg[sidx[k]];
-            if (FOLDED) {
-#pragma unroll
-                for (int k = 0; k < kPer; ++k) {
-                    const uint32_t i = tid + k * kBlock;
-                    v[k] *= a.coef[e0 + (i < cnt ? i : cnt - 1)];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const uint32_t i = tid + k * kBlock;
-                if (i < cnt) s_prod[i] = v[k];
+    // claim a worker (slice, rank), trying the slice of this block's XCD first.  Each slice
+    // hands out exactly K ranks and the grid has S*K blocks, so every block gets one.
+    if (tid == 0) {
+        const uint32_t first = xcc_id() % S;
+        uint32_t w = 0;
+        for (uint32_t k = 0; k < S; ++k) {
+            const uint32_t sl = (first + k) % S;
+            const uint32_t r = atomicAdd(a.sc.claim + sl, 1u);
+            if (r < K) {
+                w = sl * K + r;
+                break;
             }
         }
-        __syncthreads();
-        // row sums: G consecutive lanes per row, G ~ average row length (power of two <= 64)
-        const uint32_t avg = R ? cnt / R : 0;
-        const int lg = avg >= 64 ? 6 : (avg >= 32 ? 5 : (avg >= 16 ? 4 : (avg >= 8 ? 3 : (avg >= 4 ? 2 : (avg >= 2 ? 1 : 0)))));
-        const uint32_t G = 1u << lg;
-        const uint32_t per_pass = kBlock >> lg;
-        const uint32_t lane = tid & (G - 1);
-        for (uint32_t base = 0; base < R; base += per_pass) {
-            const uint32_t r = base + (tid >> lg);
-            T acc = 0;
-            if (r < R) {
-                const uint32_t b = s_ptr[r], e = s_ptr[r + 1];
-                for (uint32_t i = b + lane; i < e; i += G) acc += s_prod[i];
+        s_worker = w;
+    }
+    __syncthreads();
+    const uint32_t w = s_worker;
+    const uint32_t sl = w / K, rank = w - sl * K;
+
+    // 1) this slice's items (row sums into the slice's partial vector)
+    {
+        T *__restrict__ part = a.partial + (size_t)sl * a.sc.n_s;
+        const uint32_t *__restrict__ ptr = a.sc.sptr + (size_t)sl * a.sc.n_s;
+        const uint32_t ie = a.sc.sitem_off[sl + 1];
+        uint32_t it = a.sc.sitem_off[sl] + rank;
+        uint4 item = make_uint4(0, 0, 0, 0);
+        if (it < ie) item = a.sc.sitems[it];
+        while (it < ie) {
+            const uint32_t nit = it + K;
+            uint4 next = make_uint4(0, 0, 0, 0);
+            if (nit < ie) next = a.sc.sitems[nit];
+            if (item.y & kSplitFlag) {
+                const T p = block_row_sum<T, FOLDED>(a, item.z, item.w, s_redT);
+                if (tid == 0) a.split_partial[item.y & ~kSplitFlag] = p;
+            } else if (item.y - item.x == 1 && item.w > (uint32_t)kCap) {
+                const T p = block_row_sum<T, FOLDED>(a, item.z, item.w, s_redT);
+                if (tid == 0) part[item.x] = p;
+            } else {
+                stream_item<T, FOLDED, BASEVEC, RESID, false>(a, item, ptr, part, s_prod, s_ptr,
+                                                              s_sum, damp, c, st);
             }
-            for (uint32_t off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-            if (r < R && lane == 0) s_sum[r] = acc;
+            item = next;
+            it = nit;
         }
-        __syncthreads();
-        for (uint32_t r = tid; r < R; r += kBlock)
-            row_epilogue<T, BASEVEC, RESID>(a, r0 + r, s_sum[r], damp, c, st);
-        __syncthreads();
+    }
+    // 2) direct rows, spread over all workers
+    {
+        const uint32_t nw = S * K;
+        uint32_t it = w;
+        uint4 item = make_uint4(0, 0, 0, 0);
+        if (it < a.sc.n_ditems) item = a.sc.ditems[it];
+        while (it < a.sc.n_ditems) {
+            const uint32_t nit = it + nw;
+            uint4 next = make_uint4(0, 0, 0, 0);
+            if (nit < a.sc.n_ditems) next = a.sc.ditems[nit];
+            stream_item<T, FOLDED, BASEVEC, RESID, true>(a, item, a.sc.dptr, nullptr, s_prod,
+                                                         s_ptr, s_sum, damp, c, st);
+            item = next;
+            it = nit;
+        }
     }
 
     // per-block partials (l1, l2, sum, sink) in FP64
@@ -205,7 +318,39 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
     if (tid == 0) a.partials[blockIdx.x * 4 + 3] = v;
 }
 
-// One block: finish split rows, add all partials in a fixed order, apply the stopping rule.
+// Sliced rows: add the per-slice row sums in slice order and run the row epilogue.  Rows
+// with split segments are left to the finalize kernel.
+template <typename T, bool BASEVEC, bool RESID>
+__global__ __launch_bounds__(kCombineThreads) void combine_kernel(SweepArgs<T> a,
+                                                                  uint32_t block_base) {
+    if (a.ctl->done) return;
+    __shared__ double s_red[kCombineThreads / 64];
+    const T damp = (T)a.damp;
+    const T c = rhs_const(a);
+    const uint32_t S = a.sc.n_slices, ns = a.sc.n_s;
+    Acc4 st;
+    for (uint32_t j = blockIdx.x * kCombineThreads + threadIdx.x; j < ns;
+         j += gridDim.x * kCombineThreads) {
+        if (a.sc.split_flag[j]) continue;
+        T acc = 0;
+        for (uint32_t s = 0; s < S; ++s) acc += ld_stream(a.partial + (size_t)s * ns + j);
+        const uint32_t u = ld_stream(a.sc.srow + j);
+        row_epilogue<T, BASEVEC, RESID>(a, u, acc, damp, c, st);
+    }
+    double v;
+    const uint32_t b = block_base + blockIdx.x;
+    v = block_sum<kCombineThreads>(st.l1, s_red);
+    if (threadIdx.x == 0) a.partials[b * 4 + 0] = v;
+    v = block_sum<kCombineThreads>(st.l2, s_red);
+    if (threadIdx.x == 0) a.partials[b * 4 + 1] = v;
+    v = block_sum<kCombineThreads>(st.sum, s_red);
+    if (threadIdx.x == 0) a.partials[b * 4 + 2] = v;
+    v = block_sum<kCombineThreads>(st.sink, s_red);
+    if (threadIdx.x == 0) a.partials[b * 4 + 3] = v;
+}
+
+// One block: finish rows with split segments, add all partials in a fixed order, apply the
+// stopping rule, reset the worker claim counters for the next sweep.
 template <typename T, bool BASEVEC, bool RESID>
 __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(SweepArgs<T> a,
                                                                     FinalizeArgs f) {
@@ -214,12 +359,21 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(SweepArgs<T>
     __shared__ double s_red[kFinalizeThreads / 64];
     const T damp = (T)a.damp;
     const T c = rhs_const(a);
+    const uint32_t S = a.sc.n_slices, ns = a.sc.n_s;
     Acc4 st;
-    for (uint32_t j = threadIdx.x; j < f.n_split_rows; j += kFinalizeThreads) {
+    for (uint32_t t = threadIdx.x; t < f.n_split_rows; t += kFinalizeThreads) {
+        const uint32_t j = f.split_rows[t];
         T acc = 0;
-        const uint32_t first = f.split_first[j], count = f.split_count[j];
-        for (uint32_t k = 0; k < count; ++k) acc += a.split_partial[first + k];
-        row_epilogue<T, BASEVEC, RESID>(a, f.split_rows[j], acc, damp, c, st);
+        for (uint32_t s = 0; s < S; ++s) {
+            const uint32_t cnt = f.split_count[t * S + s];
+            if (cnt == 0) {
+                acc += a.partial[(size_t)s * ns + j];
+            } else {
+                const uint32_t first = f.split_first[t * S + s];
+                for (uint32_t k = 0; k < cnt; ++k) acc += a.split_partial[first + k];
+            }
+        }
+        row_epilogue<T, BASEVEC, RESID>(a, a.sc.srow[j], acc, damp, c, st);
     }
     // split-row terms first (in row order), then block partials in block order
     double l1 = st.l1, l2 = st.l2, sum = st.sum, sink = st.sink;
@@ -258,10 +412,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(SweepArgs<T>
             if (it >= 4 && l1 > r.stall_ratio * prev) stop = true;
         }
         ctl->done = stop ? 1u : 0u;
+        for (int s = 0; s < kSlices; ++s) a.sc.claim[s] = 0u;
     }
 }
 
-__global__ void init_ctl_kernel(PhaseCtl *ctl, const double *sink_sum) {
+__global__ void init_ctl_kernel(PhaseCtl *ctl, const double *sink_sum, uint32_t *claim) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->sink_in = sink_sum ? *sink_sum : 0.0;
         ctl->l1 = INFINITY;
@@ -273,6 +428,7 @@ __global__ void init_ctl_kernel(PhaseCtl *ctl, const double *sink_sum) {
         ctl->iterations = 0;
         ctl->done = 0;
         ctl->pad = 0;
+        for (int s = 0; s < kSlices; ++s) claim[s] = 0u;
     }
 }
 
@@ -382,12 +538,18 @@ inline unsigned grid1d(uint64_t n, int threads = 256) {
 }
 
 template <typename T, bool FOLDED, bool BASEVEC, bool RESID>
-void launch_pair(const SweepArgs<T> &a, const FinalizeArgs &f, int grid, hipStream_t s,
+void launch_pair(const SweepArgs<T> &a, const FinalizeArgs &f, int combine_grid, hipStream_t s,
                  hipEvent_t ev0, hipEvent_t ev1) {
+    const unsigned grid = a.sc.n_slices * a.sc.workers;
     if (ev0) NKPR_HIP(hipEventRecord(ev0, s));
     hipLaunchKernelGGL((sweep_kernel<T, FOLDED, BASEVEC, RESID>), dim3(grid), dim3(kBlock), 0, s,
                        a);
     NKPR_HIP(hipGetLastError());
+    if (a.sc.n_s > 0 && combine_grid > 0) {
+        hipLaunchKernelGGL((combine_kernel<T, BASEVEC, RESID>), dim3(combine_grid),
+                           dim3(kCombineThreads), 0, s, a, grid);
+        NKPR_HIP(hipGetLastError());
+    }
     if (ev1) NKPR_HIP(hipEventRecord(ev1, s));
     hipLaunchKernelGGL((finalize_kernel<T, BASEVEC, RESID>), dim3(1), dim3(kFinalizeThreads), 0,
                        s, a, f);
@@ -402,17 +564,17 @@ const void *sweep_fn() {
 }  // namespace
 
 template <typename T>
-void launch_sweep(const SweepArgs<T> &a, const FinalizeArgs &f, int kind, bool folded, int grid,
-                  hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+void launch_sweep(const SweepArgs<T> &a, const FinalizeArgs &f, int kind, bool folded,
+                  int combine_grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     if (kind == kSweepPlain) {
-        if (folded) launch_pair<T, true, false, false>(a, f, grid, s, ev0, ev1);
-        else launch_pair<T, false, false, false>(a, f, grid, s, ev0, ev1);
+        if (folded) launch_pair<T, true, false, false>(a, f, combine_grid, s, ev0, ev1);
+        else launch_pair<T, false, false, false>(a, f, combine_grid, s, ev0, ev1);
     } else if (kind == kSweepBaseVec) {
-        if (folded) launch_pair<T, true, true, false>(a, f, grid, s, ev0, ev1);
-        else launch_pair<T, false, true, false>(a, f, grid, s, ev0, ev1);
+        if (folded) launch_pair<T, true, true, false>(a, f, combine_grid, s, ev0, ev1);
+        else launch_pair<T, false, true, false>(a, f, combine_grid, s, ev0, ev1);
     } else {
-        if (folded) launch_pair<T, true, false, true>(a, f, grid, s, ev0, ev1);
-        else launch_pair<T, false, false, true>(a, f, grid, s, ev0, ev1);
+        if (folded) launch_pair<T, true, false, true>(a, f, combine_grid, s, ev0, ev1);
+        else launch_pair<T, false, false, true>(a, f, combine_grid, s, ev0, ev1);
     }
 }
 
@@ -421,7 +583,7 @@ template void launch_sweep<float>(const SweepArgs<float> &, const FinalizeArgs &
 template void launch_sweep<double>(const SweepArgs<double> &, const FinalizeArgs &, int, bool,
                                    int, hipStream_t, hipEvent_t, hipEvent_t);
 
-int sweep_grid_size(int device, bool dbl, bool folded, int kind, uint32_t n_items) {
+int sweep_resident_blocks(int device, bool dbl, bool folded, int kind) {
     const void *fn;
     if (dbl) {
         if (kind == kSweepResid) fn = folded ? sweep_fn<double, true, false, true>() : sweep_fn<double, false, false, true>();
@@ -436,10 +598,7 @@ int sweep_grid_size(int device, bool dbl, bool folded, int kind, uint32_t n_item
     int cus = 0;
     NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (per_cu < 1) per_cu = 1;
-    uint64_t g = (uint64_t)per_cu * (uint64_t)cus;
-    if (g > n_items) g = n_items;
-    if (g < 1) g = 1;
-    return (int)g;
+    return per_cu * std::max(cus, 1);
 }
 
 void launch_gather_start(const double *scores_z, const int32_t *perm, uint32_t n, double *x64,
@@ -510,8 +669,8 @@ void launch_range_max(const double *x, uint32_t n, double *partials, double *out
     NKPR_HIP(hipGetLastError());
 }
 
-void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, hipStream_t s) {
-    hipLaunchKernelGGL(init_ctl_kernel, dim3(1), dim3(64), 0, s, ctl, sink_sum);
+void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, uint32_t *claim, hipStream_t s) {
+    hipLaunchKernelGGL(init_ctl_kernel, dim3(1), dim3(64), 0, s, ctl, sink_sum, claim);
     NKPR_HIP(hipGetLastError());
 }
 

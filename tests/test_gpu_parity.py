@@ -147,6 +147,44 @@ def test_skewed_directed_weighted():
     check_fp64_parity(z, u, v, w, True)
 
 
+@pytest.fixture
+def sliced(monkeypatch):
+    """Force the source-sliced schedule (8 slices, rows with > 2 in-edges sliced) on small
+    graphs; by default it is used only from 2^20 nodes up."""
+    monkeypatch.setenv("NKPR_SLICES", "8")
+    monkeypatch.setenv("NKPR_DIRECT_MAX_DEG", "2")
+
+
+@pytest.mark.parametrize("normalized", [False, True])
+def test_sliced_schedule_parity(sliced, normalized):
+    z, u, v, w = skewed_graph(30000, 400000, 5)
+    check_fp64_parity(z, u, v, w, True, normalized=normalized)
+    check_fp64_parity(z, u, v, None, False, sinks=1, norm=0)
+    # heavy rows: split chunks and vector segments inside slices
+    rng = np.random.default_rng(3)
+    z = 200000
+    u = np.concatenate([np.arange(1, 150001), rng.integers(0, z, 20000), rng.integers(0, z, 300000)])
+    v = np.concatenate([np.zeros(150000, np.int64), np.full(20000, 7), rng.integers(0, z, 300000)])
+    w = rng.uniform(0.5, 1.5, len(u))
+    check_fp64_parity(z, u, v, w, True, norm=0)
+
+
+def test_sliced_mixed_precision(sliced):
+    g = gen.rmat(15, 16, seed=2)
+    z = g["z"]
+    rows = np.repeat(np.arange(z), np.diff(g["colptr"].astype(np.int64)))
+    w = g["weight"].astype(np.float64)
+    G = build_graph(z, g["src"], rows, w, True)
+    ref, _, _ = oracle.pagerank(oracle_graph(z, g["src"], rows, w, True), tol=1e-15, norm=0,
+                                max_iterations=3000)
+    pr = nk.centrality.PageRank(G, 0.85, 1e-8)
+    pr.precision = nk.centrality.Precision.Mixed
+    pr.run()
+    got = np.array(pr.scores())
+    assert np.abs(got - ref).sum() <= 1e-8
+    assert_same_ranking(ref, got, top=1000)
+
+
 def test_deleted_nodes_self_loops_multi_edges():
     z, u, v, w = random_graph(3000, 12000, 9, directed=True, weighted=True)
     for directed in (True, False):
