@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -64,6 +65,14 @@ template <typename T> struct Vecs {
 };
 
 void ensure(DevBuf &b, size_t bytes) { b.alloc(bytes); }
+
+constexpr int kMaxRefinements = 30;
+
+uint32_t env_u32(const char *name, uint32_t dflt) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return (uint32_t)std::strtoul(v, nullptr, 10);
+}
 
 struct RunCtx {
     nkpr_graph *g;
@@ -318,33 +327,106 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
         st->last_step = (p->norm == NKPR_L2) ? std::sqrt(g->ctl_host->l2) : g->ctl_host->l1;
         st->l1_error_bound = rel_error_bound(damp, g->ctl_host->l1, g->ctl_host->sum);
     } else {
-        // ---- mixed precision: FP32 sweeps, then FP64 residual sweeps + FP32 corrections ----
-        alloc32();
+        // ---- mixed precision: FP64 residual sweeps certify, FP32 GMRES solves corrections ----
+        // With r = b - A x from an FP64 sweep (A = I - d P, P with the sink term when sinks
+        // are distributed), A e = r is solved approximately in FP32 by GMRES(m) and x += e in
+        // FP64.  Only FP64 quantities enter the certificate 2 d/(1-d) ||r||_1 / sum(x + r).
         alloc64();
         ensure(g->r32, 4 * (size_t)n);
-        v32 = {{g->x32a.as<float>(), g->x32b.as<float>()}, {g->xh32a.as<float>(), g->xh32b.as<float>()}};
+        const int mres = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 20), 2), 64);
+        ensure(g->kbasis, 4 * (size_t)(mres + 1) * n);
+        ensure(g->kwork, 4 * (size_t)4 * n);
+        ensure(g->kdot, 8 * ((size_t)kDotGroup * 1024 + 128));
+        ensure(g->x32b, 4 * (size_t)n);
         double *xcur = g->x64a.as<double>();   // current FP64 iterate
         double *ynew = g->x64b.as<double>();   // FP64 sweep output
         double *xh = g->xh64a.as<double>();
-        launch_gather_start(sz, g->perm.as<int32_t>(), n, nullptr, v32.x[0], s);
-        if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
-        init_phase(v32.x[0], false);
-        PhaseRule r1{};
-        r1.kind = kRuleAbsStall;
-        r1.norm = NKPR_L1;
-        r1.tol = std::max(p->tol, 1e-4);  // stop the plain FP32 phase at a 1e-4 error bound
-        r1.tol_abs = 0.0;
-        r1.stall_ratio = 0.95;
-        r1.damp = damp;
-        const uint64_t cap1 = (max_it > 1) ? max_it - 1 : 0;  // keep one sweep for FP64
-        uint64_t k1 = 0;
-        if (cap1 > 0) {
-            r1.max_iterations = cap1;
-            k1 = run_phase<float>(rc, base_args32(), v32, kSweepPlain, r1, cap1, timing, &st->ms_sweeps_fp32);
-        }
-        st->sweeps_fp32 += k1;
-        total += k1;
-        launch_to_double(v32.x[k1 & 1], xcur, n, s);
+        float *V = g->kbasis.as<float>();
+        float *ky = g->kwork.as<float>();            // y = d P v
+        float *kvh = ky + n;                         // prescaled v (unweighted)
+        float *kyh = ky + 2 * (size_t)n;             // prescaled y (written, unused)
+        float *ke = ky + 3 * (size_t)n;              // correction e
+        double *dpart = g->kdot.as<double>();
+        double *dout = dpart + (size_t)kDotGroup * 1024;
+        launch_gather_start(sz, g->perm.as<int32_t>(), n, xcur, nullptr, s);
+
+        auto dots = [&](const float *w, const float *Vb, uint32_t cnt, double *host) {
+            launch_multidot(w, Vb, n, cnt, n, dpart, dout, s);
+            NKPR_HIP(hipMemcpyAsync(host, dout, 8 * cnt, hipMemcpyDeviceToHost, s));
+            NKPR_HIP(hipStreamSynchronize(s));
+        };
+        // w = A v = v - d P v  (one FP32 sweep with no teleport term)
+        auto apply = [&](const float *v, float *w) {
+            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kvh, n, g->sink_begin, s);
+            init_phase(v, false);
+            SweepArgs<float> a = base_args32();
+            a.tele = 0.0;
+            Vecs<float> vv{{const_cast<float *>(v), ky}, {kvh, kyh}};
+            PhaseRule r{};
+            r.kind = kRuleStep;
+            r.norm = NKPR_L1;
+            r.tol = -1.0;
+            r.max_iterations = 1;
+            r.damp = damp;
+            run_phase<float>(rc, a, vv, kSweepPlain, r, 1, timing, &st->ms_sweeps_fp32);
+            launch_axpby(v, ky, w, n, 1.0f, -1.0f, s);
+        };
+        // GMRES(mres) from e = 0 on A e = rhs until ||A e - rhs||_2 <= rtol ||rhs||_2
+        auto gmres = [&](const float *rhs, float *e, double rtol, uint64_t cap) -> uint64_t {
+            double beta2 = 0.0;
+            dots(rhs, rhs, 1, &beta2);
+            const double beta = std::sqrt(beta2);
+            if (!(beta > 0.0) || cap == 0) {
+                launch_axpby(rhs, nullptr, e, n, 0.0f, 0.0f, s);
+                return 0;
+            }
+            launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
+            std::vector<double> R((size_t)mres * mres, 0.0), cs(mres), sn(mres), gv(mres + 1, 0.0);
+            gv[0] = beta;
+            uint64_t applies = 0;
+            int k = 0;
+            for (int j = 0; j < mres && applies < cap; ++j) {
+                float *vj = V + (size_t)j * n;
+                float *w = V + (size_t)(j + 1) * n;
+                apply(vj, w);
+                ++applies;
+                std::vector<double> h(j + 2), h2(j + 1);
+                // classical Gram-Schmidt, applied twice
+                dots(w, V, j + 1, h.data());
+                launch_multiaxpy(w, V, n, j + 1, n, h.data(), false, s);
+                dots(w, V, j + 1, h2.data());
+                launch_multiaxpy(w, V, n, j + 1, n, h2.data(), false, s);
+                for (int i = 0; i <= j; ++i) h[i] += h2[i];
+                double hn2 = 0.0;
+                dots(w, w, 1, &hn2);
+                const double hn = std::sqrt(hn2);
+                h[j + 1] = hn;
+                for (int i = 0; i < j; ++i) {
+                    const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+                    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                    h[i] = t;
+                }
+                const double den = std::hypot(h[j], h[j + 1]);
+                cs[j] = den > 0.0 ? h[j] / den : 1.0;
+                sn[j] = den > 0.0 ? h[j + 1] / den : 0.0;
+                h[j] = den;
+                gv[j + 1] = -sn[j] * gv[j];
+                gv[j] = cs[j] * gv[j];
+                for (int i = 0; i <= j; ++i) R[(size_t)i * mres + j] = h[i];
+                k = j + 1;
+                if (hn > 0.0) launch_axpby(w, nullptr, w, n, (float)(1.0 / hn), 0.0f, s);
+                if (std::fabs(gv[j + 1]) <= rtol * beta || !(hn > 0.0)) break;
+            }
+            std::vector<double> y(k, 0.0);
+            for (int i = k - 1; i >= 0; --i) {
+                double t = gv[i];
+                for (int q = i + 1; q < k; ++q) t -= R[(size_t)i * mres + q] * y[q];
+                y[i] = R[(size_t)i * mres + i] != 0.0 ? t / R[(size_t)i * mres + i] : 0.0;
+            }
+            launch_multiaxpy(e, V, n, k, n, y.data(), true, s);
+            return applies;
+        };
+
         for (int round = 0;; ++round) {
             // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
             if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
@@ -368,31 +450,17 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
             res_sum = c.sum;
             res_sink = c.sink_out;
             result = ynew;
-            if (c.bound <= p->tol || total >= max_it || round >= 8) break;
-            // FP32 correction solve: e = d P e + r  (e_0 = r)
+            if (c.bound <= p->tol || total >= max_it || round >= kMaxRefinements) break;
+            // FP32 correction: A e = r, tightened so that the next residual certifies
             ++st->refinements;
-            launch_copy_f32(g->r32.as<float>(), v32.x[0], n, s);
-            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
-            init_phase(v32.x[0], false);
-            PhaseRule rc2{};
-            rc2.kind = kRuleAbsStall;
-            rc2.norm = NKPR_L1;
-            rc2.tol = -1.0;
-            rc2.tol_abs = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
-            rc2.stall_ratio = 0.95;
-            rc2.damp = damp;
+            const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
+            const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
             const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
-            const uint64_t cap = std::min<uint64_t>(left, 4000);
-            if (cap == 0) break;
-            rc2.max_iterations = cap;
-            SweepArgs<float> a32 = base_args32();
-            a32.tele = 0.0;
-            a32.base_vec = g->r32.as<float>();
-            const uint64_t kc = run_phase<float>(rc, a32, v32, kSweepBaseVec, rc2, cap, timing, &st->ms_sweeps_fp32);
+            const uint64_t kc = gmres(g->r32.as<float>(), ke, rtol, left);
             st->sweeps_fp32 += kc;
             st->sweeps_fp32_correction += kc;
             total += kc;
-            launch_add_correction(xcur, v32.x[kc & 1], n, s);
+            launch_add_correction(xcur, ke, n, s);
         }
     }
     st->iterations = total;

@@ -13,7 +13,7 @@
 namespace nkpr {
 
 // ---------------------------------------------------------------------------------------
-// Sweep schedule.  A sweep block is 256 threads (4 wave64s).  A "stream" work item is a run
+// Sweep schedule.  A sweep block is 512 threads (8 wave64s).  A "stream" work item is a run
 // of consecutive rows whose in-edges (at most kCap) are staged as products in LDS; a row
 // segment with more than kCap in-edges is a "vector" item reduced by the whole block; one
 // with more than kVectorMax in-edges is cut into kChunk-sized "split" items whose partial
@@ -30,8 +30,8 @@ namespace nkpr {
 // in-edges are processed unsliced ("direct") with the epilogue fused.  Which block serves
 // which slice affects only speed: any assignment gives the same result.
 // ---------------------------------------------------------------------------------------
-constexpr int kBlock = 256;
-constexpr int kCap = 2048;
+constexpr int kBlock = 512;
+constexpr int kCap = 4096;
 constexpr int kRowsMax = 512;
 constexpr uint32_t kVectorMax = 16384;
 constexpr uint32_t kChunk = 8192;
@@ -42,6 +42,10 @@ constexpr int kSliceShift = 5;
 constexpr uint32_t kDefaultDirectMaxDeg = 8;
 constexpr uint32_t kDefaultSliceReplicate = 32768;  // top-ranked sources cached by every XCD
 constexpr int kCombineThreads = 256;
+// The most gathered sources (relabelled ids [0, hub count)) are copied into LDS by every sweep
+// block: an LDS gather costs a fraction of an L2 hit.  32 KiB of FP32 values, 8 KiB of FP64.
+template <typename T> struct HubTable { static constexpr uint32_t size = 8192; };
+template <> struct HubTable<double> { static constexpr uint32_t size = 1024; };
 
 struct Error : std::runtime_error {
     int code;
@@ -203,6 +207,18 @@ void launch_range_max(const double *x, uint32_t n, double *partials, double *out
                       hipStream_t s);
 void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, uint32_t *claim, hipStream_t s);
 
+// FP32 Krylov vector kernels (nkpr_krylov.hip).  V holds vectors of length n at `stride`.
+constexpr int kDotGroup = 8;
+// out[i] = <w, V[i]> (FP64 accumulation) for i < count; partials: kDotGroup * 1024 doubles
+void launch_multidot(const float *w, const float *V, uint64_t stride, uint32_t count, uint32_t n,
+                     double *partials, double *out, hipStream_t s);
+// assign: w = sum_i coef[i] V[i];  otherwise w -= sum_i coef[i] V[i]
+void launch_multiaxpy(float *w, const float *V, uint64_t stride, uint32_t count, uint32_t n,
+                      const double *coef, bool assign, hipStream_t s);
+// out = a x + b y  (y may be nullptr)
+void launch_axpby(const float *x, const float *y, float *out, uint32_t n, float a, float b,
+                  hipStream_t s);
+
 }  // namespace nkpr
 
 // The graph handle.
@@ -245,6 +261,9 @@ struct nkpr_graph {
     nkpr::DevBuf red_out;               // double [4]
     nkpr::DevBuf split_part32, split_part64;
     nkpr::DevBuf part32, part64;        // per-slice row sums of sliced rows
+    nkpr::DevBuf kbasis;                // FP32 Krylov basis [(m+1) * n]
+    nkpr::DevBuf kwork;                 // FP32 scratch [2 * n]
+    nkpr::DevBuf kdot;                  // FP64 dot partials and results
     nkpr::DevBuf ctl;                   // PhaseCtl
     nkpr::PhaseCtl *ctl_host = nullptr; // pinned mirror
     std::vector<hipEvent_t> events;

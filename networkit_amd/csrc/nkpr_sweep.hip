@@ -98,9 +98,18 @@ __device__ __forceinline__ uint32_t xcc_id() {
 
 // Sum of the products of in-edges [e0, e0+cnt) by the whole block (vector rows and split
 // chunks).  Four independent gathers per thread per step keep enough loads in flight.
+// Gathered value of source v: from the LDS hub table when v is one of the first nh ids.
+template <typename T>
+__device__ __forceinline__ T gather(const SweepArgs<T> &a, const T *hub, uint32_t nh, int32_t v) {
+    T g;
+    if ((uint32_t)v < nh) g = hub[v];
+    else g = a.xg[v];
+    return g;
+}
+
 template <typename T, bool FOLDED>
 __device__ __forceinline__ T block_row_sum(const SweepArgs<T> &a, uint32_t e0, uint32_t cnt,
-                                           T *red) {
+                                           T *red, const T *hub, uint32_t nh) {
     T acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
     const uint32_t tid = threadIdx.x;
     const int32_t *__restrict__ srcb = a.sc.src + e0;
@@ -116,13 +125,13 @@ __device__ __forceinline__ T block_row_sum(const SweepArgs<T> &a, uint32_t e0, u
             c2 = ld_stream(cfb + i + 2 * kBlock);
             c3 = ld_stream(cfb + i + 3 * kBlock);
         }
-        acc0 += a.xg[s0] * c0;
-        acc1 += a.xg[s1] * c1;
-        acc2 += a.xg[s2] * c2;
-        acc3 += a.xg[s3] * c3;
+        acc0 += gather(a, hub, nh, s0) * c0;
+        acc1 += gather(a, hub, nh, s1) * c1;
+        acc2 += gather(a, hub, nh, s2) * c2;
+        acc3 += gather(a, hub, nh, s3) * c3;
     }
     for (; i < cnt; i += kBlock) {
-        T v = a.xg[ld_stream(srcb + i)];
+        T v = gather(a, hub, nh, ld_stream(srcb + i));
         if (FOLDED) v *= ld_stream(cfb + i);
         acc0 += v;
     }
@@ -136,8 +145,8 @@ __device__ __forceinline__ T block_row_sum(const SweepArgs<T> &a, uint32_t e0, u
 template <typename T, bool FOLDED, bool BASEVEC, bool RESID, bool DIRECT>
 __device__ __forceinline__ void stream_item(const SweepArgs<T> &a, const uint4 item,
                                             const uint32_t *__restrict__ ptr, T *__restrict__ out,
-                                            T *s_prod, uint32_t *s_ptr, T *s_sum, T damp, T c,
-                                            Acc4 &st) {
+                                            T *s_prod, uint32_t *s_ptr, T *s_sum, const T *hub,
+                                            uint32_t nh, T damp, T c, Acc4 &st) {
     const uint32_t tid = threadIdx.x;
     const uint32_t r0 = item.x, R = item.y - item.x, e0 = item.z, cnt = item.w;
     // issue every load that does not depend on another load first
@@ -181,7 +190,7 @@ __device__ __forceinline__ void stream_item(const SweepArgs<T> &a, const uint4 i
     }
     T v[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) v[k] = a.xg[sidx[k]] * cf[k];
+    for (int k = 0; k < kPer; ++k) v[k] = gather(a, hub, nh, sidx[k]) * cf[k];
     if (tid <= R) s_ptr[tid] = p0;
     if (tid + kBlock <= R) s_ptr[tid + kBlock] = p1;
     if (tid == 0 && R == 2 * kBlock) s_ptr[R] = p2;
@@ -237,6 +246,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
     __shared__ T s_redT[kBlock / 64];
     __shared__ double s_red[kBlock / 64];
     __shared__ uint32_t s_worker;
+    __shared__ T s_hub[HubTable<T>::size];
 
     const T damp = (T)a.damp;
     const T c = rhs_const(a);
@@ -259,6 +269,9 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
         }
         s_worker = w;
     }
+    // hub table: the gathered values of the most gathered sources
+    const uint32_t nh = min(min((uint32_t)HubTable<T>::size, a.sink_begin), a.n);
+    for (uint32_t i = tid; i < nh; i += kBlock) s_hub[i] = a.xg[i];
     __syncthreads();
     const uint32_t w = s_worker;
     const uint32_t sl = w / K, rank = w - sl * K;
@@ -276,14 +289,14 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
             uint4 next = make_uint4(0, 0, 0, 0);
             if (nit < ie) next = a.sc.sitems[nit];
             if (item.y & kSplitFlag) {
-                const T p = block_row_sum<T, FOLDED>(a, item.z, item.w, s_redT);
+                const T p = block_row_sum<T, FOLDED>(a, item.z, item.w, s_redT, s_hub, nh);
                 if (tid == 0) a.split_partial[item.y & ~kSplitFlag] = p;
             } else if (item.y - item.x == 1 && item.w > (uint32_t)kCap) {
-                const T p = block_row_sum<T, FOLDED>(a, item.z, item.w, s_redT);
+                const T p = block_row_sum<T, FOLDED>(a, item.z, item.w, s_redT, s_hub, nh);
                 if (tid == 0) part[item.x] = p;
             } else {
                 stream_item<T, FOLDED, BASEVEC, RESID, false>(a, item, ptr, part, s_prod, s_ptr,
-                                                              s_sum, damp, c, st);
+                                                              s_sum, s_hub, nh, damp, c, st);
             }
             item = next;
             it = nit;
@@ -300,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
             uint4 next = make_uint4(0, 0, 0, 0);
             if (nit < a.sc.n_ditems) next = a.sc.ditems[nit];
             stream_item<T, FOLDED, BASEVEC, RESID, true>(a, item, a.sc.dptr, nullptr, s_prod,
-                                                         s_ptr, s_sum, damp, c, st);
+                                                         s_ptr, s_sum, s_hub, nh, damp, c, st);
             item = next;
             it = nit;
         }
