@@ -192,7 +192,8 @@ def main():
             "n": z, "m": m, "sinks": info["sinks"], "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(m, z),
                      "kernel": "FP32 pull sweep", "bytes_per_sweep": algorithmic_bytes(
                          m, n, True, "fp32"),
                      "ms_per_sweep": ms_f32 / n_f32},
@@ -208,6 +209,22 @@ def main():
         "prep_ms": info["prep_ms"],
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(m, n):
+    """HBM bytes per FP32 sweep (sweep + combine kernels) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the counters' reading of a 1 GiB
+    copy; None when the summary was taken on another graph."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "pagerank_rmat26_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("m") != m or t.get("n") != n:
+        return None
+    return t["fp32_sweep_traffic_bytes"]
 
 
 def cpu_baseline(g, z, m, seconds):

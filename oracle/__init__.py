@@ -11,7 +11,9 @@ that the C code consumes.
 
 Parity status of each function is pinned in tests/test_oracle.py:
   pagerank        reference known answers (CentralityGTest.cpp:484-664), sparse direct solve,
-                  dense power iteration, invariants
+                  dense power iteration, invariants; with a teleport vector: sparse direct
+                  solve with that right-hand side, uniform teleport == the reference loop,
+                  linearity in the teleport vector
   ranking         brute-force sort
   in_lists        hand-built NetworKit adjacency of small graphs (directed, undirected,
                   self-loops, multi-edges)
@@ -49,6 +51,11 @@ def _load():
                                     ctypes.c_int, p, ctypes.POINTER(u64),
                                     ctypes.POINTER(ctypes.c_double)]
     lib.oracle_pagerank.restype = ctypes.c_int
+    lib.oracle_pagerank_tv.argtypes = [u64, p, p, p, p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, u64, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, p, p, ctypes.POINTER(u64),
+                                       ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_pagerank_tv.restype = ctypes.c_int
     lib.oracle_ranking.argtypes = [u64, p, p, p]
     lib.oracle_ranking.restype = u64
     lib.oracle_out_degrees.argtypes = [u64, p, p, p, p, p]
@@ -112,9 +119,11 @@ def in_lists(z, u, v, w=None, directed=True, exists=None):
 
 
 def pagerank(g, damp=0.85, tol=1e-8, norm=L2, max_iterations=2**64 - 1, normalized=False,
-             sink_mode=0, start=None, threads=0):
+             sink_mode=0, start=None, threads=0, teleport=None):
     """Run the restated PageRank::run.  ``start`` is the start vector (default 1/n in every
-    slot, as a freshly constructed reference object).  Returns (scores, iterations, maximum)."""
+    slot, as a freshly constructed reference object).  ``teleport`` (personalised PageRank, not
+    in the reference) replaces (1-d)/n by (1-d)*teleport[u].  Returns (scores, iterations,
+    maximum)."""
     lib = _load()
     z = g.z
     n = z if g.exists is None else int(g.exists.sum())
@@ -124,10 +133,13 @@ def pagerank(g, damp=0.85, tol=1e-8, norm=L2, max_iterations=2**64 - 1, normaliz
         scores = np.array(start, dtype=np.float64, copy=True)
     it = ctypes.c_uint64(0)
     mx = ctypes.c_double(0.0)
-    rc = lib.oracle_pagerank(z, _ptr(g.exists), _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w),
-                             damp, tol, norm, int(max_iterations), int(bool(normalized)),
-                             int(sink_mode), int(threads), _ptr(scores), ctypes.byref(it),
-                             ctypes.byref(mx))
+    tv = None if teleport is None else np.ascontiguousarray(teleport, dtype=np.float64)
+    if tv is not None and tv.shape != (z,):
+        raise ValueError("teleport must have one entry per slot")
+    rc = lib.oracle_pagerank_tv(z, _ptr(g.exists), _ptr(g.in_ptr), _ptr(g.in_src),
+                                _ptr(g.in_w), damp, tol, norm, int(max_iterations),
+                                int(bool(normalized)), int(sink_mode), int(threads), _ptr(tv),
+                                _ptr(scores), ctypes.byref(it), ctypes.byref(mx))
     if rc != 0:
         raise RuntimeError("oracle_pagerank failed")
     return scores, int(it.value), float(mx.value)

@@ -72,12 +72,17 @@ void oracle_out_degrees(uint64_t z, const uint64_t *in_ptr, const uint64_t *in_s
 
 /* PageRank::run.  scores[z] holds the start vector on entry (the caller applies the
  * reference's warm-start rule: a fresh object starts at 1/n, a re-run keeps the previous
- * scores) and the final scores on exit.  Returns 0 on success, -1 on bad input. */
-int oracle_pagerank(uint64_t z, const uint8_t *exists, const uint64_t *in_ptr,
-                    const uint64_t *in_src, const double *in_w, double damp, double tol,
-                    int norm, uint64_t max_iterations, int normalized, int sink_mode,
-                    int num_threads, double *scores, uint64_t *iterations_out,
-                    double *maximum_out) {
+ * scores) and the final scores on exit.  Returns 0 on success, -1 on bad input.
+ *
+ * teleport (not in the reference, personalised PageRank): NULL gives the reference's
+ * (1 - damp) / n.  Otherwise teleport[z] is a distribution t over the slots and node u
+ * receives (1 - damp) * t[u] instead; distributed sink mass stays uniform (d S / n).  With t
+ * uniform over the existing nodes this is the reference loop again. */
+int oracle_pagerank_tv(uint64_t z, const uint8_t *exists, const uint64_t *in_ptr,
+                       const uint64_t *in_src, const double *in_w, double damp, double tol,
+                       int norm, uint64_t max_iterations, int normalized, int sink_mode,
+                       int num_threads, const double *teleport, double *scores,
+                       uint64_t *iterations_out, double *maximum_out) {
     if (!in_ptr || !scores) return -1;
 #ifdef _OPENMP
     if (num_threads > 0) omp_set_num_threads(num_threads);
@@ -91,7 +96,7 @@ int oracle_pagerank(uint64_t z, const uint8_t *exists, const uint64_t *in_ptr,
         if (maximum_out) *maximum_out = normalized ? scores[0] : 1.0;
         return 0;
     }
-    const double teleport = (1.0 - damp) / (double)n;
+    const double tele_uniform = (1.0 - damp) / (double)n;
 
     double *deg = (double *)malloc(sizeof(double) * (z ? z : 1));
     uint64_t *cnt = (uint64_t *)malloc(sizeof(uint64_t) * (z ? z : 1));
@@ -126,7 +131,7 @@ int oracle_pagerank(uint64_t z, const uint8_t *exists, const uint64_t *in_ptr,
                 acc += x[v] * w / deg[v];
             }
             acc *= damp;
-            acc += teleport;
+            acc += teleport ? (1.0 - damp) * teleport[u] : tele_uniform;
             acc += sink_term;
             y[u] = acc;
         }
@@ -187,6 +192,16 @@ int oracle_pagerank(uint64_t z, const uint8_t *exists, const uint64_t *in_ptr,
     if (maximum_out) *maximum_out = mx;
     free(deg); free(cnt); free(pr);
     return 0;
+}
+
+int oracle_pagerank(uint64_t z, const uint8_t *exists, const uint64_t *in_ptr,
+                    const uint64_t *in_src, const double *in_w, double damp, double tol,
+                    int norm, uint64_t max_iterations, int normalized, int sink_mode,
+                    int num_threads, double *scores, uint64_t *iterations_out,
+                    double *maximum_out) {
+    return oracle_pagerank_tv(z, exists, in_ptr, in_src, in_w, damp, tol, norm, max_iterations,
+                              normalized, sink_mode, num_threads, NULL, scores, iterations_out,
+                              maximum_out);
 }
 
 /* Centrality::ranking order (Centrality.cpp:29-41): existing nodes, score descending, ties by
