@@ -12,6 +12,7 @@
  *   nkpr_run             PageRank::run (PageRank.cpp:20-104) incl. numberOfIterations
  *                        (PageRank.hpp:58-61) and maximum() (PageRank.cpp:95-108)
  *   nkpr_ranking         Centrality::ranking (Centrality.cpp:29-41)
+ *   nkpr_ppr_batch       no counterpart: PageRank::run once per teleport distribution
  *   nkpr_params_default  the constructor defaults PageRank(G, 0.85, 1e-8, false), the field
  *                        defaults maxIterations = UINT64_MAX, norm = L2Norm (PageRank.hpp:40-67)
  *
@@ -30,7 +31,7 @@
 extern "C" {
 #endif
 
-#define NKPR_ABI_VERSION 1
+#define NKPR_ABI_VERSION 2
 
 /* status codes */
 #define NKPR_OK 0
@@ -114,6 +115,10 @@ typedef struct {
     int32_t timing;           /* 1: time every sweep kernel with HIP events                */
     int32_t reserved;
     const volatile int32_t *interrupt; /* optional; checked between batches of sweeps      */
+    const double *teleport;   /* optional [z] teleport distribution t (personalised PageRank,
+                                 not in the reference): node u receives (1-d) t[u] instead of
+                                 (1-d)/n; host or device memory like `scores`.  NULL: the
+                                 reference's uniform teleport.  nkpr_run only.            */
 } nkpr_params;
 
 typedef struct {
@@ -148,6 +153,38 @@ void *nkpr_graph_stream(const nkpr_graph *g);
  * at 1/n everywhere; a re-run starts from the previous scores, PageRank.cpp:26) and the
  * scores on return; slots that hold no node are left untouched.  stats may be NULL. */
 int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats);
+
+/* Statistics of a batched run (nkpr_ppr_batch). */
+typedef struct {
+    uint64_t sweeps;          /* SpMM sweeps over all columns, all precisions              */
+    uint64_t sweeps_fp32;
+    uint64_t sweeps_fp64;
+    uint64_t refinements;     /* FP64 residual sweeps followed by an FP32 correction solve */
+    uint64_t max_iterations_col; /* largest per-column iteration count                     */
+    double max_error_bound;   /* largest per-column certified L1 bound (MIXED / L1_ERROR) or
+                                 last step in `norm` (STEP)                                */
+    double ms_sweeps_fp32;    /* summed SpMM kernel time (params->timing = 1)              */
+    double ms_sweeps_fp64;
+    double ms_total;
+    uint32_t tile_columns;    /* column tile width chosen by the autotuner                 */
+    uint32_t heavy_degree;    /* in-degree above which a row is split (autotuned)          */
+} nkpr_batch_stats;
+
+/* Batched personalised PageRank (no counterpart in the reference, which would run
+ * PageRank::run once per teleport distribution, PageRank.cpp:20-104).  Column j uses the
+ * teleport distribution uniform over seed_nodes[seed_offsets[j] .. seed_offsets[j+1]) (slot
+ * ids; a repeated id counts repeatedly), or proportional to seed_weights there when given:
+ * node u receives (1-d) t_j[u] instead of (1-d)/n.  Distributed sink mass stays uniform.
+ * Every column follows nkpr_run's rules for `params` on its own: the stopping rule is applied
+ * per column and a converged column is frozen; MIXED certifies every column.  All columns
+ * advance together in one sparse-matrix x dense-matrix sweep per iteration.
+ * scores: [k][z] (column j at scores + j*z), start vectors on entry, each column divided by
+ * its sum on return (normalized must be 0); host or device memory per scores_on_device.
+ * seed arrays are host memory.  col_iterations[k] and col_error_bound[k] are optional. */
+int nkpr_ppr_batch(nkpr_graph *g, const nkpr_params *p, uint32_t k,
+                   const uint64_t *seed_offsets, const int32_t *seed_nodes,
+                   const double *seed_weights, double *scores, nkpr_batch_stats *stats,
+                   uint64_t *col_iterations, double *col_error_bound);
 
 /* Centrality::ranking: ids of the existing nodes ordered by score descending, ties by id
  * ascending; ids_out and scores_out (nullable) receive *count = number of existing nodes

@@ -26,8 +26,9 @@ NKPR_STOP_STEP, NKPR_STOP_L1_ERROR = 0, 1
 
 # every symbol include/nkpr.h declares
 EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_destroy",
-            "nkpr_graph_get_info", "nkpr_graph_stream", "nkpr_run", "nkpr_ranking",
-            "nkpr_last_error", "nkpr_abi_version"]
+            "nkpr_graph_get_info", "nkpr_graph_stream", "nkpr_run", "nkpr_ppr_batch",
+            "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version"]
+ABI_VERSION = 2
 
 
 class Csc(ctypes.Structure):
@@ -51,7 +52,8 @@ class Params(ctypes.Structure):
                 ("sink_handling", ctypes.c_int32), ("normalized", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("stop", ctypes.c_int32),
                 ("scores_on_device", ctypes.c_int32), ("timing", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("interrupt", ctypes.c_void_p)]
+                ("reserved", ctypes.c_int32), ("interrupt", ctypes.c_void_p),
+                ("teleport", ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -61,6 +63,18 @@ class Stats(ctypes.Structure):
                 ("l1_error_bound", ctypes.c_double), ("maximum", ctypes.c_double),
                 ("score_sum", ctypes.c_double), ("ms_sweeps_fp32", ctypes.c_double),
                 ("ms_sweeps_fp64", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_uint64), ("sweeps_fp32", ctypes.c_uint64),
+                ("sweeps_fp64", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
+                ("max_iterations_col", ctypes.c_uint64), ("max_error_bound", ctypes.c_double),
+                ("ms_sweeps_fp32", ctypes.c_double), ("ms_sweeps_fp64", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("tile_columns", ctypes.c_uint32),
+                ("heavy_degree", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -87,6 +101,8 @@ def lib():
     L.nkpr_graph_stream.argtypes = [vp]
     L.nkpr_graph_stream.restype = vp
     L.nkpr_run.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.POINTER(Stats)]
+    L.nkpr_ppr_batch.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_uint32, vp, vp, vp, vp,
+                                 ctypes.POINTER(BatchStats), vp, vp]
     L.nkpr_ranking.argtypes = [ctypes.c_uint64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
     L.nkpr_last_error.restype = ctypes.c_char_p
     L.nkpr_abi_version.restype = ctypes.c_int
@@ -171,6 +187,36 @@ class DeviceGraph:
         st = Stats()
         check(lib().nkpr_run(self.handle, ctypes.byref(p), _addr(scores), ctypes.byref(st)))
         return st.as_dict()
+
+    def ppr_batch(self, seeds, scores=None, seed_weights=None, params=None, **kw):
+        """nkpr_ppr_batch: one personalised PageRank per seed set (a list of k sequences of
+        slot ids, optionally with weights).  ``scores`` is the [k][z] start matrix (numpy, or a
+        torch CUDA tensor with params.scores_on_device = 1); by default every column starts at
+        1/n.  Returns (scores, stats dict, per-column iterations, per-column error bound)."""
+        p = params if params is not None else default_params(**kw)
+        k = len(seeds)
+        off = np.zeros(k + 1, dtype=np.uint64)
+        off[1:] = np.cumsum([len(s) for s in seeds])
+        nodes = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.int64) for s in seeds])
+                                     if k else np.zeros(0), dtype=np.int32)
+        wts = None
+        if seed_weights is not None:
+            wts = np.ascontiguousarray(np.concatenate([np.asarray(w, dtype=np.float64)
+                                                       for w in seed_weights]), dtype=np.float64)
+            assert len(wts) == len(nodes)
+        if scores is None:
+            n = self.info()["n"]
+            scores = np.full((k, self.z), 1.0 / max(n, 1), dtype=np.float64)
+        if isinstance(scores, np.ndarray):
+            assert scores.dtype == np.float64 and scores.flags.c_contiguous
+            assert scores.shape == (k, self.z)
+        st = BatchStats()
+        its = np.zeros(max(k, 1), dtype=np.uint64)
+        bnd = np.zeros(max(k, 1), dtype=np.float64)
+        check(lib().nkpr_ppr_batch(self.handle, ctypes.byref(p), k, _addr(off), _addr(nodes),
+                                   _addr(wts), _addr(scores), ctypes.byref(st), _addr(its),
+                                   _addr(bnd)))
+        return scores, st.as_dict(), its[:k], bnd[:k]
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

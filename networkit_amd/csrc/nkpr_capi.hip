@@ -185,6 +185,7 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
     NKPR_CHECK(p->precision >= NKPR_FP64 && p->precision <= NKPR_FP32, "unknown precision");
     NKPR_CHECK(p->stop == NKPR_STOP_STEP || p->stop == NKPR_STOP_L1_ERROR, "unknown stop rule");
     NKPR_CHECK(!(p->tol < 0.0) && !std::isnan(p->tol), "tol must be >= 0");
+    NKPR_CHECK(p->teleport == nullptr, "params->teleport is not supported by this build");
     DeviceGuard dg(g->device);
     std::memset(st, 0, sizeof(*st));
     const uint64_t z = g->z;
@@ -543,6 +544,7 @@ int nkpr_graph_destroy(nkpr_graph *g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
     if (g->ctl_host) (void)hipHostFree(g->ctl_host);
+    if (g->b_ctl_host) (void)hipHostFree(g->b_ctl_host);
     hipStream_t s = g->stream;
     delete g;  // frees device buffers
     if (s) (void)hipStreamDestroy(s);
@@ -575,6 +577,19 @@ void *nkpr_graph_stream(const nkpr_graph *g) { return g ? (void *)g->stream : nu
 int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats) {
     nkpr_stats local;
     return guarded([&] { run_impl(g, p, scores, stats ? stats : &local); });
+}
+
+int nkpr_ppr_batch(nkpr_graph *g, const nkpr_params *p, uint32_t k,
+                   const uint64_t *seed_offsets, const int32_t *seed_nodes,
+                   const double *seed_weights, double *scores, nkpr_batch_stats *stats,
+                   uint64_t *col_iterations, double *col_error_bound) {
+    nkpr_batch_stats local;
+    return guarded([&] {
+        NKPR_CHECK(g != nullptr, "graph is NULL");
+        DeviceGuard dg(g->device);
+        ppr_batch_impl(g, p, k, seed_offsets, seed_nodes, seed_weights, scores,
+                       stats ? stats : &local, col_iterations, col_error_bound);
+    });
 }
 
 int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_t *ids_out,

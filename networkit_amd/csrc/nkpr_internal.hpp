@@ -219,6 +219,35 @@ void launch_multiaxpy(float *w, const float *V, uint64_t stride, uint32_t count,
 void launch_axpby(const float *x, const float *y, float *out, uint32_t n, float a, float b,
                   hipStream_t s);
 
+// Batched personalised PageRank (nkpr_batch.hip).  Per-column control block.
+struct BatchCol {
+    double sink_in;     // sink mass of the vector the next sweep reads
+    double l1, l2, sum, sink_out, prev_l1;
+    double bound;       // certified L1 bound 2 d/(1-d) l1 / sum
+    double target;      // step a correction solve must reach so the next residual certifies
+    double solve_tol;   // target of the running correction solve
+    uint64_t iterations;  // sweeps in which the column was active (all phases)
+    uint64_t phase_iter;  // sweeps of the current phase
+    uint32_t done;      // frozen for the rest of the phase
+    uint32_t final_;    // certified (or padding): frozen for the rest of the run
+};
+struct BatchCtl {
+    uint64_t sweeps;    // executed sweeps of the phase
+    uint32_t active;
+    uint32_t done;
+};
+struct BatchTune {
+    uint32_t key;       // ld * 2 + (FP64)
+    int lpr;
+    uint32_t heavy;
+};
+void batch_layout(nkpr_graph *g);
+void batch_split(nkpr_graph *g, uint32_t heavy);
+void ensure_buf(DevBuf &b, size_t bytes);
+void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint64_t *seed_off,
+                    const int32_t *seeds, const double *seed_w, double *scores,
+                    nkpr_batch_stats *st, uint64_t *col_iterations, double *col_bound);
+
 }  // namespace nkpr
 
 // The graph handle.
@@ -267,4 +296,17 @@ struct nkpr_graph {
     nkpr::DevBuf ctl;                   // PhaseCtl
     nkpr::PhaseCtl *ctl_host = nullptr; // pinned mirror
     std::vector<hipEvent_t> events;
+
+    // batched personalised PageRank: plain CSC copy and its row split (built on first use)
+    bool b_ready = false, b_split_ready = false;
+    nkpr::DevBuf b_rowptr, b_src, b_coef32, b_coef64;
+    std::vector<uint32_t> b_rowptr_host;
+    std::vector<int32_t> b_perm_host;
+    uint32_t b_heavy = 0, b_n_light = 0, b_n_heavy = 0, b_n_pieces = 0;
+    nkpr::DevBuf b_light, b_hrow, b_hpiece, b_pieces;
+    std::vector<nkpr::BatchTune> b_tuned;
+    nkpr::DevBuf b_tptr, b_tcol, b_tval32, b_tval64;
+    nkpr::DevBuf b_piece32, b_piece64, b_partials, b_col, b_ctl, b_sink_part, b_factor;
+    nkpr::DevBuf b_x64a, b_x64b, b_x32a, b_x32b, b_r32, b_zbuf;
+    nkpr::BatchCtl *b_ctl_host = nullptr;
 };

@@ -34,7 +34,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_header_layout():
     # sizes of the C structs (x86-64 SysV): keep ctypes mirrors in sync with nkpr.h
     assert ctypes.sizeof(capi.Csc) == 8 * 7 + 8
-    assert ctypes.sizeof(capi.Params) == 8 * 3 + 4 * 8 + 8
+    assert ctypes.sizeof(capi.Params) == 8 * 3 + 4 * 8 + 8 + 8
+    assert ctypes.sizeof(capi.BatchStats) == 8 * 5 + 8 * 4 + 4 * 2
     assert ctypes.sizeof(capi.Stats) == 8 * 13
     assert ctypes.sizeof(capi.GraphInfo) == 8 * 7 + 8 + 8
 
@@ -47,7 +48,7 @@ def test_params_default_is_reference_default():
     assert p.sink_handling == capi.NKPR_SINK_NONE
     assert p.normalized == 0 and p.precision == capi.NKPR_FP64
     assert p.stop == capi.NKPR_STOP_STEP
-    assert capi.lib().nkpr_abi_version() == 1
+    assert capi.lib().nkpr_abi_version() == capi.ABI_VERSION == 2
 
 
 def test_graph_create_without_device_reports_nodevice():
@@ -65,6 +66,7 @@ def test_null_arguments_are_errors_not_crashes():
     assert L.nkpr_graph_create(None, None) == capi.NKPR_ERR_INVALID or \
         L.nkpr_graph_create(None, None) == capi.NKPR_ERR_NODEVICE
     assert L.nkpr_run(None, None, None, None) != capi.NKPR_OK
+    assert L.nkpr_ppr_batch(None, None, 0, None, None, None, None, None, None, None) != capi.NKPR_OK
     assert L.nkpr_graph_destroy(None) == capi.NKPR_OK
     assert L.nkpr_params_default(None) == capi.NKPR_ERR_INVALID
 

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from closed_forms import dense_power_iteration, pagerank_closed_form
+from closed_forms import dense_power_iteration, pagerank_closed_form, solve_fixed_point
 from graphs import (EIGHT_PR, EIGHT_PR_NORMALIZED, WIKI_TOP, WIKI_TOP_NORMALIZED, eight_node,
                     random_graph, skewed_graph, wiki_vote)
 
@@ -181,3 +181,53 @@ def test_ranking_bruteforce():
     ids = [i for i in range(500) if exists[i]]
     ids.sort(key=lambda i: (-s[i], i))
     assert r.tolist() == ids
+
+
+# ---- personalised teleport (batched PPR oracle) ------------------------------------------------
+
+def _teleport(z, seeds, weights=None, exists=None):
+    t = np.zeros(z)
+    np.add.at(t, np.asarray(seeds), 1.0 if weights is None else np.asarray(weights, float))
+    return t / t.sum()
+
+
+@pytest.mark.parametrize("sink_mode", [0, 1])
+def test_oracle_teleport_matches_direct_solve(sink_mode):
+    z, u, v, w = skewed_graph(3000, 20000, 21)
+    g = oracle.in_lists(z, u, v, w, True)
+    rng = np.random.default_rng(3)
+    for k in range(4):
+        seeds = rng.choice(z, rng.integers(1, 40), replace=False)
+        t = _teleport(z, seeds, rng.uniform(0.1, 2.0, len(seeds)) if k % 2 else None)
+        s, it, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=5000,
+                                   sink_mode=sink_mode, teleport=t)
+        x, exists, sinks, n = solve_fixed_point(g, 0.85, sink_mode, teleport=0.15 * t)
+        ref = x / x.sum()
+        assert np.abs(s - ref).sum() < 1e-11
+        # the teleport enters as (1-d) t: a run with the uniform rule instead differs
+        su, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=5000,
+                                   sink_mode=sink_mode)
+        assert np.abs(su - ref).sum() > 1e-2
+
+
+def test_oracle_uniform_teleport_is_reference_loop():
+    z, u, v, w = random_graph(2000, 7000, 5, directed=True, weighted=True)
+    g = oracle.in_lists(z, u, v, w, True)
+    for sink_mode in (0, 1):
+        a, ia, ma = oracle.pagerank(g, sink_mode=sink_mode)
+        b, ib, mb = oracle.pagerank(g, sink_mode=sink_mode, teleport=np.full(z, 1.0 / z))
+        assert ia == ib and np.abs(a - b).max() <= 1e-15
+
+
+def test_oracle_teleport_linear():
+    # the unnormalised fixed point is linear in the teleport vector
+    z, u, v, w = random_graph(800, 3000, 9, directed=True)
+    g = oracle.in_lists(z, u, v, w, True)
+    t1 = _teleport(z, [1, 5, 9])
+    t2 = _teleport(z, [100, 200])
+    x1, *_ = solve_fixed_point(g, 0.85, 0, teleport=0.15 * t1)
+    x2, *_ = solve_fixed_point(g, 0.85, 0, teleport=0.15 * t2)
+    t = 0.3 * t1 + 0.7 * t2
+    s, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=5000, teleport=t)
+    mix = 0.3 * x1 + 0.7 * x2
+    assert np.abs(s - mix / mix.sum()).sum() < 1e-11
