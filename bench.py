@@ -56,11 +56,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
+    ap.add_argument("--workload", choices=["pagerank", "ppr"], default="pagerank",
+                    help="pagerank: one solve on R-MAT (the benchmark line); ppr: batched "
+                         "personalised PageRank on a Barabasi-Albert graph")
+    ap.add_argument("--ppr-nodes", type=int, default=2_000_000)
+    ap.add_argument("--ppr-columns", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ppr":
+        bench_ppr(args)
+        return
 
     import torch
 
@@ -207,6 +215,116 @@ def main():
                  "ms_per_sweep": ms64 / n64, "achieved_GBs": achieved64},
         "speedup_vs_fp64": sec_fp64 / sec_mixed,
         "prep_ms": info["prep_ms"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def spmm_bytes(m, n, k, weighted, dbl):
+    """Algorithmic bytes of one batched sweep over k columns: per in-edge the source id, its
+    coefficient (w/deg, or 1/deg of the source when unweighted) and one gathered row of k
+    values; per node the row offset, the old and the new row of k values."""
+    e = 8 if dbl else 4
+    return m * (4 + e + e * k) + n * (4 + 2 * e * k)
+
+
+def bench_ppr(args):
+    """Batched personalised PageRank: Barabasi-Albert graph (n nodes, 8 attachments per node,
+    undirected, unweighted), k teleport distributions each uniform over a random seed set of
+    U{1..1000} nodes, seed 4; mixed precision to a certified L1 error <= tol per column next
+    to the plain FP64 batch at the same certified error."""
+    import torch
+
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nn, k = args.ppr_nodes, args.ppr_columns
+    t0 = time.time()
+    u, v = gen.barabasi_albert(nn, 8, seed=4)
+    # undirected in-lists (NetworKit adjacency: edge i adds v to row u, then u to row v; BA has
+    # no self-loops), sorted stably by row on the GPU
+    tu = torch.from_numpy(u).to(dev)
+    tv = torch.from_numpy(v).to(dev)
+    rows = torch.stack([tu, tv], dim=1).reshape(-1)
+    srcs = torch.stack([tv, tu], dim=1).reshape(-1)
+    order = torch.argsort(rows, stable=True)
+    colptr = torch.zeros(nn + 1, dtype=torch.int64, device=dev)
+    colptr[1:] = torch.cumsum(torch.bincount(rows, minlength=nn), 0)
+    c = {"colptr": colptr, "src": srcs[order].to(torch.int32).contiguous()}
+    del tu, tv, rows, srcs, order
+    m = int(colptr[-1])
+    rng = np.random.default_rng(4)
+    seeds = [rng.choice(nn, int(rng.integers(1, 1001)), replace=False) for _ in range(k)]
+    log("BA n=%d m=%d arcs, %d seed sets, built in %.1f s" % (nn, m, k, time.time() - t0))
+    torch.cuda.synchronize()
+    dg = capi.DeviceGraph(nn, c["colptr"], c["src"], device=True)
+    stream = torch.cuda.ExternalStream(dg.stream())
+    x = torch.empty((k, nn), dtype=torch.float64, device=dev)
+
+    def solve(precision, tol):
+        torch.cuda.current_stream().synchronize()
+        with torch.cuda.stream(stream):
+            x.fill_(1.0 / nn)
+        p = capi.default_params(scores_on_device=1, precision=precision, tol=tol,
+                                stop=capi.NKPR_STOP_L1_ERROR, timing=1)
+        _, st, its, bnd = dg.ppr_batch(seeds, scores=x, params=p)
+        return st, bnd
+
+    solve(capi.NKPR_FP64, 1e-13)
+    ref = x.clone()
+    torch.cuda.synchronize()
+
+    def timed(precision):
+        for _ in range(args.warmup):
+            solve(precision, args.tol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sts = [solve(precision, args.tol) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t) / args.steps
+        err = float((x - ref).abs().sum(dim=1).max())
+        return sec, sts, err
+
+    sec_m, st_m, err_m = timed(capi.NKPR_MIXED)
+    sec_d, st_d, err_d = timed(capi.NKPR_FP64)
+    s0 = st_m[0][0]
+    d0 = st_d[0][0]
+    b32 = spmm_bytes(m, nn, k, False, False)
+    b64 = spmm_bytes(m, nn, k, False, True)
+    ms32 = sum(st["ms_sweeps_fp32"] for st, _ in st_m) / sum(st["sweeps_fp32"] for st, _ in st_m)
+    ms64 = sum(st["ms_sweeps_fp64"] for st, _ in st_d) / sum(st["sweeps_fp64"] for st, _ in st_d)
+    ach32 = b32 / (ms32 / 1e3) / 1e9
+    ach64 = b64 / (ms64 / 1e3) / 1e9
+    log("mixed: %.1f ms, %d fp32 + %d fp64 sweeps, max L1 err %.2e; fp32 SpMM %.3f ms = %.0f GB/s "
+        "(tile %d, heavy %d)" % (sec_m * 1e3, s0["sweeps_fp32"], s0["sweeps_fp64"], err_m, ms32,
+                                 ach32, s0["tile_columns"], s0["heavy_degree"]))
+    log("fp64: %.1f ms, %d sweeps, max L1 err %.2e; fp64 SpMM %.3f ms = %.0f GB/s" % (
+        sec_d * 1e3, d0["sweeps_fp64"], err_d, ms64, ach64))
+    sweeps_m = s0["sweeps_fp32"] + s0["sweeps_fp64"]
+    out = {
+        "metric": "batched personalised PageRank column-edge-updates/s, mixed vs FP64 at equal "
+                  "L1 error",
+        "value": m * k * sweeps_m / sec_m, "unit": "column-edge-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec_m * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "batched personalised PageRank, %d teleport distributions "
+                               "(uniform over U{1..1000} random seeds), certified L1 error <= "
+                               "%g per column, Barabasi-Albert n=%d, 8 attachments, undirected, "
+                               "seed 4" % (k, args.tol, nn), "n": nn, "m": m, "columns": k},
+        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "FP32 SpMM sweep", "bytes_per_sweep": b32,
+                     "ms_per_sweep": ms32},
+        "mixed": {"ms_per_solve": sec_m * 1e3, "sweeps_fp32": s0["sweeps_fp32"],
+                  "sweeps_fp64": s0["sweeps_fp64"], "refinements": s0["refinements"],
+                  "max_l1_error": err_m, "max_error_bound": s0["max_error_bound"],
+                  "tile_columns": s0["tile_columns"], "heavy_degree": s0["heavy_degree"]},
+        "fp64": {"ms_per_solve": sec_d * 1e3, "sweeps": d0["sweeps_fp64"], "max_l1_error": err_d,
+                 "ms_per_sweep": ms64, "achieved_GBs": ach64,
+                 "tile_columns": d0["tile_columns"]},
+        "speedup_vs_fp64": sec_d / sec_m,
     }
     print(json.dumps(out), flush=True)
 

@@ -13,6 +13,8 @@
  *                        (PageRank.hpp:58-61) and maximum() (PageRank.cpp:95-108)
  *   nkpr_ranking         Centrality::ranking (Centrality.cpp:29-41)
  *   nkpr_ppr_batch       no counterpart: PageRank::run once per teleport distribution
+ *   nkpr_adjoint         no counterpart: gradients of the scores (differentiable PageRank)
+ *   nkpr_graph_set_weights  no counterpart: a new Graph with the same structure
  *   nkpr_params_default  the constructor defaults PageRank(G, 0.85, 1e-8, false), the field
  *                        defaults maxIterations = UINT64_MAX, norm = L2Norm (PageRank.hpp:40-67)
  *
@@ -153,6 +155,28 @@ void *nkpr_graph_stream(const nkpr_graph *g);
  * at 1/n everywhere; a re-run starts from the previous scores, PageRank.cpp:26) and the
  * scores on return; slots that hold no node are left untouched.  stats may be NULL. */
 int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats);
+
+/* New edge weights for a graph made by nkpr_graph_create with weights (same structure): the
+ * arrays are laid out like nkpr_csc.weight / weight64 / out_wdeg (out_wdeg NULL: summed from
+ * the weights); device or host memory per device_ptrs.  The folded coefficients w / deg are
+ * recomputed in place; no counterpart in the reference, where a new Graph would be built. */
+int nkpr_graph_set_weights(nkpr_graph *g, const float *weight, const double *weight64,
+                           const double *out_wdeg, int32_t device_ptrs);
+
+/* Adjoint of PageRank (no counterpart in the reference).  For the sum-normalised scores x of
+ * nkpr_run (normalized = 0, NKPR_SINK_NONE, optional params->teleport t) and the gradient
+ * grad_scores = dL/dx of a loss, returns
+ *   grad_teleport[z] = dL/dt = (1-d) lambda,
+ *   grad_weight[m]   = dL/dw for every in-edge of `csc` (laid out like csc->weight),
+ * where lambda = d P^T lambda + (dL/dx - (dL/dx . x)) / score_sum.  scores and score_sum
+ * (nkpr_stats.score_sum) come from the forward run; `csc` is the structure the graph was
+ * created from, with its current weights.  params->precision: FP32 = restarted FP32 GMRES to a
+ * relative 2-norm residual <= tol; FP64 / MIXED = FP64 residual sweeps with FP32 GMRES
+ * corrections to a relative L1 residual <= tol.  z-vectors are host or device memory per
+ * scores_on_device, grad_weight per csc->device_ptrs; either output may be NULL. */
+int nkpr_adjoint(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
+                 const double *scores, double score_sum, const double *grad_scores,
+                 double *grad_weight, double *grad_teleport, nkpr_stats *stats);
 
 /* Statistics of a batched run (nkpr_ppr_batch). */
 typedef struct {

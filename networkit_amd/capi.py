@@ -26,8 +26,9 @@ NKPR_STOP_STEP, NKPR_STOP_L1_ERROR = 0, 1
 
 # every symbol include/nkpr.h declares
 EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_destroy",
-            "nkpr_graph_get_info", "nkpr_graph_stream", "nkpr_run", "nkpr_ppr_batch",
-            "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version"]
+            "nkpr_graph_get_info", "nkpr_graph_stream", "nkpr_graph_set_weights", "nkpr_run",
+            "nkpr_adjoint", "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error",
+            "nkpr_abi_version"]
 ABI_VERSION = 2
 
 
@@ -101,6 +102,9 @@ def lib():
     L.nkpr_graph_stream.argtypes = [vp]
     L.nkpr_graph_stream.restype = vp
     L.nkpr_run.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.POINTER(Stats)]
+    L.nkpr_graph_set_weights.argtypes = [vp, vp, vp, vp, ctypes.c_int32]
+    L.nkpr_adjoint.argtypes = [vp, ctypes.POINTER(Params), ctypes.POINTER(Csc), vp,
+                               ctypes.c_double, vp, vp, vp, ctypes.POINTER(Stats)]
     L.nkpr_ppr_batch.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_uint32, vp, vp, vp, vp,
                                  ctypes.POINTER(BatchStats), vp, vp]
     L.nkpr_ranking.argtypes = [ctypes.c_uint64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
@@ -167,7 +171,55 @@ class DeviceGraph:
         check(lib().nkpr_graph_create(ctypes.byref(c), ctypes.byref(h)))
         self.handle = h
         self.z = int(z)
+        self.device = device
         self._keep = None
+
+    def set_weights(self, weight, weight64=None, out_wdeg=None):
+        """nkpr_graph_set_weights: new weights, same structure (arrays of the constructor's
+        residency)."""
+        if not self.device:
+            weight = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+            weight64 = None if weight64 is None else np.ascontiguousarray(weight64, dtype=np.float64)
+            out_wdeg = None if out_wdeg is None else np.ascontiguousarray(out_wdeg, dtype=np.float64)
+        check(lib().nkpr_graph_set_weights(self.handle, _addr(weight), _addr(weight64),
+                                           _addr(out_wdeg), 1 if self.device else 0))
+
+    def adjoint(self, colptr, src, scores, score_sum, grad_scores, weight=None, weight64=None,
+                out_wdeg=None, exists=None, want_weight=True, want_teleport=True, params=None,
+                **kw):
+        """nkpr_adjoint.  (colptr, src, weight...) is the structure the graph was made from,
+        with the same residency as the constructor's arrays; scores / grad_scores follow
+        params.scores_on_device.  Returns (grad_weight, grad_teleport, stats); host outputs
+        are numpy arrays, device outputs must be passed in via kw grad_weight_out /
+        grad_teleport_out (torch tensors)."""
+        gw_out = kw.pop("grad_weight_out", None)
+        gt_out = kw.pop("grad_teleport_out", None)
+        p = params if params is not None else default_params(**kw)
+        if not self.device:
+            colptr = np.ascontiguousarray(colptr, dtype=np.uint64)
+            src = np.ascontiguousarray(src, dtype=np.int32)
+            weight = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+            weight64 = None if weight64 is None else np.ascontiguousarray(weight64, dtype=np.float64)
+            out_wdeg = None if out_wdeg is None else np.ascontiguousarray(out_wdeg, dtype=np.float64)
+        m = int(colptr[-1])
+        c = Csc(z=self.z, colptr=_addr(colptr), src=_addr(src), weight=_addr(weight),
+                weight64=_addr(weight64), out_wdeg=_addr(out_wdeg), exists=_addr(exists),
+                device_ptrs=1 if self.device else 0)
+        gw = gw_out
+        if gw is None and want_weight:
+            assert not self.device, "pass grad_weight_out for device graphs"
+            gw = np.zeros(max(m, 1), dtype=np.float64)
+        gt = gt_out
+        if gt is None and want_teleport:
+            assert not p.scores_on_device, "pass grad_teleport_out for device scores"
+            gt = np.zeros(self.z, dtype=np.float64)
+        st = Stats()
+        check(lib().nkpr_adjoint(self.handle, ctypes.byref(p), ctypes.byref(c), _addr(scores),
+                                 float(score_sum), _addr(grad_scores), _addr(gw), _addr(gt),
+                                 ctypes.byref(st)))
+        if isinstance(gw, np.ndarray):
+            gw = gw[:m]
+        return gw, gt, st.as_dict()
 
     def info(self):
         i = GraphInfo()

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "nkpr_internal.hpp"
+#include "nkpr_run.hpp"
 
 namespace nkpr {
 void create_graph(const nkpr_csc *c, nkpr_graph *g);
@@ -58,121 +59,242 @@ struct DeviceGuard {
     }
 };
 
-// Buffers of one precision for a ping-pong phase.
-template <typename T> struct Vecs {
-    T *x[2];
-    T *xh[2];
-};
+}  // namespace
 
-void ensure(DevBuf &b, size_t bytes) { b.alloc(bytes); }
+namespace nkpr {
 
-constexpr int kMaxRefinements = 30;
-
-uint32_t env_u32(const char *name, uint32_t dflt) {
-    const char *v = std::getenv(name);
-    if (!v || !*v) return dflt;
-    return (uint32_t)std::strtoul(v, nullptr, 10);
+void prepare_run(nkpr_graph *g) {
+    ensure(g->ctl, sizeof(PhaseCtl));
+    if (!g->ctl_host) NKPR_HIP(hipHostMalloc((void **)&g->ctl_host, sizeof(PhaseCtl), hipHostMallocDefault));
+    ensure(g->red_partials, 8 * 1024);
+    ensure(g->red_out, 8 * 4);
+    ensure(g->split_part32, 4 * std::max<size_t>(g->n_split_items, 1));
+    ensure(g->split_part64, 8 * std::max<size_t>(g->n_split_items, 1));
+    ensure(g->part32, 4 * std::max<size_t>((size_t)g->n_slices * g->n_s, 1));
+    ensure(g->part64, 8 * std::max<size_t>((size_t)g->n_slices * g->n_s, 1));
+    const size_t n = g->n;
+    ensure(g->x64a, 8 * n);
+    ensure(g->x64b, 8 * n);
+    ensure(g->x32a, 4 * n);
+    ensure(g->x32b, 4 * n);
+    if (!g->weighted) {
+        ensure(g->xh64a, 8 * n);
+        ensure(g->xh64b, 8 * n);
+        ensure(g->xh32a, 4 * n);
+        ensure(g->xh32b, 4 * n);
+    }
 }
 
-struct RunCtx {
-    nkpr_graph *g;
-    const nkpr_params *p;
-    nkpr_stats *st;
-    uint64_t sweeps_left;
-    int workers[2][3][2];  // [dbl][kind][folded] workers per slice
-    int combine_grid;
-};
-
-Schedule make_schedule(nkpr_graph *g, uint32_t workers) {
-    Schedule sc{};
-    sc.src = g->src.as<int32_t>();
-    sc.n_slices = g->n_slices;
-    sc.drow = g->drow.as<uint32_t>();
-    sc.dptr = g->dptr.as<uint32_t>();
-    sc.ditems = g->ditems.as<uint4>();
-    sc.n_d = g->n_d;
-    sc.n_ditems = g->n_ditems;
-    sc.srow = g->srow.as<uint32_t>();
-    sc.sptr = g->sptr.as<uint32_t>();
-    sc.sitems = g->sitems.as<uint4>();
-    for (int q = 0; q <= kSlices; ++q) sc.sitem_off[q] = g->sitem_off[q];
-    sc.n_s = g->n_s;
-    sc.split_flag = g->split_flag.as<uint8_t>();
-    sc.claim = g->claim.as<uint32_t>();
-    sc.workers = workers;
-    return sc;
-}
-
-// Launch sweeps of one phase until `done` or `max_launch` sweeps.  Returns completed sweeps.
-template <typename T>
-uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule rule,
-                   uint64_t max_launch, bool timing, double *ms_acc) {
-    nkpr_graph *g = rc.g;
+// One solve on compact vectors.  On entry g->x64a holds the FP64 start vector (and g->x32a its
+// FP32 copy for NKPR_FP32).  base64 / base32 (compact, optional) replace the constant
+// teleport (1-d)/n by a per-node right-hand side.  Returns the unnormalised FP64 result and
+// the FP64 sums of its last sweep.
+SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64,
+                       const float *base32, nkpr_stats *st) {
+    const uint32_t n = g->n;
+    const hipStream_t s = g->stream;
+    const double damp = p->damp;
+    const bool has_base = base64 != nullptr;
+    const double tele = has_base ? 0.0 : (1.0 - damp) / (double)n;
     const bool folded = g->weighted;
-    const bool dbl = sizeof(T) == 8;
-    int &workers = rc.workers[dbl][kind][folded];
-    if (workers <= 0)
-        workers = std::max(1, sweep_resident_blocks(g->device, dbl, folded, kind) / (int)g->n_slices);
-    FinalizeArgs f{};
-    f.split_rows = g->split_rows.as<uint32_t>();
-    f.split_first = g->split_first.as<uint32_t>();
-    f.split_count = g->split_count.as<uint32_t>();
-    f.n_split_rows = g->n_split_rows;
-    f.n_blocks = (uint32_t)(workers * g->n_slices + (g->n_s ? rc.combine_grid : 0));
-    f.rule = rule;
-    ensure(g->partials, 8 * 4 * (size_t)f.n_blocks);
-    a.sc = make_schedule(g, (uint32_t)workers);
-    a.partials = g->partials.as<double>();
-    a.ctl = g->ctl.as<PhaseCtl>();
+    const uint64_t max_it = p->max_iterations;
+    prepare_run(g);
+    int cus = 0;
+    NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    RunCtx rc{g, p, st, max_it, {}, 0};
+    rc.combine_grid = (int)std::min<uint64_t>(((uint64_t)g->n_s + kCombineThreads - 1) / kCombineThreads,
+                                              (uint64_t)std::max(cus, 1) * 8);
+    const bool dist = p->sink_handling == NKPR_SINK_DISTRIBUTE;
+    PhaseCtl *ctl = g->ctl.as<PhaseCtl>();
+    const int kind_plain = has_base ? kSweepBaseVec : kSweepPlain;
+    const int kind_resid = has_base ? kSweepBaseResid : kSweepResid;
 
-    uint64_t launched = 0, done_sweeps = 0;
-    uint64_t batch = (max_launch <= 2) ? max_launch : 4;
-    const size_t need_ev = 2 * 64;
-    if (timing && g->events.size() < need_ev) {
-        while (g->events.size() < need_ev) {
-            hipEvent_t e;
-            NKPR_HIP(hipEventCreate(&e));
-            g->events.push_back(e);
+    auto init_phase = [&](const void *x, bool dbl) {
+        const double *sink_sum = nullptr;
+        if (dist && g->sink_begin < n) {
+            if (dbl) launch_range_sum<double>((const double *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+            else launch_range_sum<float>((const float *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+            sink_sum = g->red_out.as<double>();
+        }
+        launch_init_ctl(ctl, sink_sum, g->claim.as<uint32_t>(), s);
+    };
+
+    auto base_args64 = [&]() {
+        SweepArgs<double> a{};
+        a.coef = folded ? g->coef64.as<double>() : nullptr;
+        a.partial = g->part64.as<double>();
+        a.inv = folded ? nullptr : g->inv64.as<double>();
+        a.split_partial = g->split_part64.as<double>();
+        a.base_vec = base64;
+        a.damp = damp;
+        a.tele = tele;
+        a.n = n;
+        a.sink_begin = g->sink_begin;
+        a.distribute = dist ? 1 : 0;
+        return a;
+    };
+    auto base_args32 = [&]() {
+        SweepArgs<float> a{};
+        a.coef = folded ? g->coef32.as<float>() : nullptr;
+        a.partial = g->part32.as<float>();
+        a.inv = folded ? nullptr : g->inv32.as<float>();
+        a.split_partial = g->split_part32.as<float>();
+        a.base_vec = base32;
+        a.damp = damp;
+        a.tele = tele;
+        a.n = n;
+        a.sink_begin = g->sink_begin;
+        a.distribute = dist ? 1 : 0;
+        return a;
+    };
+
+    SolveResult res{};
+    uint64_t total = 0;
+    const bool timing = p->timing != 0;
+    Vecs<double> v64{};
+    Vecs<float> v32{};
+
+    if (p->precision == NKPR_FP64 || p->precision == NKPR_FP32) {
+        const bool dbl = p->precision == NKPR_FP64;
+        PhaseRule rule{};
+        rule.kind = (p->stop == NKPR_STOP_STEP) ? kRuleStep : kRuleErr;
+        rule.norm = p->norm;
+        rule.tol = p->tol;
+        rule.max_iterations = max_it;
+        rule.damp = damp;
+        if (dbl) {
+            v64 = {{g->x64a.as<double>(), g->x64b.as<double>()}, {g->xh64a.as<double>(), g->xh64b.as<double>()}};
+            if (!folded) launch_prescale(v64.x[0], g->inv64.as<double>(), v64.xh[0], nullptr, nullptr, nullptr, n, g->sink_begin, s);
+            init_phase(v64.x[0], true);
+            total = run_phase<double>(rc, base_args64(), v64, kind_plain, rule, std::max<uint64_t>(max_it, 1), timing, &st->ms_sweeps_fp64);
+            st->sweeps_fp64 = total;
+            res.x = v64.x[total & 1];
+        } else {
+            v32 = {{g->x32a.as<float>(), g->x32b.as<float>()}, {g->xh32a.as<float>(), g->xh32b.as<float>()}};
+            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
+            init_phase(v32.x[0], false);
+            total = run_phase<float>(rc, base_args32(), v32, kind_plain, rule, std::max<uint64_t>(max_it, 1), timing, &st->ms_sweeps_fp32);
+            st->sweeps_fp32 = total;
+            launch_to_double(v32.x[total & 1], g->x64a.as<double>(), n, s);
+            res.x = g->x64a.as<double>();
+        }
+        res.sum = g->ctl_host->sum;
+        res.sink = g->ctl_host->sink_out;
+        st->last_step_l1 = g->ctl_host->l1;
+        st->last_step = (p->norm == NKPR_L2) ? std::sqrt(g->ctl_host->l2) : g->ctl_host->l1;
+        st->l1_error_bound = rel_error_bound(damp, g->ctl_host->l1, g->ctl_host->sum);
+    } else {
+        // ---- mixed precision: FP64 residual sweeps certify, FP32 GMRES solves corrections ----
+        // With r = b - A x from an FP64 sweep (A = I - d P, P with the sink term when sinks
+        // are distributed), A e = r is solved approximately in FP32 by GMRES(m) and x += e in
+        // FP64.  Only FP64 quantities enter the certificate 2 d/(1-d) ||r||_1 / sum(x + r).
+        double *xcur = g->x64a.as<double>();   // current FP64 iterate
+        double *ynew = g->x64b.as<double>();   // FP64 sweep output
+        double *xh = g->xh64a.as<double>();
+        Krylov kr = krylov_setup(g);
+        auto apply = [&](const float *v, float *w) {
+            apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32);
+        };
+        for (int round = 0;; ++round) {
+            // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
+            if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
+            init_phase(xcur, true);
+            PhaseRule rr{};
+            rr.kind = kRuleErr;
+            rr.norm = NKPR_L1;
+            rr.tol = p->tol;
+            rr.max_iterations = 1;
+            rr.damp = damp;
+            SweepArgs<double> a64 = base_args64();
+            a64.r32 = g->r32.as<float>();
+            Vecs<double> vr{{xcur, ynew}, {xh, g->xh64b.as<double>()}};
+            const uint64_t k = run_phase<double>(rc, a64, vr, kind_resid, rr, 1, timing, &st->ms_sweeps_fp64);
+            st->sweeps_fp64 += k;
+            total += k;
+            const PhaseCtl c = *g->ctl_host;
+            st->last_step_l1 = c.l1;
+            st->last_step = c.l1;
+            st->l1_error_bound = c.bound;
+            res.sum = c.sum;
+            res.sink = c.sink_out;
+            res.x = ynew;
+            if (c.bound <= p->tol || total >= max_it || round >= kMaxRefinements) break;
+            // FP32 correction: A e = r, tightened so that the next residual certifies
+            ++st->refinements;
+            const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
+            const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
+            const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
+            const uint64_t kc = gmres_f32(kr, apply, g->r32.as<float>(), kr.e, rtol, left, s);
+            st->sweeps_fp32 += kc;
+            st->sweeps_fp32_correction += kc;
+            total += kc;
+            launch_add_correction(xcur, kr.e, n, s);
         }
     }
-    while (launched < max_launch) {
-        if (rc.p->interrupt && *rc.p->interrupt)
-            throw Error(NKPR_ERR_INTERRUPTED, "interrupted");
-        const uint64_t nb = std::min<uint64_t>(std::min<uint64_t>(batch, 64), max_launch - launched);
-        for (uint64_t j = 0; j < nb; ++j) {
-            const uint64_t k = launched + j;
-            const int cur = (int)(k & 1), nxt = cur ^ 1;
-            a.xold = v.x[cur];
-            a.xnew = v.x[nxt];
-            a.xg = folded ? v.x[cur] : v.xh[cur];
-            a.xhat_new = folded ? nullptr : v.xh[nxt];
-            hipEvent_t e0 = timing ? g->events[2 * j] : nullptr;
-            hipEvent_t e1 = timing ? g->events[2 * j + 1] : nullptr;
-            launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
-        }
-        NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,
-                                g->stream));
-        NKPR_HIP(hipStreamSynchronize(g->stream));
-        const uint64_t it = g->ctl_host->iterations;
-        const uint64_t executed = it - done_sweeps;
-        if (timing) {
-            for (uint64_t j = 0; j < executed && j < nb; ++j) {
-                float ms = 0.f;
-                NKPR_HIP(hipEventElapsedTime(&ms, g->events[2 * j], g->events[2 * j + 1]));
-                *ms_acc += ms;
-            }
-        }
-        done_sweeps = it;
-        launched += nb;
-        if (g->ctl_host->done) break;
-        batch *= 2;
-    }
-    return done_sweeps;
+    res.total = total;
+    st->iterations = total;
+    st->score_sum = res.sum;
+    return res;
 }
 
-double rel_error_bound(double damp, double l1, double sum) {
-    return sum > 0.0 ? 2.0 * damp / (1.0 - damp) * l1 / sum : INFINITY;
+Krylov krylov_setup(nkpr_graph *g) {
+    const size_t n = g->n;
+    ensure(g->r32, 4 * n);
+    Krylov k{};
+    k.m = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 20), 2), 64);
+    ensure(g->kbasis, 4 * (size_t)(k.m + 1) * n);
+    ensure(g->kwork, 4 * (size_t)4 * n);
+    ensure(g->kdot, 8 * ((size_t)kDotGroup * 1024 + 128));
+    k.n = g->n;
+    k.V = g->kbasis.as<float>();
+    k.y = g->kwork.as<float>();
+    k.vh = k.y + n;
+    k.yh = k.y + 2 * n;
+    k.e = k.y + 3 * n;
+    k.dpart = g->kdot.as<double>();
+    k.dout = k.dpart + (size_t)kDotGroup * 1024;
+    return k;
 }
+
+// w = A v = v - d P v: one FP32 sweep with no teleport term (sinks per the run's rule).
+void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
+                    double *ms_acc) {
+    nkpr_graph *g = rc.g;
+    const hipStream_t s = g->stream;
+    const uint32_t n = g->n;
+    const bool folded = g->weighted;
+    const bool dist = rc.p->sink_handling == NKPR_SINK_DISTRIBUTE;
+    if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kr.vh, n, g->sink_begin, s);
+    const double *sink_sum = nullptr;
+    if (dist && g->sink_begin < n) {
+        launch_range_sum<float>(v, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+        sink_sum = g->red_out.as<double>();
+    }
+    launch_init_ctl(g->ctl.as<PhaseCtl>(), sink_sum, g->claim.as<uint32_t>(), s);
+    SweepArgs<float> a{};
+    a.coef = folded ? g->coef32.as<float>() : nullptr;
+    a.partial = g->part32.as<float>();
+    a.inv = folded ? nullptr : g->inv32.as<float>();
+    a.split_partial = g->split_part32.as<float>();
+    a.damp = rc.p->damp;
+    a.tele = 0.0;
+    a.n = n;
+    a.sink_begin = g->sink_begin;
+    a.distribute = dist ? 1 : 0;
+    Vecs<float> vv{{const_cast<float *>(v), kr.y}, {kr.vh, kr.yh}};
+    PhaseRule r{};
+    r.kind = kRuleStep;
+    r.norm = NKPR_L1;
+    r.tol = -1.0;
+    r.max_iterations = 1;
+    r.damp = rc.p->damp;
+    run_phase<float>(rc, a, vv, kSweepPlain, r, 1, timing, ms_acc);
+    launch_axpby(v, kr.y, w, n, 1.0f, -1.0f, s);
+}
+
+}  // namespace nkpr
+
+namespace {
 
 void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *st) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -185,7 +307,6 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
     NKPR_CHECK(p->precision >= NKPR_FP64 && p->precision <= NKPR_FP32, "unknown precision");
     NKPR_CHECK(p->stop == NKPR_STOP_STEP || p->stop == NKPR_STOP_L1_ERROR, "unknown stop rule");
     NKPR_CHECK(!(p->tol < 0.0) && !std::isnan(p->tol), "tol must be >= 0");
-    NKPR_CHECK(p->teleport == nullptr, "params->teleport is not supported by this build");
     DeviceGuard dg(g->device);
     std::memset(st, 0, sizeof(*st));
     const uint64_t z = g->z;
@@ -210,271 +331,38 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
         st->maximum = p->normalized ? (p->scores_on_device ? 0.0 : scores[0]) : 1.0;
         return;
     }
-
     const double damp = p->damp;
-    const double tele = (1.0 - damp) / (double)n;
-    const bool folded = g->weighted;
-    const uint64_t max_it = p->max_iterations;
-    ensure(g->ctl, sizeof(PhaseCtl));
-    if (!g->ctl_host) NKPR_HIP(hipHostMalloc((void **)&g->ctl_host, sizeof(PhaseCtl), hipHostMallocDefault));
-    ensure(g->red_partials, 8 * 1024);
-    ensure(g->red_out, 8 * 4);
-    int cus = 0;
-    NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-    ensure(g->split_part32, 4 * std::max<size_t>(g->n_split_items, 1));
-    ensure(g->split_part64, 8 * std::max<size_t>(g->n_split_items, 1));
+    prepare_run(g);
+    launch_gather_start(sz, g->perm.as<int32_t>(), n, g->x64a.as<double>(),
+                        p->precision == NKPR_FP32 ? g->x32a.as<float>() : nullptr, s);
 
-    ensure(g->part32, 4 * std::max<size_t>((size_t)g->n_slices * g->n_s, 1));
-    ensure(g->part64, 8 * std::max<size_t>((size_t)g->n_slices * g->n_s, 1));
-    RunCtx rc{g, p, st, max_it, {}, 0};
-    rc.combine_grid = (int)std::min<uint64_t>(((uint64_t)g->n_s + kCombineThreads - 1) / kCombineThreads,
-                                              (uint64_t)std::max(cus, 1) * 8);
-    const bool dist = p->sink_handling == NKPR_SINK_DISTRIBUTE;
-    PhaseCtl *ctl = g->ctl.as<PhaseCtl>();
-
-    auto init_phase = [&](const void *x, bool dbl) {
-        const double *sink_sum = nullptr;
-        if (dist && g->sink_begin < n) {
-            if (dbl) launch_range_sum<double>((const double *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
-            else launch_range_sum<float>((const float *)x, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
-            sink_sum = g->red_out.as<double>();
+    // personalised teleport: b[u] = (1 - d) t[slot of u]
+    const double *b64 = nullptr;
+    const float *b32 = nullptr;
+    if (p->teleport) {
+        const double *tz = p->teleport;
+        if (!p->scores_on_device) {
+            ensure(g->tbuf, 8 * z);
+            NKPR_HIP(hipMemcpyAsync(g->tbuf.p, p->teleport, 8 * z, hipMemcpyHostToDevice, s));
+            tz = g->tbuf.as<double>();
         }
-        launch_init_ctl(ctl, sink_sum, g->claim.as<uint32_t>(), s);
-    };
-
-    auto base_args64 = [&]() {
-        SweepArgs<double> a{};
-        a.coef = folded ? g->coef64.as<double>() : nullptr;
-        a.partial = g->part64.as<double>();
-        a.inv = folded ? nullptr : g->inv64.as<double>();
-        a.split_partial = g->split_part64.as<double>();
-        a.damp = damp;
-        a.tele = tele;
-        a.n = n;
-        a.sink_begin = g->sink_begin;
-        a.distribute = dist ? 1 : 0;
-        return a;
-    };
-    auto base_args32 = [&]() {
-        SweepArgs<float> a{};
-        a.coef = folded ? g->coef32.as<float>() : nullptr;
-        a.partial = g->part32.as<float>();
-        a.inv = folded ? nullptr : g->inv32.as<float>();
-        a.split_partial = g->split_part32.as<float>();
-        a.damp = damp;
-        a.tele = tele;
-        a.n = n;
-        a.sink_begin = g->sink_begin;
-        a.distribute = dist ? 1 : 0;
-        return a;
-    };
-
-    const double *result = nullptr;  // compact FP64 result
-    double res_sum = 0.0, res_sink = 0.0;
-    uint64_t total = 0;
-    const bool timing = p->timing != 0;
-
-    auto alloc64 = [&]() {
-        ensure(g->x64a, 8 * (size_t)n);
-        ensure(g->x64b, 8 * (size_t)n);
-        if (!folded) {
-            ensure(g->xh64a, 8 * (size_t)n);
-            ensure(g->xh64b, 8 * (size_t)n);
-        }
-    };
-    auto alloc32 = [&]() {
-        ensure(g->x32a, 4 * (size_t)n);
-        ensure(g->x32b, 4 * (size_t)n);
-        if (!folded) {
-            ensure(g->xh32a, 4 * (size_t)n);
-            ensure(g->xh32b, 4 * (size_t)n);
-        }
-    };
-    Vecs<double> v64{};
-    Vecs<float> v32{};
-
-    if (p->precision == NKPR_FP64 || p->precision == NKPR_FP32) {
-        const bool dbl = p->precision == NKPR_FP64;
-        PhaseRule rule{};
-        rule.kind = (p->stop == NKPR_STOP_STEP) ? kRuleStep : kRuleErr;
-        rule.norm = p->norm;
-        rule.tol = p->tol;
-        rule.max_iterations = max_it;
-        rule.damp = damp;
-        if (dbl) {
-            alloc64();
-            v64 = {{g->x64a.as<double>(), g->x64b.as<double>()}, {g->xh64a.as<double>(), g->xh64b.as<double>()}};
-            launch_gather_start(sz, g->perm.as<int32_t>(), n, v64.x[0], nullptr, s);
-            if (!folded) launch_prescale(v64.x[0], g->inv64.as<double>(), v64.xh[0], nullptr, nullptr, nullptr, n, g->sink_begin, s);
-            init_phase(v64.x[0], true);
-            total = run_phase<double>(rc, base_args64(), v64, kSweepPlain, rule, std::max<uint64_t>(max_it, 1), timing, &st->ms_sweeps_fp64);
-            st->sweeps_fp64 = total;
-            result = v64.x[total & 1];
-        } else {
-            alloc32();
-            ensure(g->x64a, 8 * (size_t)n);
-            v32 = {{g->x32a.as<float>(), g->x32b.as<float>()}, {g->xh32a.as<float>(), g->xh32b.as<float>()}};
-            launch_gather_start(sz, g->perm.as<int32_t>(), n, nullptr, v32.x[0], s);
-            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v32.x[0], g->inv32.as<float>(), v32.xh[0], n, g->sink_begin, s);
-            init_phase(v32.x[0], false);
-            total = run_phase<float>(rc, base_args32(), v32, kSweepPlain, rule, std::max<uint64_t>(max_it, 1), timing, &st->ms_sweeps_fp32);
-            st->sweeps_fp32 = total;
-            launch_to_double(v32.x[total & 1], g->x64a.as<double>(), n, s);
-            result = g->x64a.as<double>();
-        }
-        res_sum = g->ctl_host->sum;
-        res_sink = g->ctl_host->sink_out;
-        st->last_step_l1 = g->ctl_host->l1;
-        st->last_step = (p->norm == NKPR_L2) ? std::sqrt(g->ctl_host->l2) : g->ctl_host->l1;
-        st->l1_error_bound = rel_error_bound(damp, g->ctl_host->l1, g->ctl_host->sum);
-    } else {
-        // ---- mixed precision: FP64 residual sweeps certify, FP32 GMRES solves corrections ----
-        // With r = b - A x from an FP64 sweep (A = I - d P, P with the sink term when sinks
-        // are distributed), A e = r is solved approximately in FP32 by GMRES(m) and x += e in
-        // FP64.  Only FP64 quantities enter the certificate 2 d/(1-d) ||r||_1 / sum(x + r).
-        alloc64();
-        ensure(g->r32, 4 * (size_t)n);
-        const int mres = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 20), 2), 64);
-        ensure(g->kbasis, 4 * (size_t)(mres + 1) * n);
-        ensure(g->kwork, 4 * (size_t)4 * n);
-        ensure(g->kdot, 8 * ((size_t)kDotGroup * 1024 + 128));
-        ensure(g->x32b, 4 * (size_t)n);
-        double *xcur = g->x64a.as<double>();   // current FP64 iterate
-        double *ynew = g->x64b.as<double>();   // FP64 sweep output
-        double *xh = g->xh64a.as<double>();
-        float *V = g->kbasis.as<float>();
-        float *ky = g->kwork.as<float>();            // y = d P v
-        float *kvh = ky + n;                         // prescaled v (unweighted)
-        float *kyh = ky + 2 * (size_t)n;             // prescaled y (written, unused)
-        float *ke = ky + 3 * (size_t)n;              // correction e
-        double *dpart = g->kdot.as<double>();
-        double *dout = dpart + (size_t)kDotGroup * 1024;
-        launch_gather_start(sz, g->perm.as<int32_t>(), n, xcur, nullptr, s);
-
-        auto dots = [&](const float *w, const float *Vb, uint32_t cnt, double *host) {
-            launch_multidot(w, Vb, n, cnt, n, dpart, dout, s);
-            NKPR_HIP(hipMemcpyAsync(host, dout, 8 * cnt, hipMemcpyDeviceToHost, s));
-            NKPR_HIP(hipStreamSynchronize(s));
-        };
-        // w = A v = v - d P v  (one FP32 sweep with no teleport term)
-        auto apply = [&](const float *v, float *w) {
-            if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kvh, n, g->sink_begin, s);
-            init_phase(v, false);
-            SweepArgs<float> a = base_args32();
-            a.tele = 0.0;
-            Vecs<float> vv{{const_cast<float *>(v), ky}, {kvh, kyh}};
-            PhaseRule r{};
-            r.kind = kRuleStep;
-            r.norm = NKPR_L1;
-            r.tol = -1.0;
-            r.max_iterations = 1;
-            r.damp = damp;
-            run_phase<float>(rc, a, vv, kSweepPlain, r, 1, timing, &st->ms_sweeps_fp32);
-            launch_axpby(v, ky, w, n, 1.0f, -1.0f, s);
-        };
-        // GMRES(mres) from e = 0 on A e = rhs until ||A e - rhs||_2 <= rtol ||rhs||_2
-        auto gmres = [&](const float *rhs, float *e, double rtol, uint64_t cap) -> uint64_t {
-            double beta2 = 0.0;
-            dots(rhs, rhs, 1, &beta2);
-            const double beta = std::sqrt(beta2);
-            if (!(beta > 0.0) || cap == 0) {
-                launch_axpby(rhs, nullptr, e, n, 0.0f, 0.0f, s);
-                return 0;
-            }
-            launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
-            std::vector<double> R((size_t)mres * mres, 0.0), cs(mres), sn(mres), gv(mres + 1, 0.0);
-            gv[0] = beta;
-            uint64_t applies = 0;
-            int k = 0;
-            for (int j = 0; j < mres && applies < cap; ++j) {
-                float *vj = V + (size_t)j * n;
-                float *w = V + (size_t)(j + 1) * n;
-                apply(vj, w);
-                ++applies;
-                std::vector<double> h(j + 2), h2(j + 1);
-                // classical Gram-Schmidt, applied twice
-                dots(w, V, j + 1, h.data());
-                launch_multiaxpy(w, V, n, j + 1, n, h.data(), false, s);
-                dots(w, V, j + 1, h2.data());
-                launch_multiaxpy(w, V, n, j + 1, n, h2.data(), false, s);
-                for (int i = 0; i <= j; ++i) h[i] += h2[i];
-                double hn2 = 0.0;
-                dots(w, w, 1, &hn2);
-                const double hn = std::sqrt(hn2);
-                h[j + 1] = hn;
-                for (int i = 0; i < j; ++i) {
-                    const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-                    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-                    h[i] = t;
-                }
-                const double den = std::hypot(h[j], h[j + 1]);
-                cs[j] = den > 0.0 ? h[j] / den : 1.0;
-                sn[j] = den > 0.0 ? h[j + 1] / den : 0.0;
-                h[j] = den;
-                gv[j + 1] = -sn[j] * gv[j];
-                gv[j] = cs[j] * gv[j];
-                for (int i = 0; i <= j; ++i) R[(size_t)i * mres + j] = h[i];
-                k = j + 1;
-                if (hn > 0.0) launch_axpby(w, nullptr, w, n, (float)(1.0 / hn), 0.0f, s);
-                if (std::fabs(gv[j + 1]) <= rtol * beta || !(hn > 0.0)) break;
-            }
-            std::vector<double> y(k, 0.0);
-            for (int i = k - 1; i >= 0; --i) {
-                double t = gv[i];
-                for (int q = i + 1; q < k; ++q) t -= R[(size_t)i * mres + q] * y[q];
-                y[i] = R[(size_t)i * mres + i] != 0.0 ? t / R[(size_t)i * mres + i] : 0.0;
-            }
-            launch_multiaxpy(e, V, n, k, n, y.data(), true, s);
-            return applies;
-        };
-
-        for (int round = 0;; ++round) {
-            // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
-            if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
-            init_phase(xcur, true);
-            PhaseRule rr{};
-            rr.kind = kRuleErr;
-            rr.norm = NKPR_L1;
-            rr.tol = p->tol;
-            rr.max_iterations = 1;
-            rr.damp = damp;
-            SweepArgs<double> a64 = base_args64();
-            a64.r32 = g->r32.as<float>();
-            Vecs<double> vr{{xcur, ynew}, {xh, g->xh64b.as<double>()}};
-            const uint64_t k = run_phase<double>(rc, a64, vr, kSweepResid, rr, 1, timing, &st->ms_sweeps_fp64);
-            st->sweeps_fp64 += k;
-            total += k;
-            const PhaseCtl c = *g->ctl_host;
-            st->last_step_l1 = c.l1;
-            st->last_step = c.l1;
-            st->l1_error_bound = c.bound;
-            res_sum = c.sum;
-            res_sink = c.sink_out;
-            result = ynew;
-            if (c.bound <= p->tol || total >= max_it || round >= kMaxRefinements) break;
-            // FP32 correction: A e = r, tightened so that the next residual certifies
-            ++st->refinements;
-            const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
-            const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
-            const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
-            const uint64_t kc = gmres(g->r32.as<float>(), ke, rtol, left);
-            st->sweeps_fp32 += kc;
-            st->sweeps_fp32_correction += kc;
-            total += kc;
-            launch_add_correction(xcur, ke, n, s);
-        }
+        ensure(g->base64, 8 * (size_t)n);
+        ensure(g->base32, 4 * (size_t)n);
+        launch_gather_scaled(tz, g->perm.as<int32_t>(), n, 1.0 - damp, g->base64.as<double>(),
+                             g->base32.as<float>(), s);
+        b64 = g->base64.as<double>();
+        b32 = g->base32.as<float>();
     }
-    st->iterations = total;
-    st->score_sum = res_sum;
+    const SolveResult res = solve_core(g, p, b64, b32, st);
 
     // ---- normalisation (PageRank.cpp:73-92) and scatter to slot order -----------------------
     double factor;
-    if (!p->normalized) factor = res_sum;
-    else factor = (1.0 / (double)n) * ((1.0 - damp) + damp * res_sink);
-    launch_scatter_result(result, g->perm.as<int32_t>(), n, factor, sz, s);
+    if (!p->normalized) factor = res.sum;
+    else factor = (1.0 / (double)n) * ((1.0 - damp) + damp * res.sink);
+    launch_scatter_result(res.x, g->perm.as<int32_t>(), n, factor, sz, s);
     double maximum = 1.0;
     if (p->normalized) {
-        launch_range_max(result, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
+        launch_range_max(res.x, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
         double mx = 0.0, s0 = 0.0;
         NKPR_HIP(hipMemcpyAsync(&mx, g->red_out.p, 8, hipMemcpyDeviceToHost, s));
         NKPR_HIP(hipMemcpyAsync(&s0, sz, 8, hipMemcpyDeviceToHost, s));
@@ -542,9 +430,6 @@ int nkpr_graph_destroy(nkpr_graph *g) {
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
-    for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
-    if (g->ctl_host) (void)hipHostFree(g->ctl_host);
-    if (g->b_ctl_host) (void)hipHostFree(g->b_ctl_host);
     hipStream_t s = g->stream;
     delete g;  // frees device buffers
     if (s) (void)hipStreamDestroy(s);
@@ -577,6 +462,27 @@ void *nkpr_graph_stream(const nkpr_graph *g) { return g ? (void *)g->stream : nu
 int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats) {
     nkpr_stats local;
     return guarded([&] { run_impl(g, p, scores, stats ? stats : &local); });
+}
+
+int nkpr_graph_set_weights(nkpr_graph *g, const float *weight, const double *weight64,
+                           const double *out_wdeg, int32_t device_ptrs) {
+    return guarded([&] {
+        NKPR_CHECK(g != nullptr, "graph is NULL");
+        DeviceGuard dg(g->device);
+        set_weights(g, weight, weight64, out_wdeg, device_ptrs);
+    });
+}
+
+int nkpr_adjoint(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
+                 const double *scores, double score_sum, const double *grad_scores,
+                 double *grad_weight, double *grad_teleport, nkpr_stats *stats) {
+    nkpr_stats local;
+    return guarded([&] {
+        NKPR_CHECK(g != nullptr, "graph is NULL");
+        DeviceGuard dg(g->device);
+        adjoint_impl(g, p, csc, scores, score_sum, grad_scores, grad_weight, grad_teleport,
+                     stats ? stats : &local);
+    });
 }
 
 int nkpr_ppr_batch(nkpr_graph *g, const nkpr_params *p, uint32_t k,

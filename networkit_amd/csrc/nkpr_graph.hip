@@ -181,7 +181,8 @@ __global__ void reorder_edges_kernel(const uint64_t *colptr, const int32_t *src,
                                      const float *c32, const double *c64, const uint32_t *rows,
                                      uint32_t nrows, const uint32_t *dptr, const uint32_t *sptr,
                                      uint32_t n_s, uint32_t S, uint32_t rep, int32_t *src2,
-                                     float *c32_2, double *c64_2) {
+                                     float *c32_2, double *c64_2, const uint32_t *em,
+                                     uint32_t *em2) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -197,6 +198,7 @@ __global__ void reorder_edges_kernel(const uint64_t *colptr, const int32_t *src,
                     c32_2[d + (i - b)] = c32[i];
                     c64_2[d + (i - b)] = c64[i];
                 }
+                if (em) em2[d + (i - b)] = em[i];
             }
             continue;
         }
@@ -218,6 +220,7 @@ __global__ void reorder_edges_kernel(const uint64_t *colptr, const int32_t *src,
                     c32_2[dest] = c32[i + lane];
                     c64_2[dest] = c64[i + lane];
                 }
+                if (em) em2[dest] = em[i + lane];
             }
         }
     }
@@ -368,12 +371,16 @@ void build_schedule(nkpr_graph *g, const std::vector<uint64_t> &cp) {
         const double *o64 = g->weighted ? g->coef64.as<double>() : nullptr;
         float *n32 = g->weighted ? c32.as<float>() : nullptr;
         double *n64 = g->weighted ? c64.as<double>() : nullptr;
+        DevBuf em2;
+        const uint32_t *em = g->weighted ? g->emap.as<uint32_t>() : nullptr;
+        if (g->weighted) em2.alloc(4 * std::max<uint64_t>(m, 1));
+        uint32_t *emo = g->weighted ? em2.as<uint32_t>() : nullptr;
         if (g->n_d) {
             const unsigned blocks = (unsigned)std::min<uint64_t>(blocks_for((uint64_t)g->n_d * 64), 1u << 18);
             hipLaunchKernelGGL(reorder_edges_kernel, dim3(blocks), dim3(256), 0, s,
                                g->colptr.as<uint64_t>(), g->src.as<int32_t>(), o32, o64,
                                g->drow.as<uint32_t>(), g->n_d, g->dptr.as<uint32_t>(), nullptr,
-                               0u, S, rep, src2.as<int32_t>(), n32, n64);
+                               0u, S, rep, src2.as<int32_t>(), n32, n64, em, emo);
             NKPR_HIP(hipGetLastError());
         }
         if (n_s) {
@@ -381,7 +388,7 @@ void build_schedule(nkpr_graph *g, const std::vector<uint64_t> &cp) {
             hipLaunchKernelGGL(reorder_edges_kernel, dim3(blocks), dim3(256), 0, s,
                                g->colptr.as<uint64_t>(), g->src.as<int32_t>(), o32, o64,
                                g->srow.as<uint32_t>(), n_s, nullptr, g->sptr.as<uint32_t>(),
-                               n_s, S, rep, src2.as<int32_t>(), n32, n64);
+                               n_s, S, rep, src2.as<int32_t>(), n32, n64, em, emo);
             NKPR_HIP(hipGetLastError());
         }
         NKPR_HIP(hipStreamSynchronize(s));
@@ -392,6 +399,8 @@ void build_schedule(nkpr_graph *g, const std::vector<uint64_t> &cp) {
             std::swap(g->coef32.bytes, c32.bytes);
             std::swap(g->coef64.p, c64.p);
             std::swap(g->coef64.bytes, c64.bytes);
+            std::swap(g->emap.p, em2.p);
+            std::swap(g->emap.bytes, em2.bytes);
         }
     }
 
@@ -623,6 +632,11 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g) {
                            g->weighted ? g->coef32.as<float>() : nullptr,
                            g->weighted ? g->coef64.as<double>() : nullptr);
         NKPR_HIP(hipGetLastError());
+        if (g->weighted) {
+            // input position of every edge, kept so that new weights can be folded in place
+            g->emap.alloc(4 * m);
+            NKPR_HIP(hipMemcpyAsync(g->emap.p, vd.Current(), 4 * m, hipMemcpyDeviceToDevice, s));
+        }
         NKPR_HIP(hipStreamSynchronize(s));
     }
     if (!g->weighted) {
@@ -643,6 +657,92 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g) {
     NKPR_HIP(hipStreamSynchronize(s));
     g->prep_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+namespace {
+
+// Weighted out-degree of every compact source from the edges in sweep order.
+__global__ void sweep_wdeg_kernel(const int32_t *src, const uint32_t *emap, uint64_t m,
+                                  const float *w, const double *w64, double *wdeg) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = emap[i];
+        atomicAdd(&wdeg[src[i]], w64 ? w64[o] : (double)w[o]);
+    }
+}
+
+__global__ void gather_wdeg_kernel(const double *wdeg_z, const int32_t *perm, uint32_t n,
+                                   double *wdeg) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wdeg[i] = wdeg_z[perm[i]];
+}
+
+// coef = w / deg(src) for every edge in sweep order (the fold of graph creation).
+__global__ void refold_kernel(const int32_t *src, const uint32_t *emap, uint64_t m,
+                              const float *w, const double *w64, const double *wdeg,
+                              float *c32, double *c64) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = emap[i];
+        const double c = (w64 ? w64[o] : (double)w[o]) / wdeg[src[i]];
+        c64[i] = c;
+        c32[i] = (float)c;
+    }
+}
+
+}  // namespace
+
+// New edge weights for the same structure (nkpr_graph_set_weights).
+void set_weights(nkpr_graph *g, const float *w, const double *w64, const double *out_wdeg,
+                 int device_ptrs) {
+    NKPR_CHECK(g != nullptr, "graph is NULL");
+    NKPR_CHECK(g->weighted, "the graph was created without weights");
+    NKPR_CHECK(w != nullptr || w64 != nullptr, "weights are NULL");
+    const hipStream_t s = g->stream;
+    const uint64_t m = g->m, z = g->z;
+    const uint32_t n = g->n;
+    DevBuf wb, w64b, degz, deg;
+    if (!device_ptrs) {
+        if (w) {
+            wb.alloc(4 * std::max<uint64_t>(m, 1));
+            NKPR_HIP(hipMemcpyAsync(wb.p, w, 4 * m, hipMemcpyHostToDevice, s));
+            w = wb.as<float>();
+        }
+        if (w64) {
+            w64b.alloc(8 * std::max<uint64_t>(m, 1));
+            NKPR_HIP(hipMemcpyAsync(w64b.p, w64, 8 * m, hipMemcpyHostToDevice, s));
+            w64 = w64b.as<double>();
+        }
+    }
+    deg.alloc(8 * std::max<uint32_t>(n, 1));
+    if (out_wdeg) {
+        const double *dz = out_wdeg;
+        if (!device_ptrs) {
+            degz.alloc(8 * std::max<uint64_t>(z, 1));
+            NKPR_HIP(hipMemcpyAsync(degz.p, out_wdeg, 8 * z, hipMemcpyHostToDevice, s));
+            dz = degz.as<double>();
+        }
+        if (n)
+            hipLaunchKernelGGL(gather_wdeg_kernel, dim3(blocks_for(n)), dim3(256), 0, s, dz,
+                               g->perm.as<int32_t>(), n, deg.as<double>());
+    } else {
+        NKPR_HIP(hipMemsetAsync(deg.p, 0, 8 * std::max<uint32_t>(n, 1), s));
+        if (m)
+            hipLaunchKernelGGL(sweep_wdeg_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)),
+                               dim3(256), 0, s, g->src.as<int32_t>(), g->emap.as<uint32_t>(), m, w,
+                               w64, deg.as<double>());
+    }
+    NKPR_HIP(hipGetLastError());
+    if (m)
+        hipLaunchKernelGGL(refold_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)),
+                           dim3(256), 0, s, g->src.as<int32_t>(), g->emap.as<uint32_t>(), m, w, w64,
+                           deg.as<double>(), g->coef32.as<float>(), g->coef64.as<double>());
+    NKPR_HIP(hipGetLastError());
+    NKPR_HIP(hipStreamSynchronize(s));
+    // derived copies follow the coefficients
+    g->b_ready = false;
+    g->b_split_ready = false;
+    g->adj.reset();
 }
 
 }  // namespace nkpr

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -179,6 +180,7 @@ enum SweepKind : int {
     kSweepPlain = 0,      // x' = d P x + tele (+ sinks)
     kSweepBaseVec = 1,    // x' = d P x + base_vec (+ sinks): correction solve
     kSweepResid = 2,      // plain + r32 = float(x' - x)  (FP64 only)
+    kSweepBaseResid = 3,  // base vector + r32  (FP64 only)
 };
 
 // sweep (grid = n_slices * workers) + combine (combine_grid blocks, if n_s > 0) + finalize
@@ -194,6 +196,9 @@ void launch_gather_start(const double *scores_z, const int32_t *perm, uint32_t n
 void launch_prescale(const double *x64, const double *inv64, double *xhat64, const float *x32,
                      const float *inv32, float *xhat32, uint32_t n, uint32_t sink_begin,
                      hipStream_t s);
+// out64[u] = f * in[perm[u]] and out32 = (float)out64 (a slot-order vector to compact order)
+void launch_gather_scaled(const double *in_z, const int32_t *perm, uint32_t n, double f,
+                          double *out64, float *out32, hipStream_t s);
 void launch_to_double(const float *x32, double *x64, uint32_t n, hipStream_t s);
 void launch_add_correction(double *x64, const float *e32, uint32_t n, hipStream_t s);
 void launch_copy_f32(const float *in, float *out, uint32_t n, hipStream_t s);
@@ -244,6 +249,11 @@ struct BatchTune {
 void batch_layout(nkpr_graph *g);
 void batch_split(nkpr_graph *g, uint32_t heavy);
 void ensure_buf(DevBuf &b, size_t bytes);
+void set_weights(nkpr_graph *g, const float *w, const double *w64, const double *out_wdeg,
+                 int device_ptrs);
+void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
+                  const double *scores, double score_sum, const double *grad_scores,
+                  double *grad_weight, double *grad_teleport, nkpr_stats *st);
 void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint64_t *seed_off,
                     const int32_t *seeds, const double *seed_w, double *scores,
                     nkpr_batch_stats *st, uint64_t *col_iterations, double *col_bound);
@@ -267,6 +277,7 @@ struct nkpr_graph {
     nkpr::DevBuf coef64;    // double [m + kCap]  (weighted)
     nkpr::DevBuf inv32;     // float [n]   (unweighted)
     nkpr::DevBuf inv64;     // double [n]  (unweighted)
+    nkpr::DevBuf emap;      // uint32 [m]  input position of every edge in sweep order (weighted)
     std::vector<uint8_t> exists_host;  // empty: every slot exists
 
     // schedule
@@ -284,7 +295,7 @@ struct nkpr_graph {
     nkpr::DevBuf zbuf;                  // double [z] staging of host scores
     nkpr::DevBuf x64a, x64b, xh64a, xh64b;
     nkpr::DevBuf x32a, x32b, xh32a, xh32b;
-    nkpr::DevBuf r32, base64;
+    nkpr::DevBuf r32, base64, base32, tbuf;
     nkpr::DevBuf partials;              // double [grid*4]
     nkpr::DevBuf red_partials;          // double [1024]
     nkpr::DevBuf red_out;               // double [4]
@@ -309,4 +320,18 @@ struct nkpr_graph {
     nkpr::DevBuf b_piece32, b_piece64, b_partials, b_col, b_ctl, b_sink_part, b_factor;
     nkpr::DevBuf b_x64a, b_x64b, b_x32a, b_x32b, b_r32, b_zbuf;
     nkpr::BatchCtl *b_ctl_host = nullptr;
+
+    // adjoint: the transposed operator d P^T as a graph of its own (built on first use; it
+    // shares this graph's stream and device)
+    std::unique_ptr<nkpr_graph> adj;
+    nkpr::DevBuf a_vec[6];
+
+    nkpr_graph() = default;
+    nkpr_graph(const nkpr_graph &) = delete;
+    nkpr_graph &operator=(const nkpr_graph &) = delete;
+    ~nkpr_graph() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+        if (b_ctl_host) (void)hipHostFree(b_ctl_host);
+    }
 };

@@ -1,0 +1,228 @@
+// Host orchestration shared by the entry points of libnkpr.so: batches of sweeps with a
+// device-side stop flag (run_phase), the FP32 GMRES used for correction solves, and one
+// complete solve on compact vectors (solve_core).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "nkpr_internal.hpp"
+
+namespace nkpr {
+
+// Buffers of one precision for a ping-pong phase.
+template <typename T> struct Vecs {
+    T *x[2];
+    T *xh[2];
+};
+
+inline void ensure(DevBuf &b, size_t bytes) { b.alloc(bytes); }
+
+constexpr int kMaxRefinements = 30;
+
+inline uint32_t env_u32(const char *name, uint32_t dflt) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return (uint32_t)std::strtoul(v, nullptr, 10);
+}
+
+struct RunCtx {
+    nkpr_graph *g;
+    const nkpr_params *p;
+    nkpr_stats *st;
+    uint64_t sweeps_left;
+    int workers[2][4][2];  // [dbl][kind][folded] workers per slice
+    int combine_grid;
+};
+
+inline Schedule make_schedule(nkpr_graph *g, uint32_t workers) {
+    Schedule sc{};
+    sc.src = g->src.as<int32_t>();
+    sc.n_slices = g->n_slices;
+    sc.drow = g->drow.as<uint32_t>();
+    sc.dptr = g->dptr.as<uint32_t>();
+    sc.ditems = g->ditems.as<uint4>();
+    sc.n_d = g->n_d;
+    sc.n_ditems = g->n_ditems;
+    sc.srow = g->srow.as<uint32_t>();
+    sc.sptr = g->sptr.as<uint32_t>();
+    sc.sitems = g->sitems.as<uint4>();
+    for (int q = 0; q <= kSlices; ++q) sc.sitem_off[q] = g->sitem_off[q];
+    sc.n_s = g->n_s;
+    sc.split_flag = g->split_flag.as<uint8_t>();
+    sc.claim = g->claim.as<uint32_t>();
+    sc.workers = workers;
+    return sc;
+}
+
+// Launch sweeps of one phase until `done` or `max_launch` sweeps.  Returns completed sweeps.
+template <typename T>
+uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule rule,
+                   uint64_t max_launch, bool timing, double *ms_acc) {
+    nkpr_graph *g = rc.g;
+    const bool folded = g->weighted;
+    const bool dbl = sizeof(T) == 8;
+    int &workers = rc.workers[dbl][kind][folded];
+    if (workers <= 0)
+        workers = std::max(1, sweep_resident_blocks(g->device, dbl, folded, kind) / (int)g->n_slices);
+    FinalizeArgs f{};
+    f.split_rows = g->split_rows.as<uint32_t>();
+    f.split_first = g->split_first.as<uint32_t>();
+    f.split_count = g->split_count.as<uint32_t>();
+    f.n_split_rows = g->n_split_rows;
+    f.n_blocks = (uint32_t)(workers * g->n_slices + (g->n_s ? rc.combine_grid : 0));
+    f.rule = rule;
+    ensure(g->partials, 8 * 4 * (size_t)f.n_blocks);
+    a.sc = make_schedule(g, (uint32_t)workers);
+    a.partials = g->partials.as<double>();
+    a.ctl = g->ctl.as<PhaseCtl>();
+
+    uint64_t launched = 0, done_sweeps = 0;
+    uint64_t batch = (max_launch <= 2) ? max_launch : 4;
+    const size_t need_ev = 2 * 64;
+    if (timing && g->events.size() < need_ev) {
+        while (g->events.size() < need_ev) {
+            hipEvent_t e;
+            NKPR_HIP(hipEventCreate(&e));
+            g->events.push_back(e);
+        }
+    }
+    while (launched < max_launch) {
+        if (rc.p->interrupt && *rc.p->interrupt)
+            throw Error(NKPR_ERR_INTERRUPTED, "interrupted");
+        const uint64_t nb = std::min<uint64_t>(std::min<uint64_t>(batch, 64), max_launch - launched);
+        for (uint64_t j = 0; j < nb; ++j) {
+            const uint64_t k = launched + j;
+            const int cur = (int)(k & 1), nxt = cur ^ 1;
+            a.xold = v.x[cur];
+            a.xnew = v.x[nxt];
+            a.xg = folded ? v.x[cur] : v.xh[cur];
+            a.xhat_new = folded ? nullptr : v.xh[nxt];
+            hipEvent_t e0 = timing ? g->events[2 * j] : nullptr;
+            hipEvent_t e1 = timing ? g->events[2 * j + 1] : nullptr;
+            launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
+        }
+        NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,
+                                g->stream));
+        NKPR_HIP(hipStreamSynchronize(g->stream));
+        const uint64_t it = g->ctl_host->iterations;
+        const uint64_t executed = it - done_sweeps;
+        if (timing) {
+            for (uint64_t j = 0; j < executed && j < nb; ++j) {
+                float ms = 0.f;
+                NKPR_HIP(hipEventElapsedTime(&ms, g->events[2 * j], g->events[2 * j + 1]));
+                *ms_acc += ms;
+            }
+        }
+        done_sweeps = it;
+        launched += nb;
+        if (g->ctl_host->done) break;
+        batch *= 2;
+    }
+    return done_sweeps;
+}
+
+inline double rel_error_bound(double damp, double l1, double sum) {
+    return sum > 0.0 ? 2.0 * damp / (1.0 - damp) * l1 / sum : INFINITY;
+}
+
+
+
+// Result of solve_core: the unnormalised FP64 iterate (compact order) and its sums.
+struct SolveResult {
+    const double *x;
+    double sum;
+    double sink;
+    uint64_t total;
+};
+
+// FP32 Krylov workspace of a graph (basis of m + 1 vectors and four work vectors).
+struct Krylov {
+    int m;
+    uint32_t n;
+    float *V;       // basis [(m + 1) * n]
+    float *y;       // d P v
+    float *vh, *yh; // prescaled copies (unweighted graphs)
+    float *e;       // solution of the last solve
+    double *dpart, *dout;
+};
+
+void prepare_run(nkpr_graph *g);
+SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64,
+                       const float *base32, nkpr_stats *st);
+Krylov krylov_setup(nkpr_graph *g);
+void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
+                    double *ms_acc);
+
+// GMRES(m) from e = 0 on A e = rhs until ||A e - rhs||_2 <= rtol ||rhs||_2 or `cap` operator
+// applications.  Classical Gram-Schmidt applied twice; the Hessenberg least-squares problem
+// is solved on the host with Givens rotations.  Returns the number of applications.
+template <typename Apply>
+uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, double rtol,
+                   uint64_t cap, hipStream_t s) {
+    const uint32_t n = k.n;
+    const int mres = k.m;
+    auto dots = [&](const float *w, const float *Vb, uint32_t cnt, double *host) {
+        launch_multidot(w, Vb, n, cnt, n, k.dpart, k.dout, s);
+        NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * cnt, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+    };
+    double beta2 = 0.0;
+    dots(rhs, rhs, 1, &beta2);
+    const double beta = std::sqrt(beta2);
+    if (!(beta > 0.0) || cap == 0) {
+        launch_axpby(rhs, nullptr, e, n, 0.0f, 0.0f, s);
+        return 0;
+    }
+    float *V = k.V;
+    launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
+    std::vector<double> R((size_t)mres * mres, 0.0), cs(mres), sn(mres), gv(mres + 1, 0.0);
+    gv[0] = beta;
+    uint64_t applies = 0;
+    int kk = 0;
+    for (int j = 0; j < mres && applies < cap; ++j) {
+        float *vj = V + (size_t)j * n;
+        float *w = V + (size_t)(j + 1) * n;
+        apply(vj, w);
+        ++applies;
+        std::vector<double> h(j + 2), h2(j + 1);
+        dots(w, V, j + 1, h.data());
+        launch_multiaxpy(w, V, n, j + 1, n, h.data(), false, s);
+        dots(w, V, j + 1, h2.data());
+        launch_multiaxpy(w, V, n, j + 1, n, h2.data(), false, s);
+        for (int i = 0; i <= j; ++i) h[i] += h2[i];
+        double hn2 = 0.0;
+        dots(w, w, 1, &hn2);
+        const double hn = std::sqrt(hn2);
+        h[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+            h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+            h[i] = t;
+        }
+        const double den = std::hypot(h[j], h[j + 1]);
+        cs[j] = den > 0.0 ? h[j] / den : 1.0;
+        sn[j] = den > 0.0 ? h[j + 1] / den : 0.0;
+        h[j] = den;
+        gv[j + 1] = -sn[j] * gv[j];
+        gv[j] = cs[j] * gv[j];
+        for (int i = 0; i <= j; ++i) R[(size_t)i * mres + j] = h[i];
+        kk = j + 1;
+        if (hn > 0.0) launch_axpby(w, nullptr, w, n, (float)(1.0 / hn), 0.0f, s);
+        if (std::fabs(gv[j + 1]) <= rtol * beta || !(hn > 0.0)) break;
+    }
+    std::vector<double> y(kk, 0.0);
+    for (int i = kk - 1; i >= 0; --i) {
+        double t = gv[i];
+        for (int q = i + 1; q < kk; ++q) t -= R[(size_t)i * mres + q] * y[q];
+        y[i] = R[(size_t)i * mres + i] != 0.0 ? t / R[(size_t)i * mres + i] : 0.0;
+    }
+    launch_multiaxpy(e, V, n, kk, n, y.data(), true, s);
+    return applies;
+}
+
+}  // namespace nkpr

@@ -445,6 +445,15 @@ __global__ void init_ctl_kernel(PhaseCtl *ctl, const double *sink_sum, uint32_t 
     }
 }
 
+__global__ void gather_scaled_kernel(const double *in_z, const int32_t *perm, uint32_t n,
+                                     double f, double *out64, float *out32) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = f * in_z[perm[i]];
+    out64[i] = v;
+    if (out32) out32[i] = (float)v;
+}
+
 __global__ void gather_start_kernel(const double *scores_z, const int32_t *perm, uint32_t n,
                                     double *x64, float *x32) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -585,9 +594,12 @@ void launch_sweep(const SweepArgs<T> &a, const FinalizeArgs &f, int kind, bool f
     } else if (kind == kSweepBaseVec) {
         if (folded) launch_pair<T, true, true, false>(a, f, combine_grid, s, ev0, ev1);
         else launch_pair<T, false, true, false>(a, f, combine_grid, s, ev0, ev1);
-    } else {
+    } else if (kind == kSweepResid) {
         if (folded) launch_pair<T, true, false, true>(a, f, combine_grid, s, ev0, ev1);
         else launch_pair<T, false, false, true>(a, f, combine_grid, s, ev0, ev1);
+    } else {
+        if (folded) launch_pair<T, true, true, true>(a, f, combine_grid, s, ev0, ev1);
+        else launch_pair<T, false, true, true>(a, f, combine_grid, s, ev0, ev1);
     }
 }
 
@@ -600,6 +612,7 @@ int sweep_resident_blocks(int device, bool dbl, bool folded, int kind) {
     const void *fn;
     if (dbl) {
         if (kind == kSweepResid) fn = folded ? sweep_fn<double, true, false, true>() : sweep_fn<double, false, false, true>();
+        else if (kind == kSweepBaseResid) fn = folded ? sweep_fn<double, true, true, true>() : sweep_fn<double, false, true, true>();
         else if (kind == kSweepBaseVec) fn = folded ? sweep_fn<double, true, true, false>() : sweep_fn<double, false, true, false>();
         else fn = folded ? sweep_fn<double, true, false, false>() : sweep_fn<double, false, false, false>();
     } else {
@@ -628,6 +641,14 @@ void launch_prescale(const double *x64, const double *inv64, double *xhat64, con
     if (!n || !sink_begin) return;
     hipLaunchKernelGGL(prescale_kernel, dim3(grid1d(n)), dim3(256), 0, s, x64, inv64, xhat64,
                        x32, inv32, xhat32, n, sink_begin);
+    NKPR_HIP(hipGetLastError());
+}
+
+void launch_gather_scaled(const double *in_z, const int32_t *perm, uint32_t n, double f,
+                          double *out64, float *out32, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(gather_scaled_kernel, dim3(grid1d(n)), dim3(256), 0, s, in_z, perm, n, f,
+                       out64, out32);
     NKPR_HIP(hipGetLastError());
 }
 

@@ -52,17 +52,20 @@ def csc_from_edges(z, u, v, w=None, directed=True):
     v = np.asarray(v, dtype=np.int64)
     if directed:
         rows, srcs = v, u
-        seq = np.arange(len(u))
         ws = None if w is None else np.asarray(w, dtype=np.float64)
     else:
+        # entries in insertion order: edge i adds (row u, src v), then (row v, src u)
         loop = u == v
-        rows = np.concatenate([u, v[~loop]])
-        srcs = np.concatenate([v, u[~loop]])
-        idx = np.arange(len(u))
-        seq = np.concatenate([2 * idx, 2 * idx[~loop] + 1])
-        ws = None if w is None else np.concatenate([np.asarray(w, np.float64),
-                                                    np.asarray(w, np.float64)[~loop]])
-    order = np.lexsort((seq, rows))
+        rows = np.stack([u, v], axis=1).ravel()
+        srcs = np.stack([v, u], axis=1).ravel()
+        keep = np.ones(len(rows), bool)
+        keep[1::2] = ~loop
+        rows, srcs = rows[keep], srcs[keep]
+        ws = None
+        if w is not None:
+            wf = np.asarray(w, np.float64)
+            ws = np.stack([wf, wf], axis=1).ravel()[keep]
+    order = np.argsort(rows, kind="stable")
     colptr = np.zeros(z + 1, dtype=np.uint64)
     if len(rows):
         colptr[1:] = np.cumsum(np.bincount(rows, minlength=z))
