@@ -56,9 +56,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
-    ap.add_argument("--workload", choices=["pagerank", "ppr"], default="pagerank",
+    ap.add_argument("--workload", choices=["pagerank", "ppr", "adjoint"], default="pagerank",
                     help="pagerank: one solve on R-MAT (the benchmark line); ppr: batched "
-                         "personalised PageRank on a Barabasi-Albert graph")
+                         "personalised PageRank on a Barabasi-Albert graph; adjoint: forward "
+                         "and backward of differentiable PageRank on R-MAT scale 20")
     ap.add_argument("--ppr-nodes", type=int, default=2_000_000)
     ap.add_argument("--ppr-columns", type=int, default=256)
     args = ap.parse_args()
@@ -68,6 +69,9 @@ def main():
         sys.exit(2)
     if args.workload == "ppr":
         bench_ppr(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
 
     import torch
@@ -326,6 +330,67 @@ def bench_ppr(args):
                  "tile_columns": d0["tile_columns"]},
         "speedup_vs_fp64": sec_d / sec_m,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """Differentiable PageRank: directed R-MAT scale 20, FP32 weights U(0.5,1.5), loss
+    L = sum_i c_i x_i with c ~ N(0,1) (seed 5).  Forward = solve to a certified L1 error <= tol,
+    backward = nkpr_adjoint (dL/dw for every edge and dL/dt) to a relative residual <= tol, both
+    in mixed precision; the FP32-only variant (restarted FP32 GMRES adjoint, FP32 power
+    forward at tol 1e-5) is reported beside it."""
+    import torch
+
+    from networkit_amd import autograd as nkad
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    g = gen.rmat(20, 16, seed=5, device="cuda")
+    torch.cuda.synchronize()
+    z = g["z"]
+    m = g["src"].numel()
+    rng = np.random.default_rng(5)
+    c = torch.tensor(rng.normal(size=z), device=dev)
+    res = {}
+    for name, prec, tol in (("mixed", capi.NKPR_MIXED, args.tol), ("fp32", capi.NKPR_FP32, 1e-5)):
+        op = nkad.PageRankOperator(g["colptr"], g["src"], tol=tol, precision=prec)
+        w = g["weight"].clone().requires_grad_(True)
+        fwd, bwd = [], []
+        for it in range(args.warmup + args.steps):
+            w.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x = op(w)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L = (x.double() * c).sum()
+            L.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if it >= args.warmup:
+                fwd.append(t1 - t0)
+                bwd.append(t2 - t1)
+        f = sum(fwd) / len(fwd)
+        b = sum(bwd) / len(bwd)
+        res[name] = {"forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f,
+                     "forward_sweeps": op.last_forward["iterations"],
+                     "backward_sweeps": op.last_backward["iterations"],
+                     "backward_residual": op.last_backward["l1_error_bound"]}
+        log("%s: forward %.2f ms (%d sweeps), backward %.2f ms (%d sweeps), ratio %.2f" % (
+            name, f * 1e3, res[name]["forward_sweeps"], b * 1e3, res[name]["backward_sweeps"],
+            b / f))
+    r = res["mixed"]
+    out = {"metric": "differentiable PageRank forward + backward time, backward / forward",
+           "value": r["backward_over_forward"], "unit": "ratio", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["forward_ms"] + r["backward_ms"], "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+           "config": {"workload": "forward and adjoint of PageRank on directed R-MAT scale 20, "
+                                  "edge factor 16, FP32 weights U(0.5,1.5), loss sum c_i x_i, "
+                                  "c ~ N(0,1), seed 5; tol %g" % args.tol, "n": z, "m": m},
+           "mixed": r, "fp32": res["fp32"]}
     print(json.dumps(out), flush=True)
 
 

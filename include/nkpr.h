@@ -14,6 +14,7 @@
  *   nkpr_ranking         Centrality::ranking (Centrality.cpp:29-41)
  *   nkpr_ppr_batch       no counterpart: PageRank::run once per teleport distribution
  *   nkpr_adjoint         no counterpart: gradients of the scores (differentiable PageRank)
+ *   nkpr_graph_create_batch / nkpr_run_batch  no counterpart: PageRank::run once per graph
  *   nkpr_graph_set_weights  no counterpart: a new Graph with the same structure
  *   nkpr_params_default  the constructor defaults PageRank(G, 0.85, 1e-8, false), the field
  *                        defaults maxIterations = UINT64_MAX, norm = L2Norm (PageRank.hpp:40-67)
@@ -177,6 +178,21 @@ int nkpr_graph_set_weights(nkpr_graph *g, const float *weight, const double *wei
 int nkpr_adjoint(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
                  const double *scores, double score_sum, const double *grad_scores,
                  double *grad_weight, double *grad_teleport, nkpr_stats *stats);
+
+/* A batch of graphs as one handle (no counterpart in the reference).  `csc` is the
+ * block-diagonal union of the graphs: graph q owns the slots [graph_offsets[q],
+ * graph_offsets[q+1]) (graph_offsets[0] = 0, graph_offsets[num_graphs] = csc->z, host memory)
+ * and no in-list entry may join two graphs. */
+int nkpr_graph_create_batch(const nkpr_csc *csc, uint32_t num_graphs,
+                            const uint64_t *graph_offsets, nkpr_graph **out);
+
+/* PageRank::run on every graph of a batch handle at once.  Each graph follows the rules of
+ * nkpr_run on its own: its own n, teleport (1-d)/n_q, sink mass, stopping rule and iteration
+ * count (a graph that has converged is frozen), normalisation and maximum().  scores[z] is in
+ * the union's slot order (start vectors on entry).  iterations[num_graphs] and
+ * maximum[num_graphs] (host) are optional; stats->iterations counts the sweeps of the union. */
+int nkpr_run_batch(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_t *iterations,
+                   double *maximum, nkpr_stats *stats);
 
 /* Statistics of a batched run (nkpr_ppr_batch). */
 typedef struct {

@@ -25,10 +25,10 @@ NKPR_FP64, NKPR_MIXED, NKPR_FP32 = 0, 1, 2
 NKPR_STOP_STEP, NKPR_STOP_L1_ERROR = 0, 1
 
 # every symbol include/nkpr.h declares
-EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_destroy",
-            "nkpr_graph_get_info", "nkpr_graph_stream", "nkpr_graph_set_weights", "nkpr_run",
-            "nkpr_adjoint", "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error",
-            "nkpr_abi_version"]
+EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch",
+            "nkpr_graph_destroy", "nkpr_graph_get_info", "nkpr_graph_stream",
+            "nkpr_graph_set_weights", "nkpr_run", "nkpr_run_batch", "nkpr_adjoint",
+            "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version"]
 ABI_VERSION = 2
 
 
@@ -102,6 +102,9 @@ def lib():
     L.nkpr_graph_stream.argtypes = [vp]
     L.nkpr_graph_stream.restype = vp
     L.nkpr_run.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.POINTER(Stats)]
+    L.nkpr_graph_create_batch.argtypes = [ctypes.POINTER(Csc), ctypes.c_uint32, vp,
+                                          ctypes.POINTER(vp)]
+    L.nkpr_run_batch.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Stats)]
     L.nkpr_graph_set_weights.argtypes = [vp, vp, vp, vp, ctypes.c_int32]
     L.nkpr_adjoint.argtypes = [vp, ctypes.POINTER(Params), ctypes.POINTER(Csc), vp,
                                ctypes.c_double, vp, vp, vp, ctypes.POINTER(Stats)]
@@ -154,7 +157,7 @@ class DeviceGraph:
     (set ``device=True``).  The arrays only have to live until the constructor returns."""
 
     def __init__(self, z, colptr, src, weight=None, out_wdeg=None, exists=None, weight64=None,
-                 device=False):
+                 device=False, graph_offsets=None):
         self._keep = (colptr, src, weight, out_wdeg, exists, weight64)
         if not device:
             colptr = np.ascontiguousarray(colptr, dtype=np.uint64)
@@ -168,7 +171,14 @@ class DeviceGraph:
                 weight64=_addr(weight64), out_wdeg=_addr(out_wdeg), exists=_addr(exists),
                 device_ptrs=1 if device else 0)
         h = ctypes.c_void_p()
-        check(lib().nkpr_graph_create(ctypes.byref(c), ctypes.byref(h)))
+        self.num_graphs = 0
+        if graph_offsets is None:
+            check(lib().nkpr_graph_create(ctypes.byref(c), ctypes.byref(h)))
+        else:
+            off = np.ascontiguousarray(graph_offsets, dtype=np.uint64)
+            self.num_graphs = len(off) - 1
+            check(lib().nkpr_graph_create_batch(ctypes.byref(c), self.num_graphs, _addr(off),
+                                                ctypes.byref(h)))
         self.handle = h
         self.z = int(z)
         self.device = device
@@ -239,6 +249,21 @@ class DeviceGraph:
         st = Stats()
         check(lib().nkpr_run(self.handle, ctypes.byref(p), _addr(scores), ctypes.byref(st)))
         return st.as_dict()
+
+    def run_batch(self, scores, params=None, **kw):
+        """nkpr_run_batch on a batch handle (graph_offsets given at construction).  Returns
+        (stats dict, per-graph iterations, per-graph maximum)."""
+        p = params if params is not None else default_params(**kw)
+        if isinstance(scores, np.ndarray):
+            assert scores.dtype == np.float64 and scores.flags.c_contiguous
+            assert len(scores) == self.z
+        k = max(self.num_graphs, 1)
+        its = np.zeros(k, dtype=np.uint64)
+        mx = np.zeros(k, dtype=np.float64)
+        st = Stats()
+        check(lib().nkpr_run_batch(self.handle, ctypes.byref(p), _addr(scores), _addr(its),
+                                   _addr(mx), ctypes.byref(st)))
+        return st.as_dict(), its[:self.num_graphs], mx[:self.num_graphs]
 
     def ppr_batch(self, seeds, scores=None, seed_weights=None, params=None, **kw):
         """nkpr_ppr_batch: one personalised PageRank per seed set (a list of k sequences of

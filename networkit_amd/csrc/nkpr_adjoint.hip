@@ -27,7 +27,8 @@
 
 namespace nkpr {
 
-void create_graph(const nkpr_csc *c, nkpr_graph *g);
+void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off = nullptr,
+                  uint32_t n_groups = 0);
 
 namespace {
 

@@ -21,7 +21,8 @@
 #include "nkpr_run.hpp"
 
 namespace nkpr {
-void create_graph(const nkpr_csc *c, nkpr_graph *g);
+void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off = nullptr,
+                  uint32_t n_groups = 0);
 }
 
 using namespace nkpr;
@@ -398,7 +399,9 @@ int nkpr_params_default(nkpr_params *p) {
     });
 }
 
-int nkpr_graph_create(const nkpr_csc *csc, nkpr_graph **out) {
+namespace {
+int create_handle(const nkpr_csc *csc, nkpr_graph **out, const uint64_t *group_off,
+                  uint32_t n_groups) {
     return guarded([&] {
         NKPR_CHECK(out != nullptr, "out is NULL");
         *out = nullptr;
@@ -415,12 +418,36 @@ int nkpr_graph_create(const nkpr_csc *csc, nkpr_graph **out) {
         g->device = dev;
         try {
             NKPR_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-            create_graph(csc, g);
+            create_graph(csc, g, group_off, n_groups);
         } catch (...) {
             nkpr_graph_destroy(g);
             throw;
         }
         *out = g;
+    });
+}
+}  // namespace
+
+int nkpr_graph_create(const nkpr_csc *csc, nkpr_graph **out) {
+    return create_handle(csc, out, nullptr, 0);
+}
+
+int nkpr_graph_create_batch(const nkpr_csc *csc, uint32_t num_graphs,
+                            const uint64_t *graph_offsets, nkpr_graph **out) {
+    if (graph_offsets == nullptr || num_graphs == 0) {
+        g_last_error = "a batch needs num_graphs >= 1 and graph_offsets";
+        return NKPR_ERR_INVALID;
+    }
+    return create_handle(csc, out, graph_offsets, num_graphs);
+}
+
+int nkpr_run_batch(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_t *iterations,
+                   double *maximum, nkpr_stats *stats) {
+    nkpr_stats local;
+    return guarded([&] {
+        NKPR_CHECK(g != nullptr, "graph is NULL");
+        DeviceGuard dg(g->device);
+        run_batch_impl(g, p, scores, iterations, maximum, stats ? stats : &local);
     });
 }
 
@@ -445,6 +472,7 @@ int nkpr_graph_get_info(const nkpr_graph *g, nkpr_graph_info *info) {
         info->n = g->n;
         info->m = g->m;
         info->sinks = g->n - g->sink_begin;
+        for (uint32_t q = 0; q < g->n_groups; ++q) info->sinks += g->goff[q + 1] - g->gsink[q];
         info->max_in_degree = g->max_in_degree;
         info->work_items = g->n_items;
         info->device_bytes = g->perm.bytes + g->colptr.bytes + g->src.bytes + g->coef32.bytes +
@@ -461,7 +489,10 @@ void *nkpr_graph_stream(const nkpr_graph *g) { return g ? (void *)g->stream : nu
 
 int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *stats) {
     nkpr_stats local;
-    return guarded([&] { run_impl(g, p, scores, stats ? stats : &local); });
+    return guarded([&] {
+        NKPR_CHECK(g == nullptr || g->n_groups == 0, "a batch handle runs with nkpr_run_batch");
+        run_impl(g, p, scores, stats ? stats : &local);
+    });
 }
 
 int nkpr_graph_set_weights(nkpr_graph *g, const float *weight, const double *weight64,
@@ -479,6 +510,7 @@ int nkpr_adjoint(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
     nkpr_stats local;
     return guarded([&] {
         NKPR_CHECK(g != nullptr, "graph is NULL");
+        NKPR_CHECK(g->n_groups == 0, "the adjoint runs on single graphs");
         DeviceGuard dg(g->device);
         adjoint_impl(g, p, csc, scores, score_sum, grad_scores, grad_weight, grad_teleport,
                      stats ? stats : &local);
@@ -492,6 +524,7 @@ int nkpr_ppr_batch(nkpr_graph *g, const nkpr_params *p, uint32_t k,
     nkpr_batch_stats local;
     return guarded([&] {
         NKPR_CHECK(g != nullptr, "graph is NULL");
+        NKPR_CHECK(g->n_groups == 0, "batched personalised PageRank runs on single graphs");
         DeviceGuard dg(g->device);
         ppr_batch_impl(g, p, k, seed_offsets, seed_nodes, seed_weights, scores,
                        stats ? stats : &local, col_iterations, col_error_bound);

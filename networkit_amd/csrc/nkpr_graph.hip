@@ -32,7 +32,7 @@ namespace {
 inline unsigned blocks_for(uint64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // validation flags
-constexpr uint32_t kBadSrc = 1, kBadPtr = 2, kBadExists = 4;
+constexpr uint32_t kBadSrc = 1, kBadPtr = 2, kBadExists = 4, kBadGroup = 8;
 
 __global__ void validate_ptr_kernel(const uint64_t *colptr, uint64_t z, uint64_t m,
                                     const uint8_t *exists, uint32_t *flags) {
@@ -66,6 +66,48 @@ __global__ void relabel_keys_kernel(const uint32_t *outcnt, const uint8_t *exist
     // ascending sort of keys == descending out-degree; deleted slots last
     keys[u] = ex ? (0xFFFFFFFEu - outcnt[u]) : 0xFFFFFFFFu;
     vals[u] = (int32_t)u;
+}
+
+// Graph of every slot of a batch: the group whose slot range [off[g], off[g+1]) holds it.
+__global__ void group_id_kernel(const uint64_t *off, uint32_t groups, uint64_t z, uint32_t *gid) {
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= z) return;
+    uint32_t lo = 0, hi = groups;  // last g with off[g] <= u
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (off[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    gid[u] = lo;
+}
+
+// Batch relabelling: graph id first, then out-degree descending, deleted slots last.
+__global__ void group_keys_kernel(const uint32_t *outcnt, const uint8_t *exists,
+                                  const uint32_t *gid, uint32_t groups, uint64_t z,
+                                  uint64_t *keys, int32_t *vals, uint32_t *cnt_exist,
+                                  uint32_t *cnt_nonsink) {
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= z) return;
+    const bool ex = !exists || exists[u];
+    keys[u] = ex ? (((uint64_t)gid[u] << 32) | (0xFFFFFFFEu - outcnt[u]))
+                 : (((uint64_t)groups << 32) | 0xFFFFFFFFu);
+    vals[u] = (int32_t)u;
+    if (ex) {
+        atomicAdd(&cnt_exist[gid[u]], 1u);
+        if (outcnt[u] > 0) atomicAdd(&cnt_nonsink[gid[u]], 1u);
+    }
+}
+
+// Every in-edge must stay inside its graph.
+__global__ void group_edges_kernel(const uint64_t *colptr, const int32_t *src, const uint32_t *gid,
+                                   uint64_t z, uint32_t *flags) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < z; r += nwaves) {
+        const uint32_t g = gid[r];
+        for (uint64_t i = colptr[r] + lane; i < colptr[r + 1]; i += 64)
+            if (gid[src[i]] != g) atomicOr(flags, kBadGroup);
+    }
 }
 
 __global__ void count_kernel(const uint32_t *outcnt, const uint8_t *exists, uint64_t z,
@@ -129,11 +171,11 @@ __global__ void edge_output_kernel(const uint64_t *keys, const uint32_t *vals, u
     }
 }
 
-__global__ void inv_degree_kernel(const int32_t *perm, uint32_t n, uint32_t sink_begin,
+__global__ void inv_degree_kernel(const int32_t *perm, uint32_t n, const uint32_t *outcnt,
                                   const double *wdeg, double *inv64, float *inv32) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const double v = (r < sink_begin) ? 1.0 / wdeg[perm[r]] : 0.0;
+    const double v = (outcnt[perm[r]] > 0) ? 1.0 / wdeg[perm[r]] : 0.0;
     inv64[r] = v;
     inv32[r] = (float)v;
 }
@@ -460,7 +502,8 @@ int bits_for(uint64_t v) {
 
 }  // namespace
 
-void create_graph(const nkpr_csc *c, nkpr_graph *g) {
+void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
+                  uint32_t n_groups) {
     const auto t0 = std::chrono::steady_clock::now();
     NKPR_CHECK(c != nullptr, "csc is NULL");
     NKPR_CHECK(c->z < 0x7FFFFFFFull, "node-id bound must be < 2^31");
@@ -554,7 +597,66 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g) {
 
     DevBuf k1, k2, v1, v2, tmp;
     g->perm.alloc(4 * std::max<uint64_t>(n, 1));
-    if (z) {
+    if (group_off) {
+        // batch of graphs: relabel inside every graph, graphs stay contiguous in id order
+        NKPR_CHECK(n_groups >= 1, "a batch needs at least one graph");
+        NKPR_CHECK(group_off[0] == 0 && group_off[n_groups] == z, "graph offsets must span [0, z)");
+        for (uint32_t q = 0; q < n_groups; ++q)
+            NKPR_CHECK(group_off[q] <= group_off[q + 1], "graph offsets must be non-decreasing");
+        DevBuf offb, gidb, ce, cn, fl;
+        offb.alloc(8 * ((uint64_t)n_groups + 1));
+        NKPR_HIP(hipMemcpyAsync(offb.p, group_off, 8 * ((uint64_t)n_groups + 1), hipMemcpyHostToDevice, s));
+        gidb.alloc(4 * std::max<uint64_t>(z, 1));
+        ce.alloc(4 * (uint64_t)n_groups);
+        cn.alloc(4 * (uint64_t)n_groups);
+        fl.alloc(4);
+        NKPR_HIP(hipMemsetAsync(ce.p, 0, 4 * (uint64_t)n_groups, s));
+        NKPR_HIP(hipMemsetAsync(cn.p, 0, 4 * (uint64_t)n_groups, s));
+        NKPR_HIP(hipMemsetAsync(fl.p, 0, 4, s));
+        DevBuf k64a, k64b;
+        k64a.alloc(8 * std::max<uint64_t>(z, 1));
+        k64b.alloc(8 * std::max<uint64_t>(z, 1));
+        v1.alloc(4 * std::max<uint64_t>(z, 1));
+        v2.alloc(4 * std::max<uint64_t>(z, 1));
+        if (z) {
+            hipLaunchKernelGGL(group_id_kernel, dim3(blocks_for(z)), dim3(256), 0, s,
+                               offb.as<uint64_t>(), n_groups, z, gidb.as<uint32_t>());
+            hipLaunchKernelGGL(group_keys_kernel, dim3(blocks_for(z)), dim3(256), 0, s,
+                               outcnt_b.as<uint32_t>(), exists, gidb.as<uint32_t>(), n_groups, z,
+                               k64a.as<uint64_t>(), v1.as<int32_t>(), ce.as<uint32_t>(), cn.as<uint32_t>());
+            const unsigned eb = (unsigned)std::min<uint64_t>(blocks_for(z * 64), 1u << 18);
+            hipLaunchKernelGGL(group_edges_kernel, dim3(eb), dim3(256), 0, s, colptr_in, src_in,
+                               gidb.as<uint32_t>(), z, fl.as<uint32_t>());
+            NKPR_HIP(hipGetLastError());
+            hipcub::DoubleBuffer<uint64_t> kb(k64a.as<uint64_t>(), k64b.as<uint64_t>());
+            hipcub::DoubleBuffer<int32_t> vb(v1.as<int32_t>(), v2.as<int32_t>());
+            const int end_bit = 32 + bits_for((uint64_t)n_groups + 1);
+            size_t tb = 0;
+            NKPR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (size_t)z, 0, end_bit, s));
+            tmp.alloc(tb);
+            NKPR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, kb, vb, (size_t)z, 0, end_bit, s));
+            if (n)
+                NKPR_HIP(hipMemcpyAsync(g->perm.p, vb.Current(), 4 * (size_t)n,
+                                        hipMemcpyDeviceToDevice, s));
+        }
+        std::vector<uint32_t> he(n_groups), hn(n_groups);
+        uint32_t hf = 0;
+        NKPR_HIP(hipMemcpyAsync(he.data(), ce.p, 4 * (uint64_t)n_groups, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipMemcpyAsync(hn.data(), cn.p, 4 * (uint64_t)n_groups, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipMemcpyAsync(&hf, fl.p, 4, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+        NKPR_CHECK(!(hf & kBadGroup), "an edge joins two graphs of the batch");
+        g->n_groups = n_groups;
+        g->goff.assign((size_t)n_groups + 1, 0);
+        g->gsink.assign(n_groups, 0);
+        g->gslot.assign(group_off, group_off + n_groups + 1);
+        for (uint32_t q = 0; q < n_groups; ++q) {
+            g->goff[q + 1] = g->goff[q] + he[q];
+            g->gsink[q] = g->goff[q] + hn[q];
+        }
+        // no global sink range: sinks are the tail of every graph's range
+        g->sink_begin = n;
+    } else if (z) {
         k1.alloc(4 * z);
         k2.alloc(4 * z);
         v1.alloc(4 * z);
@@ -644,7 +746,7 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g) {
         g->inv64.alloc(8 * std::max<uint64_t>(n, 1));
         if (n)
             hipLaunchKernelGGL(inv_degree_kernel, dim3(blocks_for(n)), dim3(256), 0, s,
-                               g->perm.as<int32_t>(), n, g->sink_begin, wdeg, g->inv64.as<double>(),
+                               g->perm.as<int32_t>(), n, outcnt_b.as<uint32_t>(), wdeg, g->inv64.as<double>(),
                                g->inv32.as<float>());
         NKPR_HIP(hipGetLastError());
     }

@@ -254,6 +254,8 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
 void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
                   const double *scores, double score_sum, const double *grad_scores,
                   double *grad_weight, double *grad_teleport, nkpr_stats *st);
+void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_t *iterations,
+                    double *maximum, nkpr_stats *st);
 void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint64_t *seed_off,
                     const int32_t *seeds, const double *seed_w, double *scores,
                     nkpr_batch_stats *st, uint64_t *col_iterations, double *col_bound);
@@ -320,6 +322,14 @@ struct nkpr_graph {
     nkpr::DevBuf b_piece32, b_piece64, b_partials, b_col, b_ctl, b_sink_part, b_factor;
     nkpr::DevBuf b_x64a, b_x64b, b_x32a, b_x32b, b_r32, b_zbuf;
     nkpr::BatchCtl *b_ctl_host = nullptr;
+
+    // batch of graphs (nkpr_graph_create_batch): graph q owns compact ids [goff[q], goff[q+1]),
+    // its sinks are [gsink[q], goff[q+1]); slots [gslot[q], gslot[q+1])
+    uint32_t n_groups = 0;
+    std::vector<uint32_t> goff, gsink;
+    std::vector<uint64_t> gslot;
+    nkpr::DevBuf q_chunks, q_part, q_ctl, q_add, q_factor, q_flag;
+    uint32_t q_nchunks = 0;
 
     // adjoint: the transposed operator d P^T as a graph of its own (built on first use; it
     // shares this graph's stream and device)

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <functional>
 #include <vector>
 
 #include "nkpr_internal.hpp"
@@ -62,7 +63,8 @@ inline Schedule make_schedule(nkpr_graph *g, uint32_t workers) {
 // Launch sweeps of one phase until `done` or `max_launch` sweeps.  Returns completed sweeps.
 template <typename T>
 uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule rule,
-                   uint64_t max_launch, bool timing, double *ms_acc) {
+                   uint64_t max_launch, bool timing, double *ms_acc,
+                   const std::function<void(const T *, T *)> &post = {}) {
     nkpr_graph *g = rc.g;
     const bool folded = g->weighted;
     const bool dbl = sizeof(T) == 8;
@@ -105,6 +107,7 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
             hipEvent_t e0 = timing ? g->events[2 * j] : nullptr;
             hipEvent_t e1 = timing ? g->events[2 * j + 1] : nullptr;
             launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
+            if (post) post(v.x[cur], v.x[nxt]);
         }
         NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,
                                 g->stream));
