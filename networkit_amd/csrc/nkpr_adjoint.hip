@@ -72,6 +72,17 @@ __global__ void transpose_edges_kernel(const uint32_t *order, uint64_t m, const 
     }
 }
 
+// Coefficients of the transposed entries after new forward weights.
+__global__ void transpose_coef_kernel(const uint32_t *order, uint64_t m, const float *c32,
+                                      const double *c64, float *tw32, double *tw64) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = order[j];
+        tw32[j] = c32[e];
+        tw64[j] = c64[e];
+    }
+}
+
 __global__ void widen_kernel(const uint32_t *in, uint64_t n, uint64_t *out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = in[i];
@@ -166,8 +177,23 @@ double device_dot(const double *a, const double *b, uint64_t n, double *part, do
 // The graph of d P^T: rows are forward sources, entries their out-edges with the forward's
 // folded coefficients.
 void build_adjoint_graph(nkpr_graph *g) {
-    if (g->adj) return;
     batch_layout(g);
+    if (g->adj && g->adj_stale) {
+        // same structure, new coefficients: fold them into the transposed graph in place
+        const uint64_t m = g->m;
+        DevBuf tw32, tw64;
+        tw32.alloc(4 * std::max<uint64_t>(m, 1));
+        tw64.alloc(8 * std::max<uint64_t>(m, 1));
+        if (m)
+            hipLaunchKernelGGL(transpose_coef_kernel, dim3(nblocks(m)), dim3(256), 0, g->stream,
+                               g->adj_order.as<uint32_t>(), m, g->b_coef32.as<float>(),
+                               g->b_coef64.as<double>(), tw32.as<float>(), tw64.as<double>());
+        NKPR_HIP(hipGetLastError());
+        set_weights(g->adj.get(), tw32.as<float>(), tw64.as<double>(), g->adj_ones.as<double>(), 1);
+        g->adj_stale = false;
+        return;
+    }
+    if (g->adj) return;
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
@@ -237,6 +263,15 @@ void build_adjoint_graph(nkpr_graph *g) {
     c.device_ptrs = 1;
     create_graph(&c, a.get());
     g->adj = std::move(a);
+    g->adj_stale = false;
+    if (g->weighted) {
+        // kept to refresh the coefficients when the forward weights change
+        g->adj_order.alloc(4 * std::max<uint64_t>(m, 1));
+        if (m) NKPR_HIP(hipMemcpyAsync(g->adj_order.p, vb.Current(), 4 * m, hipMemcpyDeviceToDevice, s));
+        std::swap(g->adj_ones.p, ones.p);
+        std::swap(g->adj_ones.bytes, ones.bytes);
+        NKPR_HIP(hipStreamSynchronize(s));
+    }
 }
 
 }  // namespace

@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -598,12 +600,27 @@ template <typename T> int occupancy_blocks(int device, int lpr, bool folded) {
 // Host side
 // ---------------------------------------------------------------------------------------------
 
-// Builds the plain CSC copy used by the batched sweeps (once per graph).
+// Builds the plain CSC copy used by the batched sweeps (once per graph; after new weights
+// only the coefficients are copied again).
 void batch_layout(nkpr_graph *g) {
-    if (g->b_ready) return;
+    if (g->b_ready && g->b_coef_ready) return;
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
+    if (g->b_ready) {
+        const uint64_t rows = (uint64_t)g->n_d + g->n_s;
+        if (rows > 0 && g->weighted) {
+            hipLaunchKernelGGL(plain_layout_kernel, dim3((unsigned)((rows * 64 + 255) / 256)), dim3(256), 0, s,
+                               g->colptr.as<uint64_t>(), g->drow.as<uint32_t>(),
+                               g->dptr.as<uint32_t>(), g->n_d, g->srow.as<uint32_t>(),
+                               g->sptr.as<uint32_t>(), g->n_s, g->n_slices, g->src.as<int32_t>(),
+                               g->coef32.as<float>(), g->coef64.as<double>(), g->b_src.as<int32_t>(),
+                               g->b_coef32.as<float>(), g->b_coef64.as<double>());
+            NKPR_HIP(hipGetLastError());
+        }
+        g->b_coef_ready = true;
+        return;
+    }
     g->b_rowptr.alloc(4 * ((size_t)n + 1));
     hipLaunchKernelGGL(rowptr32_kernel, dim3(blocks_for((uint64_t)n + 1)), dim3(256), 0, s,
                        g->colptr.as<uint64_t>(), n, g->b_rowptr.as<uint32_t>());
@@ -635,6 +652,7 @@ void batch_layout(nkpr_graph *g) {
                             hipMemcpyDeviceToHost, s));
     NKPR_HIP(hipStreamSynchronize(s));
     g->b_ready = true;
+    g->b_coef_ready = true;
 }
 
 // Light-row list and heavy-row pieces for one threshold (cached per threshold).
@@ -1002,6 +1020,22 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
 
     const uint64_t max_it = std::max<uint64_t>(p->max_iterations, 1);
     double *result = nullptr;
+    const bool debug = std::getenv("NKPR_DEBUG") != nullptr;
+    auto debug_phase = [&](const char *name, uint64_t sweeps) {
+        if (!debug) return;
+        std::vector<BatchCol> c(k);
+        NKPR_HIP(hipMemcpyAsync(c.data(), br.col, sizeof(BatchCol) * k, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+        double mb = 0.0, ml1 = 0.0;
+        uint32_t fin = 0;
+        for (const auto &x : c) {
+            mb = std::max(mb, x.bound);
+            ml1 = std::max(ml1, x.l1);
+            fin += x.final_;
+        }
+        std::fprintf(stderr, "[nkpr batch] %s: %llu sweeps, max bound %.3e, max l1 %.3e, certified %u/%u\n",
+                     name, (unsigned long long)sweeps, mb, ml1, fin, k);
+    };
     if (p->precision == NKPR_FP64) {
         configure<double>(br, x64[0], x64[1]);
         PhaseRule r{};
@@ -1050,6 +1084,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
             ra.damp = d;
             phase_init<float>(br, x32[0]);
             uint64_t total = batch_phase<float>(br, x32[0], x32[1], kBPlain, ra, max_it, nullptr, nullptr, &st->ms_sweeps_fp32);
+            debug_phase("fp32 power", total);
             st->sweeps_fp32 += total;
             hipLaunchKernelGGL(f32_to_f64_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
                                (const float *)x32[total & 1], x64[0], (uint64_t)cells);
@@ -1064,6 +1099,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
                 rr.damp = d;
                 phase_init<double>(br, x64[0]);
                 total += batch_phase<double>(br, x64[0], x64[1], kBResid, rr, 1, nullptr, r32, &st->ms_sweeps_fp64);
+                debug_phase("fp64 residual", 1);
                 st->sweeps_fp64 += 1;
                 result = x64[1];
                 NKPR_HIP(hipMemcpyAsync(br.ctl_host, br.ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
@@ -1090,6 +1126,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
                 rc.damp = d;
                 phase_init<float>(br, x32[0]);
                 const uint64_t kc = batch_phase<float>(br, x32[0], x32[1], kBBase, rc, rc.max_iterations, r32, nullptr, &st->ms_sweeps_fp32);
+                debug_phase("fp32 correction", kc);
                 st->sweeps_fp32 += kc;
                 total += kc;
                 hipLaunchKernelGGL(batch_update_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,

@@ -265,6 +265,23 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
     const uint32_t n = g->n;
     const bool folded = g->weighted;
     const bool dist = rc.p->sink_handling == NKPR_SINK_DISTRIBUTE;
+    if (binned_enabled(g)) {
+        binned_layout(g);
+        while (timing && g->events.size() < 2) {
+            hipEvent_t e;
+            NKPR_HIP(hipEventCreate(&e));
+            g->events.push_back(e);
+        }
+        binned_apply(g, v, w, rc.p->damp, dist, timing ? g->events[0] : nullptr,
+                     timing ? g->events[1] : nullptr);
+        if (timing) {
+            float ms = 0.f;
+            NKPR_HIP(hipEventSynchronize(g->events[1]));
+            NKPR_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
+            *ms_acc += ms;
+        }
+        return;
+    }
     if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kr.vh, n, g->sink_begin, s);
     const double *sink_sum = nullptr;
     if (dist && g->sink_begin < n) {

@@ -842,9 +842,9 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
     NKPR_HIP(hipGetLastError());
     NKPR_HIP(hipStreamSynchronize(s));
     // derived copies follow the coefficients
-    g->b_ready = false;
-    g->b_split_ready = false;
-    g->adj.reset();
+    g->b_coef_ready = false;
+    g->bn_coef_ready = false;
+    g->adj_stale = true;
 }
 
 }  // namespace nkpr

@@ -249,6 +249,10 @@ struct BatchTune {
 void batch_layout(nkpr_graph *g);
 void batch_split(nkpr_graph *g, uint32_t heavy);
 void ensure_buf(DevBuf &b, size_t bytes);
+bool binned_enabled(const nkpr_graph *g);
+void binned_layout(nkpr_graph *g);
+void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
+                  hipEvent_t ev0, hipEvent_t ev1);
 void set_weights(nkpr_graph *g, const float *w, const double *w64, const double *out_wdeg,
                  int device_ptrs);
 void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
@@ -311,7 +315,7 @@ struct nkpr_graph {
     std::vector<hipEvent_t> events;
 
     // batched personalised PageRank: plain CSC copy and its row split (built on first use)
-    bool b_ready = false, b_split_ready = false;
+    bool b_ready = false, b_split_ready = false, b_coef_ready = false;
     nkpr::DevBuf b_rowptr, b_src, b_coef32, b_coef64;
     std::vector<uint32_t> b_rowptr_host;
     std::vector<int32_t> b_perm_host;
@@ -331,9 +335,17 @@ struct nkpr_graph {
     nkpr::DevBuf q_chunks, q_part, q_ctl, q_add, q_factor, q_flag;
     uint32_t q_nchunks = 0;
 
+    // binned FP32 operator for the GMRES correction solves (nkpr_binned.hip, built on first use)
+    bool bn_ready = false, bn_coef_ready = false;
+    nkpr::DevBuf bn_order, bn_seg, bn_src, bn_dst, bn_coef, bn_val, bn_p1, bn_p2, bn_y;
+    uint32_t bn_np1 = 0, bn_np2 = 0, bn_nbins = 0, bn_multi = 0;
+
     // adjoint: the transposed operator d P^T as a graph of its own (built on first use; it
     // shares this graph's stream and device)
     std::unique_ptr<nkpr_graph> adj;
+    bool adj_stale = false;          // forward coefficients changed since adj was built
+    nkpr::DevBuf adj_order;          // uint32 [m] plain-copy edge of every transposed entry
+    nkpr::DevBuf adj_ones;           // double [n] unit out-degrees of the transposed graph
     nkpr::DevBuf a_vec[6];
 
     nkpr_graph() = default;
