@@ -38,6 +38,9 @@ def algorithmic_bytes(m, n, weighted, kind):
     elif kind == "fp32corr":
         e = 12 if weighted else 8
         v = 20 if weighted else 28
+    elif kind == "fp64":  # plain FP64 sweep
+        e = 20 if weighted else 12
+        v = 24 if weighted else 40
     else:  # fp64 residual sweep (writes the FP32 residual too)
         e = 20 if weighted else 12
         v = 28 if weighted else 44
@@ -56,10 +59,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
-    ap.add_argument("--workload", choices=["pagerank", "ppr", "adjoint"], default="pagerank",
-                    help="pagerank: one solve on R-MAT (the benchmark line); ppr: batched "
-                         "personalised PageRank on a Barabasi-Albert graph; adjoint: forward "
-                         "and backward of differentiable PageRank on R-MAT scale 20")
+    ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "adjoint"],
+                    default="pagerank",
+                    help="pagerank: one solve on R-MAT (the benchmark line); rhg: the same on "
+                         "an undirected random hyperbolic graph; ppr: batched personalised "
+                         "PageRank on a Barabasi-Albert graph; adjoint: forward and backward "
+                         "of differentiable PageRank on R-MAT scale 20")
+    ap.add_argument("--rhg-nodes", type=int, default=10_000_000)
     ap.add_argument("--ppr-nodes", type=int, default=2_000_000)
     ap.add_argument("--ppr-columns", type=int, default=256)
     args = ap.parse_args()
@@ -70,6 +76,8 @@ def main():
     if args.workload == "ppr":
         bench_ppr(args)
         return
+    if args.workload == "rhg":
+        args.no_cpu = True
     if args.workload == "adjoint":
         bench_adjoint(args)
         return
@@ -82,11 +90,23 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     t0 = time.time()
-    g = gen.rmat(args.scale, args.edge_factor, seed=2, device="cuda")
+    if args.workload == "rhg":
+        g = gen.rhg(args.rhg_nodes, 20.0, 3.0, seed=3)
+        workload = ("PageRank solve to certified L1 error <= %g (damp 0.85, no sink handling) of "
+                    "an undirected threshold random hyperbolic graph, n=%d, average degree 20, "
+                    "power-law exponent 3, temperature 0, unit weights, seed 3"
+                    % (args.tol, args.rhg_nodes))
+    else:
+        g = gen.rmat(args.scale, args.edge_factor, seed=2, device="cuda")
+        workload = ("PageRank solve to certified L1 error <= %g (damp 0.85, no sink handling) of "
+                    "a directed R-MAT graph, scale %d, edge factor %d, a/b/c/d=.57/.19/.19/.05, "
+                    "duplicates and self-loops removed, FP32 weights U(0.5,1.5), seed 2"
+                    % (args.tol, args.scale, args.edge_factor))
+    weighted = g["weight"] is not None
     torch.cuda.synchronize()
     z = g["z"]
     m = g["src"].numel()
-    log("R-MAT scale %d: z=%d m=%d generated in %.1f s" % (args.scale, z, m, time.time() - t0))
+    log("%s graph: z=%d m=%d generated in %.1f s" % (args.workload, z, m, time.time() - t0))
     t0 = time.time()
     dg = capi.DeviceGraph(z, g["colptr"], g["src"], g["weight"], device=True)
     info = dg.info()
@@ -167,11 +187,11 @@ def main():
     # every FP32 sweep of the mixed path is an operator application y = d P v (GMRES)
     n_f32 = sum(s["sweeps_fp32"] for s in st_mixed)
     ms_f32 = sum(s["ms_sweeps_fp32"] for s in st_mixed)
-    bytes_f32 = n_f32 * algorithmic_bytes(m, n, True, "fp32")
+    bytes_f32 = n_f32 * algorithmic_bytes(m, n, weighted, "fp32")
     achieved = bytes_f32 / (ms_f32 / 1e3) / 1e9
     ms64 = sum(s["ms_sweeps_fp64"] for s in st_fp64)
     n64 = sum(s["sweeps_fp64"] for s in st_fp64)
-    achieved64 = n64 * (20 * m + 24 * n) / (ms64 / 1e3) / 1e9
+    achieved64 = n64 * algorithmic_bytes(m, n, weighted, "fp64") / (ms64 / 1e3) / 1e9
     log("mixed: %.1f ms/solve, %d fp32 (GMRES) + %d fp64 sweeps, %d refinements, L1 err %.2e; "
         "fp32 sweep %.3f ms = %.0f GB/s" % (sec_mixed * 1e3, s0["sweeps_fp32"],
                                             s0["sweeps_fp64"], s0["refinements"],
@@ -196,18 +216,14 @@ def main():
         "dtype": "fp32+fp64",
         "data": "synthetic",
         "config": {
-            "workload": "PageRank solve to certified L1 error <= %g (damp 0.85, no sink "
-                        "handling) of a directed R-MAT graph, scale %d, edge factor %d, "
-                        "a/b/c/d=.57/.19/.19/.05, duplicates and self-loops removed, "
-                        "FP32 weights U(0.5,1.5), seed 2" % (args.tol, args.scale,
-                                                              args.edge_factor),
+            "workload": workload,
             "n": z, "m": m, "sinks": info["sinks"], "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(m, z),
-                     "kernel": "FP32 pull sweep", "bytes_per_sweep": algorithmic_bytes(
-                         m, n, True, "fp32"),
+                     "kernel": "FP32 operator application y = d P v (GMRES)",
+                     "bytes_per_sweep": algorithmic_bytes(m, n, weighted, "fp32"),
                      "ms_per_sweep": ms_f32 / n_f32},
         "cpu_baseline": cpu,
         "mixed": {"ms_per_solve": sec_mixed * 1e3, "sweeps_fp32": s0["sweeps_fp32"],
@@ -419,7 +435,7 @@ def cpu_baseline(g, z, m, seconds):
     t0 = time.time()
     colptr = g["colptr"].cpu().numpy().astype(np.uint64)
     src = g["src"].cpu().numpy().astype(np.uint64)
-    w = g["weight"].cpu().numpy().astype(np.float64)
+    w = None if g["weight"] is None else g["weight"].cpu().numpy().astype(np.float64)
     og = oracle.InLists(z, colptr, src, w)
     log("cpu baseline: graph on host in %.1f s, %d threads" % (time.time() - t0, threads))
     t1 = oracle.time_sweeps(og, 0.85, 1, threads)

@@ -37,10 +37,10 @@ inline unsigned nblocks(uint64_t n, int t = 256) {
 }
 
 __global__ void row_of_edge_kernel(const uint32_t *rowptr, uint32_t n, uint32_t *row) {
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (w >= n) return;
-    for (uint32_t e = rowptr[w] + lane; e < rowptr[w + 1]; e += 64) row[e] = (uint32_t)w;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw)
+        for (uint32_t e = rowptr[w] + lane; e < rowptr[w + 1]; e += 64) row[e] = (uint32_t)w;
 }
 
 __global__ void key_val_kernel(const int32_t *src, uint64_t m, uint32_t *key, uint32_t *val,
@@ -207,7 +207,7 @@ void build_adjoint_graph(nkpr_graph *g) {
     ptr32.alloc(4 * ((uint64_t)n + 1));
     NKPR_HIP(hipMemsetAsync(cnt.p, 0, 4 * ((uint64_t)n + 1), s));
     if (n)
-        hipLaunchKernelGGL(row_of_edge_kernel, dim3((unsigned)(((uint64_t)n * 64 + 255) / 256)),
+        hipLaunchKernelGGL(row_of_edge_kernel, dim3((unsigned)std::min<uint64_t>(((uint64_t)n * 64 + 255) / 256, 1u << 18)),
                            dim3(256), 0, s, g->b_rowptr.as<uint32_t>(), n, row.as<uint32_t>());
     if (m)
         hipLaunchKernelGGL(key_val_kernel, dim3(nblocks(m)), dim3(256), 0, s,

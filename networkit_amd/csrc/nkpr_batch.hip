@@ -451,38 +451,36 @@ __global__ void col_range_sum_kernel(const T *x, uint32_t ld, uint32_t begin, ui
 // X[u][j] = scores[j * z + perm[u]] (j < k), 0 in the padding columns.
 __global__ void batch_gather_kernel(const double *scores, uint64_t z, const int32_t *perm,
                                     uint32_t n, uint32_t k, uint32_t ld, double *x64) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)n * ld) return;
-    const uint32_t u = (uint32_t)(i / ld), j = (uint32_t)(i % ld);
-    x64[i] = j < k ? scores[(uint64_t)j * z + (uint32_t)perm[u]] : 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * ld; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = (uint32_t)(i / ld), j = (uint32_t)(i % ld);
+        x64[i] = j < k ? scores[(uint64_t)j * z + (uint32_t)perm[u]] : 0.0;
+    }
 }
 
 __global__ void batch_scatter_kernel(const double *x64, const int32_t *perm, uint32_t n,
                                      uint32_t k, uint32_t ld, uint64_t z, const double *factor,
                                      double *scores) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)n * k) return;
-    const uint32_t j = (uint32_t)(i / n), u = (uint32_t)(i % n);
-    scores[(uint64_t)j * z + (uint32_t)perm[u]] = x64[(size_t)u * ld + j] / factor[j];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * k; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(i / n), u = (uint32_t)(i % n);
+        scores[(uint64_t)j * z + (uint32_t)perm[u]] = x64[(size_t)u * ld + j] / factor[j];
+    }
 }
 
 __global__ void f64_to_f32_kernel(const double *in, float *out, uint64_t cnt) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cnt) out[i] = (float)in[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
 }
 
 __global__ void f32_to_f64_kernel(const float *in, double *out, uint64_t cnt) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cnt) out[i] = (double)in[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) out[i] = (double)in[i];
 }
 
 // After a correction solve: x = y for certified columns, x += e for the others.
 __global__ void batch_update_kernel(double *x, const double *y, const float *e, uint32_t ld,
                                     uint64_t cnt, const BatchCol *col) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    const uint32_t c = (uint32_t)(i % ld);
-    x[i] = col[c].final_ ? y[i] : x[i] + (double)e[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = (uint32_t)(i % ld);
+        x[i] = col[c].final_ ? y[i] : x[i] + (double)e[i];
+    }
 }
 
 // A correction solve stops each column at the target its last residual sweep set.
@@ -491,10 +489,29 @@ __global__ void batch_set_solve_tol_kernel(BatchCol *col, uint32_t ld) {
     if (c < ld) col[c].solve_tol = col[c].target;
 }
 
+// Column totals of per-block column sums: out[c] = sum_b part[b * ld + c].
+__global__ void col_total_kernel(const double *part, uint32_t nblocks, uint32_t ld, double *out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ld) return;
+    double t = 0.0;
+    for (uint32_t b = 0; b < nblocks; ++b) t += part[(size_t)b * ld + c];
+    out[c] = t;
+}
+
+// e0[u][c] += shift * total[c]: with a column-stochastic operator (no sinks, or distributed
+// sinks) 1^T (I - dP) = (1-d) 1^T, so the solution of e = dPe + r sums to 1^T r / (1-d).
+// Starting there leaves no error along the eigenvalue-d mode, which would otherwise decay only
+// as d^k.
+__global__ void col_shift_kernel(float *e, uint32_t ld, uint64_t cnt, const double *total,
+                                 double shift) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        e[i] += (float)(shift * total[i % ld]);
+}
+
 // Correction solves start from e = r.
 __global__ void copy_f32_kernel(const float *in, float *out, uint64_t cnt) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cnt) out[i] = in[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) out[i] = in[i];
 }
 
 // Plain CSC out of the sliced sweep layout: row r's in-edges in slice order.
@@ -503,9 +520,10 @@ __global__ void plain_layout_kernel(const uint64_t *colptr, const uint32_t *drow
                                     const uint32_t *sptr, uint32_t n_s, uint32_t S,
                                     const int32_t *src, const float *c32, const double *c64,
                                     int32_t *osrc, float *o32, double *o64) {
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (w >= (uint64_t)n_d + n_s) return;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+         w < (uint64_t)n_d + n_s; w += nw) {
     uint32_t row, nseg;
     if (w < n_d) {
         row = drow[w];
@@ -533,11 +551,17 @@ __global__ void plain_layout_kernel(const uint64_t *colptr, const uint32_t *drow
         }
         out += e - b;
     }
+    }
 }
 
 __global__ void rowptr32_kernel(const uint64_t *colptr, uint32_t n, uint32_t *rowptr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= n) rowptr[i] = (uint32_t)colptr[i];
+}
+
+// capped grid of a grid-stride kernel
+unsigned grid_for(uint64_t cnt, int threads = 256) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((cnt + threads - 1) / threads, 1u << 18));
 }
 
 unsigned blocks_for(uint64_t cnt, int threads = 256) {
@@ -610,7 +634,7 @@ void batch_layout(nkpr_graph *g) {
     if (g->b_ready) {
         const uint64_t rows = (uint64_t)g->n_d + g->n_s;
         if (rows > 0 && g->weighted) {
-            hipLaunchKernelGGL(plain_layout_kernel, dim3((unsigned)((rows * 64 + 255) / 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL(plain_layout_kernel, dim3((unsigned)std::min<uint64_t>((rows * 64 + 255) / 256, 1u << 18)), dim3(256), 0, s,
                                g->colptr.as<uint64_t>(), g->drow.as<uint32_t>(),
                                g->dptr.as<uint32_t>(), g->n_d, g->srow.as<uint32_t>(),
                                g->sptr.as<uint32_t>(), g->n_s, g->n_slices, g->src.as<int32_t>(),
@@ -632,7 +656,7 @@ void batch_layout(nkpr_graph *g) {
     }
     const uint64_t rows = (uint64_t)g->n_d + g->n_s;
     if (rows > 0) {
-        const unsigned blocks = (unsigned)((rows * 64 + 255) / 256);
+        const unsigned blocks = (unsigned)std::min<uint64_t>((rows * 64 + 255) / 256, 1u << 18);
         hipLaunchKernelGGL(plain_layout_kernel, dim3(blocks), dim3(256), 0, s,
                            g->colptr.as<uint64_t>(), g->drow.as<uint32_t>(),
                            g->dptr.as<uint32_t>(), g->n_d, g->srow.as<uint32_t>(),
@@ -1014,7 +1038,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
         sz = g->b_zbuf.as<double>();
         NKPR_HIP(hipMemcpyAsync(sz, scores, 8 * (size_t)z * k, hipMemcpyHostToDevice, s));
     }
-    hipLaunchKernelGGL(batch_gather_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
+    hipLaunchKernelGGL(batch_gather_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
                        (const double *)sz, z, g->perm.as<int32_t>(), n, k, ld, x64[0]);
     NKPR_HIP(hipGetLastError());
 
@@ -1052,7 +1076,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
         ensure_buf(g->b_x32a, 4 * cells);
         ensure_buf(g->b_x32b, 4 * cells);
         float *x32[2] = {g->b_x32a.as<float>(), g->b_x32b.as<float>()};
-        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
                            (const double *)x64[0], x32[0], (uint64_t)cells);
         NKPR_HIP(hipGetLastError());
         configure<float>(br, x32[0], x32[1]);
@@ -1066,7 +1090,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
             phase_init<float>(br, x32[0]);
             const uint64_t K = batch_phase<float>(br, x32[0], x32[1], kBPlain, r, max_it, nullptr, nullptr, &st->ms_sweeps_fp32);
             st->sweeps_fp32 = K;
-            hipLaunchKernelGGL(f32_to_f64_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
+            hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
                                (const float *)x32[K & 1], x64[0], (uint64_t)cells);
             NKPR_HIP(hipGetLastError());
             result = x64[0];
@@ -1086,7 +1110,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
             uint64_t total = batch_phase<float>(br, x32[0], x32[1], kBPlain, ra, max_it, nullptr, nullptr, &st->ms_sweeps_fp32);
             debug_phase("fp32 power", total);
             st->sweeps_fp32 += total;
-            hipLaunchKernelGGL(f32_to_f64_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
+            hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
                                (const float *)x32[total & 1], x64[0], (uint64_t)cells);
             NKPR_HIP(hipGetLastError());
             configure<double>(br, x64[0], x64[1]);
@@ -1111,9 +1135,22 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
                 if (open == 0 || total >= max_it || round >= 30) break;
                 ++st->refinements;
                 // correction: e0 = r; per column stop at its own target step
-                hipLaunchKernelGGL(copy_f32_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
+                hipLaunchKernelGGL(copy_f32_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
                                    (const float *)r32, x32[0], (uint64_t)cells);
                 NKPR_HIP(hipGetLastError());
+                if (g->sink_begin == n || p->sink_handling == NKPR_SINK_DISTRIBUTE) {
+                    const uint32_t rpb = std::max<uint32_t>(256, (n + 1023) / 1024);
+                    const uint32_t blocks = (n + rpb - 1) / rpb;
+                    hipLaunchKernelGGL(col_range_sum_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                                       (const float *)r32, ld, 0u, n, rpb, br.sink_part);
+                    double *tot = br.sink_part + (size_t)1024 * ld;
+                    hipLaunchKernelGGL(col_total_kernel, dim3((ld + 255) / 256), dim3(256), 0, s,
+                                       (const double *)br.sink_part, blocks, ld, tot);
+                    hipLaunchKernelGGL(col_shift_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
+                                       x32[0], ld, (uint64_t)cells, (const double *)tot,
+                                       d / (1.0 - d) / (double)n);
+                    NKPR_HIP(hipGetLastError());
+                }
                 hipLaunchKernelGGL(batch_set_solve_tol_kernel, dim3((ld + 255) / 256), dim3(256), 0, s,
                                    br.col, ld);
                 NKPR_HIP(hipGetLastError());
@@ -1129,7 +1166,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
                 debug_phase("fp32 correction", kc);
                 st->sweeps_fp32 += kc;
                 total += kc;
-                hipLaunchKernelGGL(batch_update_kernel, dim3(blocks_for(cells)), dim3(256), 0, s,
+                hipLaunchKernelGGL(batch_update_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
                                    x64[0], (const double *)x64[1], (const float *)x32[kc & 1], ld,
                                    (uint64_t)cells, (const BatchCol *)br.col);
                 NKPR_HIP(hipGetLastError());
@@ -1155,7 +1192,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
     }
     ensure_buf(g->b_factor, 8 * (size_t)k);
     NKPR_HIP(hipMemcpyAsync(g->b_factor.p, factor.data(), 8 * (size_t)k, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for((uint64_t)n * k)), dim3(256), 0, s,
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(grid_for((uint64_t)n * k)), dim3(256), 0, s,
                        (const double *)result, g->perm.as<int32_t>(), n, k, ld, z,
                        g->b_factor.as<double>(), sz);
     NKPR_HIP(hipGetLastError());

@@ -338,7 +338,7 @@ struct nkpr_graph {
     // binned FP32 operator for the GMRES correction solves (nkpr_binned.hip, built on first use)
     bool bn_ready = false, bn_coef_ready = false;
     nkpr::DevBuf bn_order, bn_seg, bn_src, bn_dst, bn_coef, bn_val, bn_p1, bn_p2, bn_y;
-    uint32_t bn_np1 = 0, bn_np2 = 0, bn_nbins = 0, bn_multi = 0;
+    uint32_t bn_np1 = 0, bn_np2 = 0, bn_nbins = 0, bn_multi = 0, bn_big = 0;
 
     // adjoint: the transposed operator d P^T as a graph of its own (built on first use; it
     // shares this graph's stream and device)

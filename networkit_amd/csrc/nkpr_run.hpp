@@ -162,8 +162,9 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
                     double *ms_acc);
 
 // GMRES(m) from e = 0 on A e = rhs until ||A e - rhs||_2 <= rtol ||rhs||_2 or `cap` operator
-// applications.  Classical Gram-Schmidt applied twice; the Hessenberg least-squares problem
-// is solved on the host with Givens rotations.  Returns the number of applications.
+// applications.  Classical Gram-Schmidt with selective reorthogonalisation; the Hessenberg
+// least-squares problem is solved on the host with Givens rotations.  Returns the number of
+// applications.
 template <typename Apply>
 uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, double rtol,
                    uint64_t cap, hipStream_t s) {
@@ -192,15 +193,24 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         float *w = V + (size_t)(j + 1) * n;
         apply(vj, w);
         ++applies;
-        std::vector<double> h(j + 2), h2(j + 1);
-        dots(w, V, j + 1, h.data());
+        // one classical Gram-Schmidt pass; w is stored as basis vector j+1, so the same
+        // multi-dot also returns w.w and the new norm follows from Pythagoras.  A second pass
+        // runs only when the pass cancelled more than half of w (selective
+        // reorthogonalisation, the criterion of Daniel, Gragg, Kaufman and Stewart).
+        std::vector<double> h(j + 2), h2(j + 2);
+        dots(w, V, j + 2, h.data());
+        double ww = h[j + 1];
         launch_multiaxpy(w, V, n, j + 1, n, h.data(), false, s);
-        dots(w, V, j + 1, h2.data());
-        launch_multiaxpy(w, V, n, j + 1, n, h2.data(), false, s);
-        for (int i = 0; i <= j; ++i) h[i] += h2[i];
-        double hn2 = 0.0;
-        dots(w, w, 1, &hn2);
-        const double hn = std::sqrt(hn2);
+        double hn2 = ww;
+        for (int i = 0; i <= j; ++i) hn2 -= h[i] * h[i];
+        if (!(hn2 > 0.5 * ww)) {
+            dots(w, V, j + 2, h2.data());
+            launch_multiaxpy(w, V, n, j + 1, n, h2.data(), false, s);
+            for (int i = 0; i <= j; ++i) h[i] += h2[i];
+            hn2 = h2[j + 1];
+            for (int i = 0; i <= j; ++i) hn2 -= h2[i] * h2[i];
+        }
+        const double hn = std::sqrt(std::max(hn2, 0.0));
         h[j + 1] = hn;
         for (int i = 0; i < j; ++i) {
             const double t = cs[i] * h[i] + sn[i] * h[i + 1];

@@ -34,6 +34,11 @@ def lib():
         L.nkgen_erdos_renyi.restype = ctypes.c_int64
         L.nkgen_barabasi_albert.argtypes = [u64, u64, u64, vp, vp, ctypes.c_int64]
         L.nkgen_barabasi_albert.restype = ctypes.c_int64
+        L.nkgen_rhg_points.argtypes = [u64, u64, ctypes.c_double, ctypes.c_double, vp, vp, vp]
+        L.nkgen_rhg_points.restype = ctypes.c_int
+        L.nkgen_rhg.argtypes = [u64, u64, ctypes.c_double, ctypes.c_double, vp, vp,
+                                ctypes.c_int64]
+        L.nkgen_rhg.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -155,3 +160,31 @@ def barabasi_albert(n, k, seed=4):
     got = L.nkgen_barabasi_albert(seed, n, k, _ptr(u), _ptr(v), m)
     assert got == m
     return u[:m], v[:m]
+
+
+def rhg_points(n, avg_degree=20.0, gamma=3.0, seed=3):
+    """Host copy of the threshold random hyperbolic graph's points: (radius r[n], angle
+    theta[n], disk radius R)."""
+    r = np.empty(n, dtype=np.float64)
+    t = np.empty(n, dtype=np.float64)
+    R = ctypes.c_double(0.0)
+    if lib().nkgen_rhg_points(seed, n, avg_degree, gamma, _ptr(r), _ptr(t), ctypes.byref(R)):
+        raise ValueError("bad random hyperbolic graph parameters")
+    return r, t, R.value
+
+
+def rhg(n, avg_degree=20.0, gamma=3.0, seed=3):
+    """Undirected threshold random hyperbolic graph (temperature 0) built on the GPU: n points,
+    power-law exponent gamma, average degree ~avg_degree (asymptotic disk radius).  Returns the
+    in-lists as torch CUDA tensors: dict(z, colptr int64[n+1], src int32[m], weight None)."""
+    import torch
+    m = lib().nkgen_rhg(seed, n, avg_degree, gamma, None, None, 0)
+    if m < 0:
+        raise RuntimeError("nkgen_rhg failed (%d)" % m)
+    colptr = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+    src = torch.empty(max(m, 1), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    got = lib().nkgen_rhg(seed, n, avg_degree, gamma, _ptr(colptr), _ptr(src), m)
+    if got != m:
+        raise RuntimeError("nkgen_rhg failed (%d)" % got)
+    return dict(z=n, colptr=colptr, src=src[:m], weight=None)

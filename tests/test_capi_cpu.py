@@ -67,6 +67,10 @@ def test_null_arguments_are_errors_not_crashes():
         L.nkpr_graph_create(None, None) == capi.NKPR_ERR_NODEVICE
     assert L.nkpr_run(None, None, None, None) != capi.NKPR_OK
     assert L.nkpr_ppr_batch(None, None, 0, None, None, None, None, None, None, None) != capi.NKPR_OK
+    assert L.nkpr_run_batch(None, None, None, None, None, None) != capi.NKPR_OK
+    assert L.nkpr_adjoint(None, None, None, None, 1.0, None, None, None, None) != capi.NKPR_OK
+    assert L.nkpr_graph_set_weights(None, None, None, None, 0) != capi.NKPR_OK
+    assert L.nkpr_graph_create_batch(None, 0, None, None) == capi.NKPR_ERR_INVALID
     assert L.nkpr_graph_destroy(None) == capi.NKPR_OK
     assert L.nkpr_params_default(None) == capi.NKPR_ERR_INVALID
 
