@@ -147,6 +147,12 @@ def main():
             gbs = (per_edge * m + per_node * n) / (ms / 1e3) / 1e9
             log("%s: %d sweeps, %.3f ms/sweep, %.0f GB/s algorithmic, %.3g edge-updates/s" % (
                 name, k, ms, gbs, m / (ms / 1e3)))
+        # one mixed solve: its FP32 sweeps are the operator applications of the GMRES corrections
+        st = solve(capi.NKPR_MIXED, args.tol)
+        ms = st["ms_sweeps_fp32"] / st["sweeps_fp32"]
+        log("operator (%s): %d applications, %.3f ms each, %.0f GB/s algorithmic" % (
+            operator_name(m), st["sweeps_fp32"], ms,
+            algorithmic_bytes(m, n, weighted, "fp32") / (ms / 1e3) / 1e9))
         return
 
     # tightly converged FP64 reference (not timed)
@@ -222,7 +228,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(m, z),
-                     "kernel": "FP32 operator application y = d P v (GMRES)",
+                     "kernel": "FP32 operator application y = d P v (GMRES): " + operator_name(m),
                      "bytes_per_sweep": algorithmic_bytes(m, n, weighted, "fp32"),
                      "ms_per_sweep": ms_f32 / n_f32},
         "cpu_baseline": cpu,
@@ -410,10 +416,19 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def operator_name(m):
+    """The kernels of one application of the GMRES operator: the library takes the binned
+    two-pass operator for graphs of 2^26 edges and more (NKPR_BINNED=0/1 forces either)."""
+    env = os.environ.get("NKPR_BINNED", "")
+    binned = (int(env) != 0) if env else m >= 1 << 26
+    return ("binned (bin_scatter_kernel + bin_gather_kernel + bin_reduce_kernel)" if binned
+            else "pull sweep (sweep_kernel<float> + combine_kernel<float>)")
+
+
 def measured_traffic(m, n):
-    """HBM bytes per FP32 sweep (sweep + combine kernels) from the committed rocprofv3
+    """HBM bytes per application of the GMRES operator from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the counters' reading of a 1 GiB
-    copy; None when the summary was taken on another graph."""
+    copy; None when the summary was taken on another graph or another operator."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "pagerank_rmat26_traffic.json")
     try:
@@ -421,9 +436,9 @@ def measured_traffic(m, n):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("m") != m or t.get("n") != n:
+    if t.get("m") != m or t.get("n") != n or t.get("operator") != operator_name(m):
         return None
-    return t["fp32_sweep_traffic_bytes"]
+    return t["operator_traffic_bytes"]
 
 
 def cpu_baseline(g, z, m, seconds):

@@ -265,8 +265,7 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
     const uint32_t n = g->n;
     const bool folded = g->weighted;
     const bool dist = rc.p->sink_handling == NKPR_SINK_DISTRIBUTE;
-    if (binned_enabled(g)) {
-        binned_layout(g);
+    if (binned_enabled(g) && binned_layout(g)) {
         while (timing && g->events.size() < 2) {
             hipEvent_t e;
             NKPR_HIP(hipEventCreate(&e));

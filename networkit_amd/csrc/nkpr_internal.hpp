@@ -250,7 +250,7 @@ void batch_layout(nkpr_graph *g);
 void batch_split(nkpr_graph *g, uint32_t heavy);
 void ensure_buf(DevBuf &b, size_t bytes);
 bool binned_enabled(const nkpr_graph *g);
-void binned_layout(nkpr_graph *g);
+bool binned_layout(nkpr_graph *g);
 void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
                   hipEvent_t ev0, hipEvent_t ev1);
 void set_weights(nkpr_graph *g, const float *w, const double *w64, const double *out_wdeg,
@@ -336,9 +336,13 @@ struct nkpr_graph {
     uint32_t q_nchunks = 0;
 
     // binned FP32 operator for the GMRES correction solves (nkpr_binned.hip, built on first use)
-    bool bn_ready = false, bn_coef_ready = false;
-    nkpr::DevBuf bn_order, bn_seg, bn_src, bn_dst, bn_coef, bn_val, bn_p1, bn_p2, bn_y;
-    uint32_t bn_np1 = 0, bn_np2 = 0, bn_nbins = 0, bn_multi = 0, bn_big = 0;
+    bool bn_ready = false, bn_coef_ready = false, bn_off = false;
+    uint64_t bn_mp = 0;  // edges of the binned copy, padding included
+    nkpr::DevBuf bn_order, bn_pos, bn_src, bn_dst, bn_coef, bn_val, bn_p1, bn_p2, bn_mb, bn_part,
+        bn_vmax;
+    double bn_insum = 0.0;
+    uint32_t bn_np1 = 0, bn_np2 = 0, bn_multi = 0, bn_nmb = 0;
+    uint32_t bn_grid1 = 0, bn_grid2 = 0;  // persistent grid sizes of the two passes
 
     // adjoint: the transposed operator d P^T as a graph of its own (built on first use; it
     // shares this graph's stream and device)

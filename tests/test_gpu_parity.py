@@ -341,3 +341,4 @@ def test_mixed_binned_operator(monkeypatch, weighted, sinks, item):
     assert np.abs(x - ref).sum() <= 1e-9
     # the correction solves converge as with the pull operator (a wrong product would not)
     assert st["sweeps_fp32"] <= 40 and st["refinements"] <= 4
+
