@@ -352,8 +352,12 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
                                               (uint64_t)std::max(cus, 1) * 8);
     const bool timing = p->timing != 0;
     Krylov kr = krylov_setup(a);
+    // w = A v (residual updates) and w = B v (the GMRES Krylov operator), A = I - B
     auto apply = [&](const float *v, float *w) {
-        apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32);
+        apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32, true);
+    };
+    auto apply_b = [&](const float *v, float *w) {
+        apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32, false);
     };
     auto sweep_args = [&]() {
         SweepArgs<double> sa{};
@@ -383,7 +387,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
         launch_copy_f32(a->base32.as<float>(), r, n, s);
         auto norm2 = [&](const float *v) {
             double h = 0.0;
-            launch_multidot(v, v, n, 1, n, kr.dpart, kr.dout, s);
+            launch_multidot(v, v, kr.ld, 1, n, kr.dpart, kr.dout, s);
             NKPR_HIP(hipMemcpyAsync(&h, kr.dout, 8, hipMemcpyDeviceToHost, s));
             NKPR_HIP(hipStreamSynchronize(s));
             return std::sqrt(h);
@@ -392,9 +396,9 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
         double r2 = g2;
         for (int round = 0; round < kMaxRefinements && total < cap && r2 > p->tol * g2; ++round) {
             const double rtol = std::min(0.5, std::max(1e-7, p->tol * g2 / r2));
-            total += gmres_f32(kr, apply, r, kr.e, rtol, cap - total, s);
+            total += gmres_f32(kr, apply_b, r, kr.e, rtol, cap - total, s);
             launch_add_correction(lam, kr.e, n, s);
-            float *w = kr.V + (size_t)n;  // free once the solve has returned
+            float *w = kr.V + kr.ld;  // free once the solve has returned
             apply(kr.e, w);
             ++total;
             launch_axpby(r, w, r, n, 1.0f, -1.0f, s);
@@ -431,7 +435,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
             ++st->refinements;
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / l1));
             const uint64_t left = p->max_iterations > total + 1 ? p->max_iterations - total - 1 : 0;
-            const uint64_t kc = gmres_f32(kr, apply, a->r32.as<float>(), kr.e, rtol, left, s);
+            const uint64_t kc = gmres_f32(kr, apply_b, a->r32.as<float>(), kr.e, rtol, left, s);
             st->sweeps_fp32 += kc;
             st->sweeps_fp32_correction += kc;
             total += kc;

@@ -312,10 +312,11 @@ __device__ __forceinline__ void stream_quads(unsigned long long *acc, const uint
 // wave streams an equal, contiguous share of the item's quads (64 consecutive quads per load,
 // one per lane) into the bin's LDS accumulator.  A bin held by one item is written out as
 // w = v - (d y + st) at once; a split bin's items write their accumulators to slots of their
-// own.
+// own.  Without `minus` the output is B v = d y + st instead (the operator whose Krylov space
+// GMRES builds).
 __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
     const uint4 *items, uint32_t nitems, const uint16_t *bdst, const float *bval, long long *part,
-    uint32_t n, const uint32_t *vmax_bits, double insum, const float *v, float d,
+    uint32_t n, const uint32_t *vmax_bits, double insum, const float *v, bool minus, float d,
     const double *sink_sum, float *w, uint32_t *queue) {
     __shared__ unsigned long long acc[kBin];
     __shared__ uint32_t slot;
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
             if (u >= n) break;
             if (item.w & kItemSole) {
                 const float y = (float)((double)(long long)acc[i] * unscale);
-                w[u] = v[u] - (d * y + st);
+                w[u] = minus ? v[u] - (d * y + st) : d * y + st;
             } else {
                 part[(size_t)(item.w >> 2) * kBin + i] = (long long)acc[i];
             }
@@ -350,10 +351,10 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
 }
 
 // Bins split over several items: y = the items' partial accumulators added in item order, then
-// w = v - (d y + st).  mb[j] = {bin, first slot, slot count}.
+// w = v - (d y + st) (minus) or w = d y + st.  mb[j] = {bin, first slot, slot count}.
 __global__ void bin_reduce_kernel(const uint4 *mb, uint32_t nmb, const long long *part, uint32_t n,
-                                  const uint32_t *vmax_bits, double insum, const float *v, float d,
-                                  const double *sink_sum, float *w) {
+                                  const uint32_t *vmax_bits, double insum, const float *v, bool minus,
+                                  float d, const double *sink_sum, float *w) {
     const double unscale = ldexp(1.0, -fixed_exponent(vmax_bits, insum));
     const float st = sink_sum ? (float)(d * *sink_sum / (double)n) : 0.f;
     const uint64_t total = (uint64_t)nmb * kBin;
@@ -366,7 +367,7 @@ __global__ void bin_reduce_kernel(const uint4 *mb, uint32_t nmb, const long long
         long long acc = 0;
         for (uint32_t q = 0; q < b.z; ++q) acc += part[(size_t)(b.y + q) * kBin + i];
         const float y = (float)((double)acc * unscale);
-        w[u] = v[u] - (d * y + st);
+        w[u] = minus ? v[u] - (d * y + st) : d * y + st;
     }
 }
 
@@ -622,9 +623,9 @@ bool binned_layout(nkpr_graph *g) {
     return true;
 }
 
-// w = v - d P v (- d S(v)/n): the GMRES operator through the binned passes.
+// w = B v = d P v (+ d S(v)/n) through the binned passes, or w = A v = v - B v (minus).
 void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
-                  hipEvent_t ev0, hipEvent_t ev1) {
+                  bool minus, hipEvent_t ev0, hipEvent_t ev1) {
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     if (!n) return;
@@ -654,11 +655,11 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
     hipLaunchKernelGGL(bin_gather_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(g->bn_np2, 1), g->bn_grid2)),
                        dim3(kPassThreads), 0, s, g->bn_p2.as<uint4>(), g->bn_np2,
                        g->bn_dst.as<uint16_t>(), g->bn_val.as<float>(), g->bn_part.as<long long>(), n,
-                       (const uint32_t *)ctl, g->bn_insum, v, (float)damp, sink_sum, w, ctl + 2);
+                       (const uint32_t *)ctl, g->bn_insum, v, minus, (float)damp, sink_sum, w, ctl + 2);
     if (g->bn_nmb)
         hipLaunchKernelGGL(bin_reduce_kernel, dim3(nb((uint64_t)g->bn_nmb * kBin)), dim3(256), 0, s,
                            g->bn_mb.as<uint4>(), g->bn_nmb, (const long long *)g->bn_part.as<long long>(), n,
-                           (const uint32_t *)ctl, g->bn_insum, v, (float)damp, sink_sum, w);
+                           (const uint32_t *)ctl, g->bn_insum, v, minus, (float)damp, sink_sum, w);
     if (ev1) NKPR_HIP(hipEventRecord(ev1, s));
     NKPR_HIP(hipGetLastError());
 }

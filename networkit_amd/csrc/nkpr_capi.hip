@@ -193,8 +193,8 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         double *ynew = g->x64b.as<double>();   // FP64 sweep output
         double *xh = g->xh64a.as<double>();
         Krylov kr = krylov_setup(g);
-        auto apply = [&](const float *v, float *w) {
-            apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32);
+        auto apply_b = [&](const float *v, float *w) {
+            apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32, false);
         };
         for (int round = 0;; ++round) {
             // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
@@ -225,7 +225,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
             const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
-            const uint64_t kc = gmres_f32(kr, apply, g->r32.as<float>(), kr.e, rtol, left, s);
+            const uint64_t kc = gmres_f32(kr, apply_b, g->r32.as<float>(), kr.e, rtol, left, s);
             st->sweeps_fp32 += kc;
             st->sweeps_fp32_correction += kc;
             total += kc;
@@ -243,23 +243,26 @@ Krylov krylov_setup(nkpr_graph *g) {
     ensure(g->r32, 4 * n);
     Krylov k{};
     k.m = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 20), 2), 64);
-    ensure(g->kbasis, 4 * (size_t)(k.m + 1) * n);
-    ensure(g->kwork, 4 * (size_t)4 * n);
-    ensure(g->kdot, 8 * ((size_t)kDotGroup * 1024 + 128));
+    const size_t ld = (n + 63) / 64 * 64;  // every vector on a 256-byte boundary
+    ensure(g->kbasis, 4 * (size_t)(k.m + 1) * ld);
+    ensure(g->kwork, 4 * (size_t)4 * ld);
+    ensure(g->kdot, 8 * ((size_t)kDotPartials + 128));
     k.n = g->n;
+    k.ld = ld;
     k.V = g->kbasis.as<float>();
     k.y = g->kwork.as<float>();
-    k.vh = k.y + n;
-    k.yh = k.y + 2 * n;
-    k.e = k.y + 3 * n;
+    k.vh = k.y + ld;
+    k.yh = k.y + 2 * ld;
+    k.e = k.y + 3 * ld;
     k.dpart = g->kdot.as<double>();
-    k.dout = k.dpart + (size_t)kDotGroup * 1024;
+    k.dout = k.dpart + (size_t)kDotPartials;
     return k;
 }
 
-// w = A v = v - d P v: one FP32 sweep with no teleport term (sinks per the run's rule).
+// w = A v = v - B v (minus) or w = B v, B v = d P v: one FP32 sweep with no teleport term
+// (sinks per the run's rule).
 void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
-                    double *ms_acc) {
+                    double *ms_acc, bool minus) {
     nkpr_graph *g = rc.g;
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
@@ -271,7 +274,7 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
             NKPR_HIP(hipEventCreate(&e));
             g->events.push_back(e);
         }
-        binned_apply(g, v, w, rc.p->damp, dist, timing ? g->events[0] : nullptr,
+        binned_apply(g, v, w, rc.p->damp, dist, minus, timing ? g->events[0] : nullptr,
                      timing ? g->events[1] : nullptr);
         if (timing) {
             float ms = 0.f;
@@ -298,7 +301,8 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
     a.n = n;
     a.sink_begin = g->sink_begin;
     a.distribute = dist ? 1 : 0;
-    Vecs<float> vv{{const_cast<float *>(v), kr.y}, {kr.vh, kr.yh}};
+    // the sweep writes B v to kr.y (A v follows) or straight to w
+    Vecs<float> vv{{const_cast<float *>(v), minus ? kr.y : w}, {kr.vh, kr.yh}};
     PhaseRule r{};
     r.kind = kRuleStep;
     r.norm = NKPR_L1;
@@ -306,7 +310,7 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
     r.max_iterations = 1;
     r.damp = rc.p->damp;
     run_phase<float>(rc, a, vv, kSweepPlain, r, 1, timing, ms_acc);
-    launch_axpby(v, kr.y, w, n, 1.0f, -1.0f, s);
+    if (minus) launch_axpby(v, kr.y, w, n, 1.0f, -1.0f, s);
 }
 
 }  // namespace nkpr

@@ -138,12 +138,11 @@ __global__ void group_sink_term_kernel(const GroupCtl *gc, const uint32_t *gn, u
     if (q < groups) term[q] = gn[q] ? damp * gc[q].sink / (double)gn[q] : 0.0;
 }
 
-// w[u] = v[u] - y[u] - term[q]: the union operator A v with per-graph sink terms
-__global__ void group_apply_kernel(const Chunk *ch, const float *v, const float *y,
-                                   const double *term, float *w) {
+// w[u] = y[u] + term[q]: the union operator B v = d P v with per-graph sink terms
+__global__ void group_apply_kernel(const Chunk *ch, const float *y, const double *term, float *w) {
     const Chunk c = ch[blockIdx.x];
     const float t = (float)term[c.group];
-    for (uint32_t u = c.begin + threadIdx.x; u < c.end; u += blockDim.x) w[u] = v[u] - y[u] - t;
+    for (uint32_t u = c.begin + threadIdx.x; u < c.end; u += blockDim.x) w[u] = y[u] + t;
 }
 
 __global__ void group_scatter_kernel(const Chunk *ch, const double *x, const int32_t *perm,
@@ -384,13 +383,14 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
             Krylov kr = krylov_setup(g);
             const int dist = p->sink_handling == NKPR_SINK_DISTRIBUTE;
             // A v = v - d P v - (per graph) d S_q(v) / n_q
-            auto apply = [&](const float *v, float *w) {
+            // w = B v = d P v (+ per-graph sink terms), the GMRES Krylov operator (A = I - B)
+            auto apply_b = [&](const float *v, float *w) {
                 if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kr.vh, n, n, s);
                 launch_init_ctl(ctl, nullptr, g->claim.as<uint32_t>(), s);
                 SweepArgs<float> a = args32();
                 a.base_vec = nullptr;
                 a.tele = 0.0;
-                Vecs<float> vv{{const_cast<float *>(v), kr.y}, {kr.vh, kr.yh}};
+                Vecs<float> vv{{const_cast<float *>(v), dist ? kr.y : w}, {kr.vh, kr.yh}};
                 run_phase<float>(rc, a, vv, kSweepPlain, counting_rule(1), 1, timing, &st->ms_sweeps_fp32);
                 if (dist) {
                     PhaseRule r0{};
@@ -404,9 +404,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                     hipLaunchKernelGGL(group_sink_term_kernel, dim3(nblk(G)), dim3(256), 0, s,
                                        (const GroupCtl *)q.gc, (const uint32_t *)q.gn, G, damp, q.add);
                     hipLaunchKernelGGL(group_apply_kernel, dim3(q.nch), dim3(256), 0, s, (const Chunk *)q.chunks,
-                                       v, (const float *)kr.y, (const double *)q.add, w);
-                } else {
-                    launch_axpby(v, kr.y, w, n, 1.0f, -1.0f, s);
+                                       (const float *)kr.y, (const double *)q.add, w);
                 }
                 NKPR_HIP(hipGetLastError());
             };
@@ -441,7 +439,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                 const double target = 0.5 * p->tol * min_sum * (1.0 - damp) / (2.0 * damp);
                 const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(l1, 1e-300)));
                 const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;
-                const uint64_t kc = gmres_f32(kr, apply, g->r32.as<float>(), kr.e, rtol, left, s);
+                const uint64_t kc = gmres_f32(kr, apply_b, g->r32.as<float>(), kr.e, rtol, left, s);
                 st->sweeps_fp32 += kc;
                 st->sweeps_fp32_correction += kc;
                 total += kc;

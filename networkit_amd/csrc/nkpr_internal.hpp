@@ -212,14 +212,23 @@ void launch_range_max(const double *x, uint32_t n, double *partials, double *out
                       hipStream_t s);
 void launch_init_ctl(PhaseCtl *ctl, const double *sink_sum, uint32_t *claim, hipStream_t s);
 
-// FP32 Krylov vector kernels (nkpr_krylov.hip).  V holds vectors of length n at `stride`.
-constexpr int kDotGroup = 8;
-// out[i] = <w, V[i]> (FP64 accumulation) for i < count; partials: kDotGroup * 1024 doubles
-void launch_multidot(const float *w, const float *V, uint64_t stride, uint32_t count, uint32_t n,
+// FP32 Krylov vector kernels (nkpr_krylov.hip).  V holds vectors of length n at stride ld (a
+// multiple of 4 floats; every vector 16-byte aligned).
+constexpr int kDotGroup = 24;   // vectors per launch
+constexpr int kDotStride = 32;  // doubles per block in the partials buffer (>= kDotGroup + 1)
+constexpr int kDotPartials = kDotStride * 1024;
+struct Coefs {
+    float c[kDotGroup];
+};
+// out[i] = <w, V[i]> (FP64 accumulation) for i < count; partials: kDotPartials doubles
+void launch_multidot(const float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                      double *partials, double *out, hipStream_t s);
 // assign: w = sum_i coef[i] V[i];  otherwise w -= sum_i coef[i] V[i]
-void launch_multiaxpy(float *w, const float *V, uint64_t stride, uint32_t count, uint32_t n,
+void launch_multiaxpy(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                       const double *coef, bool assign, hipStream_t s);
+// w -= sum_i coef[i] V[i], then out[i] = <V[i], w> for i < count and out[count] = <w, w>
+void launch_axpy_dot(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
+                     const double *coef, double *partials, double *out, hipStream_t s);
 // out = a x + b y  (y may be nullptr)
 void launch_axpby(const float *x, const float *y, float *out, uint32_t n, float a, float b,
                   hipStream_t s);
@@ -251,7 +260,7 @@ void batch_split(nkpr_graph *g, uint32_t heavy);
 void ensure_buf(DevBuf &b, size_t bytes);
 bool binned_enabled(const nkpr_graph *g);
 bool binned_layout(nkpr_graph *g);
-void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
+void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute, bool minus,
                   hipEvent_t ev0, hipEvent_t ev1);
 void set_weights(nkpr_graph *g, const float *w, const double *w64, const double *out_wdeg,
                  int device_ptrs);

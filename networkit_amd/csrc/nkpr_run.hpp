@@ -147,7 +147,8 @@ struct SolveResult {
 struct Krylov {
     int m;
     uint32_t n;
-    float *V;       // basis [(m + 1) * n]
+    uint64_t ld;    // stride of the basis vectors: n rounded up to 64 floats
+    float *V;       // basis [(m + 1) * ld]
     float *y;       // d P v
     float *vh, *yh; // prescaled copies (unweighted graphs)
     float *e;       // solution of the last solve
@@ -159,20 +160,32 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
                        const float *base32, nkpr_stats *st);
 Krylov krylov_setup(nkpr_graph *g);
 void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
-                    double *ms_acc);
+                    double *ms_acc, bool minus);
 
-// GMRES(m) from e = 0 on A e = rhs until ||A e - rhs||_2 <= rtol ||rhs||_2 or `cap` operator
-// applications.  Classical Gram-Schmidt with selective reorthogonalisation; the Hessenberg
-// least-squares problem is solved on the host with Givens rotations.  Returns the number of
-// applications.
+// GMRES(m) from e = 0 on A e = rhs, A = I - B, until ||A e - rhs||_2 <= rtol ||rhs||_2 or `cap`
+// operator applications.  `apply(v, w)` computes w = B v.  The Krylov spaces of A and B are
+// the same, so the Arnoldi process runs on B, B V_j = V_{j+1} H_B, and the least-squares
+// problem uses A V_j = V_{j+1} (I~ - H_B).  On B the Gram-Schmidt pass cancels little: A v_j
+// is mostly v_j itself (B is a contraction), so Arnoldi on A would cancel most of every new
+// vector and need the second pass each step.  Classical Gram-Schmidt with selective
+// reorthogonalisation; the Hessenberg least-squares problem is solved on the host with Givens
+// rotations.  Returns the number of applications.
 template <typename Apply>
 uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, double rtol,
                    uint64_t cap, hipStream_t s) {
     const uint32_t n = k.n;
     const int mres = k.m;
+    const uint64_t ld = k.ld;
+    float *const V = k.V;
     auto dots = [&](const float *w, const float *Vb, uint32_t cnt, double *host) {
-        launch_multidot(w, Vb, n, cnt, n, k.dpart, k.dout, s);
+        launch_multidot(w, Vb, ld, cnt, n, k.dpart, k.dout, s);
         NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * cnt, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+    };
+    // w -= V c (cnt vectors), then host[i] = <V[i], w> for i < cnt and host[cnt] = <w, w>
+    auto project = [&](float *w, uint32_t cnt, const double *c, double *host) {
+        launch_axpy_dot(w, V, ld, cnt, n, c, k.dpart, k.dout, s);
+        NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * (cnt + 1), hipMemcpyDeviceToHost, s));
         NKPR_HIP(hipStreamSynchronize(s));
     };
     double beta2 = 0.0;
@@ -182,36 +195,36 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         launch_axpby(rhs, nullptr, e, n, 0.0f, 0.0f, s);
         return 0;
     }
-    float *V = k.V;
     launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
     std::vector<double> R((size_t)mres * mres, 0.0), cs(mres), sn(mres), gv(mres + 1, 0.0);
     gv[0] = beta;
     uint64_t applies = 0;
     int kk = 0;
     for (int j = 0; j < mres && applies < cap; ++j) {
-        float *vj = V + (size_t)j * n;
-        float *w = V + (size_t)(j + 1) * n;
+        float *vj = V + (size_t)j * ld;
+        float *w = V + (size_t)(j + 1) * ld;
         apply(vj, w);
         ++applies;
         // one classical Gram-Schmidt pass; w is stored as basis vector j+1, so the same
-        // multi-dot also returns w.w and the new norm follows from Pythagoras.  A second pass
-        // runs only when the pass cancelled more than half of w (selective
+        // multi-dot also returns w.w.  The projection kernel also returns the dots of the
+        // projected w with the basis and its squared norm, which a second pass needs: that
+        // pass runs only when the first cancelled more than half of w (selective
         // reorthogonalisation, the criterion of Daniel, Gragg, Kaufman and Stewart).
-        std::vector<double> h(j + 2), h2(j + 2);
+        std::vector<double> h(j + 2), h2(j + 2), h3(j + 2);
         dots(w, V, j + 2, h.data());
-        double ww = h[j + 1];
-        launch_multiaxpy(w, V, n, j + 1, n, h.data(), false, s);
-        double hn2 = ww;
-        for (int i = 0; i <= j; ++i) hn2 -= h[i] * h[i];
+        const double ww = h[j + 1];
+        project(w, j + 1, h.data(), h2.data());
+        double hn2 = h2[j + 1];
         if (!(hn2 > 0.5 * ww)) {
-            dots(w, V, j + 2, h2.data());
-            launch_multiaxpy(w, V, n, j + 1, n, h2.data(), false, s);
+            project(w, j + 1, h2.data(), h3.data());
             for (int i = 0; i <= j; ++i) h[i] += h2[i];
-            hn2 = h2[j + 1];
-            for (int i = 0; i <= j; ++i) hn2 -= h2[i] * h2[i];
+            hn2 = h3[j + 1];
         }
         const double hn = std::sqrt(std::max(hn2, 0.0));
-        h[j + 1] = hn;
+        // column j of the Hessenberg matrix of A: I~ - H_B
+        for (int i = 0; i <= j; ++i) h[i] = -h[i];
+        h[j] += 1.0;
+        h[j + 1] = -hn;
         for (int i = 0; i < j; ++i) {
             const double t = cs[i] * h[i] + sn[i] * h[i + 1];
             h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
@@ -234,7 +247,7 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         for (int q = i + 1; q < kk; ++q) t -= R[(size_t)i * mres + q] * y[q];
         y[i] = R[(size_t)i * mres + i] != 0.0 ? t / R[(size_t)i * mres + i] : 0.0;
     }
-    launch_multiaxpy(e, V, n, kk, n, y.data(), true, s);
+    launch_multiaxpy(e, V, ld, kk, n, y.data(), true, s);
     return applies;
 }
 
