@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -226,6 +227,9 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
             const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
             const uint64_t kc = gmres_f32(kr, apply_b, g->r32.as<float>(), kr.e, rtol, left, s);
+            if (std::getenv("NKPR_DEBUG"))
+                std::fprintf(stderr, "[nkpr] refinement %d: residual l1 %.3e, bound %.3e, rtol %.3e, %llu applications\n",
+                             round, c.l1, c.bound, rtol, (unsigned long long)kc);
             st->sweeps_fp32 += kc;
             st->sweeps_fp32_correction += kc;
             total += kc;

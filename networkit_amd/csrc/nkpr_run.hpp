@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <functional>
 #include <vector>
@@ -198,6 +199,10 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
     launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
     std::vector<double> R((size_t)mres * mres, 0.0), cs(mres), sn(mres), gv(mres + 1, 0.0);
     gv[0] = beta;
+    const bool debug = std::getenv("NKPR_DEBUG") != nullptr;
+    // second pass when |w'|^2 <= eta2 |w|^2 (NKPR_GMRES_REORTH overrides eta2)
+    double reorth_eta2 = 0.01;
+    if (const char *ev = std::getenv("NKPR_GMRES_REORTH")) reorth_eta2 = std::atof(ev);
     uint64_t applies = 0;
     int kk = 0;
     for (int j = 0; j < mres && applies < cap; ++j) {
@@ -208,14 +213,16 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         // one classical Gram-Schmidt pass; w is stored as basis vector j+1, so the same
         // multi-dot also returns w.w.  The projection kernel also returns the dots of the
         // projected w with the basis and its squared norm, which a second pass needs: that
-        // pass runs only when the first cancelled more than half of w (selective
-        // reorthogonalisation, the criterion of Daniel, Gragg, Kaufman and Stewart).
+        // pass runs only when the first cancelled all but a tenth of w's norm (selective
+        // reorthogonalisation, the criterion of Daniel, Gragg, Kaufman and Stewart with
+        // eta = 0.1; the FP64 residual sweeps check every correction anyway).
         std::vector<double> h(j + 2), h2(j + 2), h3(j + 2);
         dots(w, V, j + 2, h.data());
         const double ww = h[j + 1];
         project(w, j + 1, h.data(), h2.data());
         double hn2 = h2[j + 1];
-        if (!(hn2 > 0.5 * ww)) {
+        const bool reorth = !(hn2 > reorth_eta2 * ww);
+        if (reorth) {
             project(w, j + 1, h2.data(), h3.data());
             for (int i = 0; i <= j; ++i) h[i] += h2[i];
             hn2 = h3[j + 1];
@@ -239,6 +246,9 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         for (int i = 0; i <= j; ++i) R[(size_t)i * mres + j] = h[i];
         kk = j + 1;
         if (hn > 0.0) launch_axpby(w, nullptr, w, n, (float)(1.0 / hn), 0.0f, s);
+        if (debug)
+            std::fprintf(stderr, "[nkpr]   gmres %d: relative residual %.3e%s\n", j,
+                         std::fabs(gv[j + 1]) / beta, reorth ? " (reorthogonalised)" : "");
         if (std::fabs(gv[j + 1]) <= rtol * beta || !(hn > 0.0)) break;
     }
     std::vector<double> y(kk, 0.0);
