@@ -449,21 +449,74 @@ __global__ void col_range_sum_kernel(const T *x, uint32_t ld, uint32_t begin, ui
 }
 
 // X[u][j] = scores[j * z + perm[u]] (j < k), 0 in the padding columns.
-__global__ void batch_gather_kernel(const double *scores, uint64_t z, const int32_t *perm,
-                                    uint32_t n, uint32_t k, uint32_t ld, double *x64) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * ld; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = (uint32_t)(i / ld), j = (uint32_t)(i % ld);
-        x64[i] = j < k ? scores[(uint64_t)j * z + (uint32_t)perm[u]] : 0.0;
+// The caller's scores are column-major in slot order (scores[j * z + slot]); the iterates are
+// node-major in compact order (x[u * ld + j]).  Both conversions go through 64 x 64 tiles in
+// LDS, indexed by slot: every global access is a run of 64 consecutive doubles on either side
+// (slot_u: compact id of a slot, or -1 for a slot without a node).
+constexpr uint32_t kTile = 64;
+
+// x64[u * ld + j] = scores[j * z + slot(u)] (0 for the padding columns j >= k)
+__global__ __launch_bounds__(256) void batch_gather_kernel(const double *scores, uint64_t z,
+                                                           const int32_t *slot_u, uint32_t k,
+                                                           uint32_t ld, double *x64) {
+    __shared__ double tile[kTile][kTile + 1];
+    const uint32_t lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const uint64_t st = (z + kTile - 1) / kTile, ct = (ld + kTile - 1) / kTile;
+    for (uint64_t t = blockIdx.x; t < st * ct; t += gridDim.x) {
+        const uint64_t s0 = (t / ct) * kTile;
+        const uint32_t j0 = (uint32_t)(t % ct) * kTile;
+        const uint64_t slot = s0 + lane;
+        for (uint32_t jj = row; jj < kTile; jj += 4) {
+            const uint32_t j = j0 + jj;
+            tile[jj][lane] = (j < k && slot < z) ? scores[(uint64_t)j * z + slot] : 0.0;
+        }
+        __syncthreads();
+        for (uint32_t ss = row; ss < kTile; ss += 4) {
+            if (s0 + ss >= z) break;
+            const int32_t u = slot_u[s0 + ss];
+            const uint32_t j = j0 + lane;
+            if (u >= 0 && j < ld) x64[(uint64_t)u * ld + j] = tile[lane][ss];
+        }
+        __syncthreads();
     }
 }
 
-__global__ void batch_scatter_kernel(const double *x64, const int32_t *perm, uint32_t n,
-                                     uint32_t k, uint32_t ld, uint64_t z, const double *factor,
-                                     double *scores) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * k; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t j = (uint32_t)(i / n), u = (uint32_t)(i % n);
-        scores[(uint64_t)j * z + (uint32_t)perm[u]] = x64[(size_t)u * ld + j] / factor[j];
+// scores[j * z + slot(u)] = x64[u * ld + j] / factor[j] for j < k; slots without a node keep
+// their values
+__global__ __launch_bounds__(256) void batch_scatter_kernel(const double *x64, const int32_t *slot_u,
+                                                            uint32_t k, uint32_t ld, uint64_t z,
+                                                            const double *factor, double *scores) {
+    __shared__ double tile[kTile][kTile + 1];
+    __shared__ int32_t su[kTile];
+    const uint32_t lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const uint64_t st = (z + kTile - 1) / kTile, ct = (k + kTile - 1) / kTile;
+    for (uint64_t t = blockIdx.x; t < st * ct; t += gridDim.x) {
+        const uint64_t s0 = (t / ct) * kTile;
+        const uint32_t j0 = (uint32_t)(t % ct) * kTile;
+        if (threadIdx.x < kTile) su[threadIdx.x] = (s0 + threadIdx.x < z) ? slot_u[s0 + threadIdx.x] : -1;
+        __syncthreads();
+        for (uint32_t ss = row; ss < kTile; ss += 4) {
+            const int32_t u = su[ss];
+            const uint32_t j = j0 + lane;
+            tile[lane][ss] = (u >= 0 && j < k) ? x64[(uint64_t)u * ld + j] / factor[j] : 0.0;
+        }
+        __syncthreads();
+        for (uint32_t jj = row; jj < kTile; jj += 4) {
+            const uint32_t j = j0 + jj;
+            if (j < k && su[lane] >= 0) scores[(uint64_t)j * z + s0 + lane] = tile[jj][lane];
+        }
+        __syncthreads();
     }
+}
+
+// slot_u[slot] = compact id of the node in that slot, -1 for slots without one
+__global__ void slot_u_kernel(uint64_t z, int32_t *slot_u) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z; i += (uint64_t)gridDim.x * blockDim.x)
+        slot_u[i] = -1;
+}
+__global__ void slot_u_set_kernel(const int32_t *perm, uint32_t n, int32_t *slot_u) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+        slot_u[perm[u]] = (int32_t)u;
 }
 
 __global__ void f64_to_f32_kernel(const double *in, float *out, uint64_t cnt) {
@@ -1038,8 +1091,18 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
         sz = g->b_zbuf.as<double>();
         NKPR_HIP(hipMemcpyAsync(sz, scores, 8 * (size_t)z * k, hipMemcpyHostToDevice, s));
     }
-    hipLaunchKernelGGL(batch_gather_kernel, dim3(grid_for(cells)), dim3(256), 0, s,
-                       (const double *)sz, z, g->perm.as<int32_t>(), n, k, ld, x64[0]);
+    if (!g->b_slot_ready) {
+        ensure_buf(g->b_slot_u, 4 * std::max<uint64_t>(z, 1));
+        hipLaunchKernelGGL(slot_u_kernel, dim3(grid_for(z)), dim3(256), 0, s, z, g->b_slot_u.as<int32_t>());
+        hipLaunchKernelGGL(slot_u_set_kernel, dim3(grid_for(n)), dim3(256), 0, s, g->perm.as<int32_t>(), n,
+                           g->b_slot_u.as<int32_t>());
+        NKPR_HIP(hipGetLastError());
+        g->b_slot_ready = true;
+    }
+    const uint64_t gather_tiles = ((z + kTile - 1) / kTile) * ((ld + kTile - 1) / kTile);
+    hipLaunchKernelGGL(batch_gather_kernel, dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(gather_tiles, 1), 1u << 16)),
+                       dim3(256), 0, s, (const double *)sz, z, (const int32_t *)g->b_slot_u.as<int32_t>(), k, ld,
+                       x64[0]);
     NKPR_HIP(hipGetLastError());
 
     const uint64_t max_it = std::max<uint64_t>(p->max_iterations, 1);
@@ -1192,8 +1255,9 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
     }
     ensure_buf(g->b_factor, 8 * (size_t)k);
     NKPR_HIP(hipMemcpyAsync(g->b_factor.p, factor.data(), 8 * (size_t)k, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3(grid_for((uint64_t)n * k)), dim3(256), 0, s,
-                       (const double *)result, g->perm.as<int32_t>(), n, k, ld, z,
+    const uint64_t scatter_tiles = ((z + kTile - 1) / kTile) * ((k + kTile - 1) / kTile);
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(scatter_tiles, 1), 1u << 16)),
+                       dim3(256), 0, s, (const double *)result, (const int32_t *)g->b_slot_u.as<int32_t>(), k, ld, z,
                        g->b_factor.as<double>(), sz);
     NKPR_HIP(hipGetLastError());
     if (!p->scores_on_device)

@@ -334,6 +334,8 @@ struct nkpr_graph {
     nkpr::DevBuf b_tptr, b_tcol, b_tval32, b_tval64;
     nkpr::DevBuf b_piece32, b_piece64, b_partials, b_col, b_ctl, b_sink_part, b_factor;
     nkpr::DevBuf b_x64a, b_x64b, b_x32a, b_x32b, b_r32, b_zbuf;
+    nkpr::DevBuf b_slot_u;       // int32 [z]: compact id of a slot, -1 without a node
+    bool b_slot_ready = false;
     nkpr::BatchCtl *b_ctl_host = nullptr;
 
     // batch of graphs (nkpr_graph_create_batch): graph q owns compact ids [goff[q], goff[q+1]),
