@@ -6,7 +6,7 @@
 // line when it misses L2, which bounds the sweep by the gather request rate.  Here the product
 // is split in two streaming passes (propagation blocking: Beamer, Asanovic and Patterson,
 // "Reducing PageRank Communication via Propagation Blocking", IPDPS 2017) over a second copy
-// of the edges, cut into segments by (source chunk of 32768 ids, destination bin of 16384 ids)
+// of the edges, cut into segments by (source chunk of 32768 ids, destination bin of 8192 ids)
 // and sorted by source inside a segment:
 //   pass 1  per source chunk, segments in chunk-major order: v of the chunk is staged in LDS and
 //           every edge writes its contribution val = v[src] * w/deg(src), each segment's run to
@@ -36,10 +36,17 @@
 namespace nkpr {
 namespace {
 
-constexpr uint32_t kSpan = 32768;   // ids per source chunk (FP32 values staged in LDS)
-constexpr int kSpanShift = 15;
-constexpr uint32_t kBin = 16384;    // ids per destination bin (64-bit accumulators in LDS)
-constexpr int kBinShift = 14;
+#ifndef NKPR_SPAN_SHIFT
+#define NKPR_SPAN_SHIFT 15
+#endif
+#ifndef NKPR_BIN_SHIFT
+#define NKPR_BIN_SHIFT 13
+#endif
+constexpr int kSpanShift = NKPR_SPAN_SHIFT;
+constexpr uint32_t kSpan = 1u << kSpanShift;  // ids per source chunk (FP32 values staged in LDS)
+constexpr int kBinShift = NKPR_BIN_SHIFT;
+constexpr uint32_t kBin = 1u << kBinShift;    // ids per destination bin (64-bit accumulators in LDS)
+static_assert(kSpanShift <= 15 && kBinShift <= 16, "offsets are stored in 16 bits");
 constexpr int kPassThreads = 1024;
 constexpr uint64_t kPass1Edges = 1u << 19;  // edges per pass-1 work item (at most)
 constexpr uint64_t kPass2Edges = 1u << 20;  // edges per pass-2 work item (at most)
@@ -51,12 +58,15 @@ constexpr uint32_t kPadEdge = 0xFFFFFFFFu;
 // padded edge positions (and the quad loop counters) are 32-bit
 constexpr uint64_t kMaxPadded = (1ull << 32) - (1ull << 16);
 // Quads per lane and step of the two passes: the next step's loads are issued before this
-// step's quads are used, so up to 2 * k quads per lane are in flight.
+// step's quads are used, so up to 2 * k quads per lane are in flight.  Two pass-2 quads keep
+// the pass-2 kernel within 64 VGPRs, so that with 8192-row bins (64 KB of accumulators) two
+// blocks share a CU; measured on R-MAT scale 26 this beats 16384-row bins with deeper
+// prefetch.  A 16384-id chunk (two pass-1 blocks per CU) measured slower than 32768.
 #ifndef NKPR_P1_QUADS
 #define NKPR_P1_QUADS 1
 #endif
 #ifndef NKPR_P2_QUADS
-#define NKPR_P2_QUADS 8
+#define NKPR_P2_QUADS 2
 #endif
 constexpr int kP1Quads = NKPR_P1_QUADS, kP2Quads = NKPR_P2_QUADS;
 

@@ -324,21 +324,90 @@ def test_capi_rejects_bad_input():
 @pytest.mark.parametrize("item", ["0", "3000"])
 def test_mixed_binned_operator(monkeypatch, weighted, sinks, item):
     """The GMRES correction solves through the two-pass binned operator (forced on a graph
-    spanning several 32768-id chunks and bins; item=3000 splits bins over work items, whose
-    partial sums meet in global atomics): every result still certifies at tol and matches the
-    converged oracle."""
-    monkeypatch.setenv("NKPR_BINNED", "1")
+    spanning several source chunks and destination bins; item=3000 splits bins over work items,
+    whose accumulators are added in order by the reduce kernel): every result still certifies
+    at tol and matches the converged oracle, and the solve takes the same refinements and
+    (within one) the same operator applications as with the pull-sweep operator.  Both
+    operators are accurate to FP32; a wrong product (a lost segment, a wrong bin or sink term)
+    would slow the correction solves down."""
     monkeypatch.setenv("NKPR_BINNED_ITEM", item)
     z, u, v, w = skewed_graph(120000, 900000, 61, weighted=weighted)
     g = oracle_graph(z, u, v, w, True)
     ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=0, max_iterations=5000, sink_mode=sinks)
     w32 = None if g.in_w is None else g.in_w.astype(np.float32)
-    dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, weight64=g.in_w)
-    x = np.full(z, 1.0 / z)
-    st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
-                                       stop=capi.NKPR_STOP_L1_ERROR, sink_handling=sinks))
-    assert st["l1_error_bound"] <= 1e-9
-    assert np.abs(x - ref).sum() <= 1e-9
-    # the correction solves converge as with the pull operator (a wrong product would not)
-    assert st["sweeps_fp32"] <= 40 and st["refinements"] <= 4
+    stats = {}
+    for binned in ("0", "1"):
+        monkeypatch.setenv("NKPR_BINNED", binned)
+        dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, weight64=g.in_w)
+        x = np.full(z, 1.0 / z)
+        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
+                                           stop=capi.NKPR_STOP_L1_ERROR, sink_handling=sinks))
+        assert st["l1_error_bound"] <= 1e-9
+        assert np.abs(x - ref).sum() <= 1e-9
+        stats[binned] = st
+    assert stats["1"]["refinements"] == stats["0"]["refinements"] <= 4
+    assert abs(int(stats["1"]["sweeps_fp32"]) - int(stats["0"]["sweeps_fp32"])) <= 1
 
+
+
+def test_mixed_default_operator_on_large_graph(monkeypatch):
+    """R-MAT scale 23 (about 1.3e8 edges, above the 2^26 edges where the GMRES corrections take
+    the binned operator by default): the mixed solve certifies at 1e-9 and agrees with a
+    certified FP64 power-method solve (the reference arithmetic, pinned to the oracle on the
+    smaller graphs above) to the sum of the two bounds."""
+    import torch
+    monkeypatch.delenv("NKPR_BINNED", raising=False)
+    monkeypatch.delenv("NKPR_BINNED_ITEM", raising=False)
+    g = gen.rmat(23, 16, seed=4, device="cuda")
+    assert g["src"].numel() >= 1 << 26
+    dg = capi.DeviceGraph(g["z"], g["colptr"], g["src"], g["weight"], device=True)
+    xs = {}
+    for prec, tol in ((capi.NKPR_FP64, 1e-12), (capi.NKPR_MIXED, 1e-9)):
+        x = torch.full((g["z"],), 1.0 / g["z"], dtype=torch.float64, device="cuda")
+        st = dg.run(x, capi.default_params(scores_on_device=1, precision=prec, tol=tol,
+                                           stop=capi.NKPR_STOP_L1_ERROR))
+        torch.cuda.synchronize()
+        assert st["l1_error_bound"] <= tol
+        xs[prec] = x
+    assert float((xs[capi.NKPR_MIXED] - xs[capi.NKPR_FP64]).abs().sum()) <= 1e-9 + 1e-12
+    del g, dg
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("case", ["edgeless", "single_loop", "star_sinks", "deleted", "one_bin"])
+def test_mixed_binned_operator_edge_cases(monkeypatch, case):
+    """The binned operator forced on degenerate layouts: no edges (no pass-1 items, every bin
+    empty), one node with a self-loop, a star whose leaves are all sinks (one source chunk),
+    deleted slots, and a graph inside one bin.  Each mixed solve certifies and matches the
+    converged oracle (distributed sinks, so the sink term of the operator is exercised)."""
+    monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED_ITEM", "8")
+    exists = None
+    if case == "edgeless":
+        z, u, v, w = 7, np.zeros(0, np.int64), np.zeros(0, np.int64), None
+    elif case == "single_loop":
+        z, u, v, w = 1, np.array([0]), np.array([0]), None
+    elif case == "star_sinks":
+        z = 40000
+        u = np.zeros(z - 1, np.int64)
+        v = np.arange(1, z, dtype=np.int64)
+        w = np.random.default_rng(3).uniform(0.5, 1.5, z - 1)
+    elif case == "deleted":
+        z, u, v, w = random_graph(3000, 12000, 63, weighted=True)
+        exists = np.ones(z, np.uint8)
+        exists[[5, 77, 2999]] = 0
+        keep = (exists[u] == 1) & (exists[v] == 1)
+        u, v, w = u[keep], v[keep], w[keep]
+    else:
+        z, u, v, w = random_graph(5000, 40000, 64, weighted=False)
+    g = oracle.in_lists(z, u, v, w, True, exists=exists)
+    ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=0, max_iterations=20000, sink_mode=1)
+    w32 = None if g.in_w is None else g.in_w.astype(np.float32)
+    dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, exists=exists,
+                          weight64=g.in_w)
+    x = np.full(z, 1.0 / max(int(g.exists.sum()) if g.exists is not None else z, 1))
+    st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-10,
+                                       stop=capi.NKPR_STOP_L1_ERROR, sink_handling=1))
+    assert st["l1_error_bound"] <= 1e-10
+    ex = np.ones(z, bool) if exists is None else exists.astype(bool)
+    assert np.abs(x[ex] - ref[ex]).sum() <= 1e-10
