@@ -1,26 +1,29 @@
-// Binned application of the FP32 operator y = d P v for the GMRES correction solves of the
-// mixed-precision path (the product of networkit/cpp/centrality/PageRank.cpp:56-66 without
-// the teleport term).
+// Binned applications of the PageRank product P x: the FP32 operator y = d P v of the GMRES
+// correction solves of the mixed-precision path, and the FP64 sweep x' = d P x + b with the
+// reference's epilogue (the product of networkit/cpp/centrality/PageRank.cpp:56-66).
 //
-// The pull sweep gathers v[src] once per in-edge: a random 4-byte access that costs a 128-byte
-// line when it misses L2, which bounds the sweep by the gather request rate.  Here the product
-// is split in two streaming passes (propagation blocking: Beamer, Asanovic and Patterson,
+// The pull sweep gathers x[src] once per in-edge: a random access that costs a 128-byte line
+// when it misses L2, which bounds the sweep by the gather request rate.  Here the product is
+// split in two streaming passes (propagation blocking: Beamer, Asanovic and Patterson,
 // "Reducing PageRank Communication via Propagation Blocking", IPDPS 2017) over a second copy
-// of the edges, cut into segments by (source chunk of 32768 ids, destination bin of 8192 ids)
-// and sorted by source inside a segment:
-//   pass 1  per source chunk, segments in chunk-major order: v of the chunk is staged in LDS and
-//           every edge writes its contribution val = v[src] * w/deg(src), each segment's run to
+// of the edges, cut into segments by (source chunk, destination bin of 8192 ids) and sorted by
+// source inside a segment:
+//   pass 1  per source chunk, segments in chunk-major order: x of the chunk is staged in LDS
+//           and every edge writes its contribution x[src] * w/deg(src), each segment's run to
 //           the segment's place in bin-major order;
-//   pass 2  per destination bin, segments in bin-major order, so one contiguous stream per bin:
-//           the contributions are added into an LDS accumulator of 64-bit fixed-point
-//           integers, which is written out as w = v - (d y + st) (a bin split over several work
-//           items: the items' accumulators go to slots of their own, added in order).
-// Every gather is an LDS access and every HBM access streams: about 17 bytes per edge (pass 1
-// reads the source offset 2, the coefficient 4 and the bin-major place 1 and writes the
-// contribution 4; pass 2 reads the contribution 4 and the destination offset 2) instead of one
-// random line per gather.  Integer accumulation makes the result independent of the order of
-// the atomics (the operator is deterministic) and is finer than FP32: the scale puts the
-// largest possible row sum just below 2^61 units.
+//   pass 2  per destination bin, segments in bin-major order, so one contiguous stream per
+//           bin: the contributions are added into LDS accumulators of fixed-point integers,
+//           from which the rows are finished (a bin split over several work items: the items'
+//           accumulators go to slots of their own, added in order by the reduce kernel).
+// Every gather is an LDS access and every HBM access streams.  Integer accumulation makes the
+// result independent of the order of the atomics (deterministic), and exact: each row is the
+// correctly scaled sum of its rounded products, rounded once.
+//   FP32 copy (chunks of 32768 ids, FP32 table 128 KB): 64-bit accumulators, the largest
+//       possible row sum just below 2^61 units; about 17 bytes per edge moved.
+//   FP64 copy (chunks of 16384 ids, FP64 table 128 KB): 128-bit accumulators as two 64-bit
+//       integers (whole units and 2^-lo_shift fractions of a unit), so the rows carry more
+//       than FP64 precision; about 29 bytes per edge moved.  The row epilogue, the reductions
+//       and the stopping rule are those of the pull sweep (nkpr_sweep.hip).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -42,22 +45,22 @@ namespace {
 #ifndef NKPR_BIN_SHIFT
 #define NKPR_BIN_SHIFT 13
 #endif
-constexpr int kSpanShift = NKPR_SPAN_SHIFT;
-constexpr uint32_t kSpan = 1u << kSpanShift;  // ids per source chunk (FP32 values staged in LDS)
+constexpr int kSpan32Shift = NKPR_SPAN_SHIFT;  // ids per source chunk of the FP32 copy
+constexpr int kSpan64Shift = 14;               // ids per source chunk of the FP64 copy
 constexpr int kBinShift = NKPR_BIN_SHIFT;
-constexpr uint32_t kBin = 1u << kBinShift;    // ids per destination bin (64-bit accumulators in LDS)
-static_assert(kSpanShift <= 15 && kBinShift <= 16, "offsets are stored in 16 bits");
+constexpr uint32_t kBin = 1u << kBinShift;     // ids per destination bin
+static_assert(kSpan32Shift <= 15 && kBinShift <= 16, "offsets are stored in 16 bits");
 constexpr int kPassThreads = 1024;
 constexpr uint64_t kPass1Edges = 1u << 19;  // edges per pass-1 work item (at most)
 constexpr uint64_t kPass2Edges = 1u << 20;  // edges per pass-2 work item (at most)
-constexpr uint32_t kItemSole = 1;           // the bin's only pass-2 item: writes w directly
-// Every segment is padded to a multiple of kQuad edges (padding slots contribute 0), so both
-// passes read and write whole quads with 4-, 8- and 16-byte accesses.
+constexpr uint32_t kItemSole = 1;           // the bin's only pass-2 item: finishes the rows
+// Every segment is padded to a multiple of kQuad edges (padding slots read the zero entry
+// tab[span] of pass 1, so they contribute 0): both passes read and write whole quads.
 constexpr uint32_t kQuad = 4;
 constexpr uint32_t kPadEdge = 0xFFFFFFFFu;
 // padded edge positions (and the quad loop counters) are 32-bit
 constexpr uint64_t kMaxPadded = (1ull << 32) - (1ull << 16);
-// Quads per lane and step of the two passes: the next step's loads are issued before this
+// Quads per lane and step of the two FP32 passes: the next step's loads are issued before this
 // step's quads are used, so up to 2 * k quads per lane are in flight.  Two pass-2 quads keep
 // the pass-2 kernel within 64 VGPRs, so that with 8192-row bins (64 KB of accumulators) two
 // blocks share a CU; measured on R-MAT scale 26 this beats 16384-row bins with deeper
@@ -70,17 +73,23 @@ constexpr uint64_t kMaxPadded = (1ull << 32) - (1ull << 16);
 #endif
 constexpr int kP1Quads = NKPR_P1_QUADS, kP2Quads = NKPR_P2_QUADS;
 
-// Pass 2 adds the contributions as 64-bit fixed-point integers: integer addition is exact and
-// associative (the result does not depend on the order of the atomics) and LDS integer
-// atomics run at the rate of plain LDS stores.  The scale is 2^e with every partial sum below
-// 2^61 units: |sum| <= max|v| * max_u sum_in coef(u) = bound.
-__device__ __forceinline__ int fixed_exponent(const uint32_t *vmax_bits, double insum) {
-    const double bound = (double)__uint_as_float(*vmax_bits) * insum;
+// Control words of one application: the largest |x| of the sources (the bits of a
+// non-negative double order like unsigned integers) and the queue counters of the passes.
+struct BinCtl {
+    unsigned long long vmax;
+    uint32_t q1, q2;
+};
+
+// The fixed-point scale is 2^e with every row sum below 2^61 units:
+// |sum| <= max|x| * max_u sum_in coef(u) = bound.
+__device__ __forceinline__ int fixed_exponent(const BinCtl *ctl, double insum) {
+    const double bound = __longlong_as_double((long long)ctl->vmax) * insum;
     return bound > 0.0 ? 60 - ilogb(bound) : 0;
 }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // four contributions of pass 2: destination offsets (u16 pairs) and values
 struct Quad {
@@ -110,6 +119,8 @@ inline unsigned nb(uint64_t n, int t = 256) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + t - 1) / t, 65536));
 }
 
+// ---- layout construction ------------------------------------------------------------------
+
 __global__ void bin_row_kernel(const uint32_t *rowptr, uint32_t n, uint32_t *row) {
     const int lane = threadIdx.x & 63;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -117,25 +128,25 @@ __global__ void bin_row_kernel(const uint32_t *rowptr, uint32_t n, uint32_t *row
         for (uint32_t e = rowptr[w] + lane; e < rowptr[w + 1]; e += 64) row[e] = (uint32_t)w;
 }
 
-// key = ((chunk * nbins + bin) << 15) | source offset: inside a segment the edges are sorted
+// key = ((chunk * nbins + bin) << ss) | source offset: inside a segment the edges are sorted
 // by source, which spreads the destinations of a wave's atomics over the accumulator
-// (chunks of kSpan sources, bins of kBin destinations)
+// (chunks of 2^ss sources, bins of kBin destinations)
 __global__ void bin_key_kernel(const int32_t *src, const uint32_t *row, uint64_t m, uint32_t nbins,
-                               uint64_t *key, uint32_t *val) {
+                               int ss, uint64_t *key, uint32_t *val) {
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t v = (uint32_t)src[e];
-        const uint64_t seg = (uint64_t)(v >> kSpanShift) * nbins + (row[e] >> kBinShift);
-        key[e] = (seg << kSpanShift) | (v & (kSpan - 1));
+        const uint64_t seg = (uint64_t)(v >> ss) * nbins + (row[e] >> kBinShift);
+        key[e] = (seg << ss) | (v & ((1u << ss) - 1));
         val[e] = (uint32_t)e;
     }
 }
 
 // seg[s] = first sorted edge of segment s (binary search over the sorted keys)
-__global__ void bin_seg_kernel(const uint64_t *key, uint64_t m, uint64_t nseg, uint32_t *seg) {
+__global__ void bin_seg_kernel(const uint64_t *key, uint64_t m, uint64_t nseg, int ss, uint32_t *seg) {
     for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx <= nseg;
          sidx += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t x = sidx << kSpanShift;
+        const uint64_t x = sidx << ss;
         uint64_t lo = 0, hi = m;
         while (lo < hi) {
             const uint64_t mid = lo + (hi - lo) / 2;
@@ -148,11 +159,11 @@ __global__ void bin_seg_kernel(const uint64_t *key, uint64_t m, uint64_t nseg, u
 
 // order[p] = edge at chunk-major padded position p: the sorted edge j of segment s goes to
 // cseg[s] + (j - rseg[s]) (rseg: unpadded segment starts, cseg: padded ones)
-__global__ void bin_place_kernel(const uint64_t *key, const uint32_t *ids, uint64_t m,
+__global__ void bin_place_kernel(const uint64_t *key, const uint32_t *ids, uint64_t m, int ss,
                                  const uint32_t *rseg, const uint32_t *cseg, uint32_t *order) {
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
          j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t sidx = key[j] >> kSpanShift;
+        const uint64_t sidx = key[j] >> ss;
         order[cseg[sidx] + ((uint32_t)j - rseg[sidx])] = ids[j];
     }
 }
@@ -170,52 +181,91 @@ __global__ void bin_pos_kernel(const uint32_t *cseg, const uint32_t *bseg, uint6
 }
 
 // Source offsets and coefficients in chunk-major order, destination offsets in bin-major
-// order.  A padding slot reads the zero entry tab[kSpan] of pass 1, so it contributes 0.
-__global__ void bin_emit_kernel(const uint32_t *order, const uint32_t *pos, uint64_t mp,
-                                const int32_t *src, const uint32_t *row, const float *coef,
-                                uint16_t *bsrc, uint16_t *bdst, float *bcoef) {
+// order.  A padding slot gets the source offset 2^ss (the zero table entry).
+template <typename C>
+__global__ void bin_emit_kernel(const uint32_t *order, const uint32_t *pos, uint64_t mp, int ss,
+                                const int32_t *src, const uint32_t *row, const C *coef,
+                                uint16_t *bsrc, uint16_t *bdst, C *bcoef) {
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < mp;
          j += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t e = order[j];
         const uint32_t jb = pos[j / kQuad] + (uint32_t)(j % kQuad);
         if (e == kPadEdge) {
-            bsrc[j] = (uint16_t)kSpan;
+            bsrc[j] = (uint16_t)(1u << ss);
             bdst[jb] = 0;
-            if (bcoef) bcoef[j] = 0.f;
+            if (bcoef) bcoef[j] = (C)0;
             continue;
         }
-        bsrc[j] = (uint16_t)((uint32_t)src[e] & (kSpan - 1));
+        bsrc[j] = (uint16_t)((uint32_t)src[e] & ((1u << ss) - 1));
         bdst[jb] = (uint16_t)(row[e] & (kBin - 1));
         if (bcoef) bcoef[j] = coef[e];
     }
 }
 
-// The passes run one persistent block per LDS-sized slot of the device; a block takes the next
-// work item from a queue counter until the list is done (the host sorts the items by
-// decreasing size, so the largest start first and the short ones fill in at the end).  Every
-// item body ends with a barrier, after which thread 0 may overwrite the shared slot.
+// largest in-coefficient sum of a row: sum_in w/deg(src) (weighted) or sum_in 1/deg(src); one
+// wave per row, the block's largest sum goes to `best` with one atomic
+template <typename C>
+__global__ void bin_insum_kernel(const uint32_t *rowptr, uint32_t n, const int32_t *src,
+                                 const C *coef, const C *inv, unsigned long long *best) {
+    __shared__ double wbest[4];
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    double mine = 0.0;
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
+        double sum = 0.0;
+        for (uint32_t e = rowptr[w] + lane; e < rowptr[w + 1]; e += 64)
+            sum += coef ? (double)coef[e] : (double)inv[src[e]];
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        mine = fmax(mine, sum);
+    }
+    if (lane == 0) wbest[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wbest[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) b = fmax(b, wbest[q]);
+        atomicMax(best, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+__global__ void bin_clear_kernel(BinCtl *ctl) {
+    if (threadIdx.x == 0) {
+        ctl->vmax = 0ull;
+        ctl->q1 = 0u;
+        ctl->q2 = 0u;
+    }
+}
+
+// The passes run persistent blocks, as many as fit on the device; a block takes the next work
+// item from a queue counter until the list is done (the host sorts the items by decreasing
+// size, so the largest start first and the short ones fill in at the end).  Every item body
+// ends with a barrier, after which thread 0 may overwrite the shared slot.
 __device__ __forceinline__ uint32_t next_item(uint32_t *queue, uint32_t *slot) {
     if (threadIdx.x == 0) *slot = atomicAdd(queue, 1u);
     __syncthreads();
     return *slot;
 }
 
+__device__ __forceinline__ void note_vmax(BinCtl *ctl, double vmax) {
+    for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&ctl->vmax, (unsigned long long)__double_as_longlong(vmax));
+}
+
+// ---- FP32 operator ------------------------------------------------------------------------
+
 // Pass 1: item = {chunk, first edge, end edge} (chunk-major, quad-aligned).  The contribution
 // of an edge is v[src] * coef (weighted) or v[src] / deg(src) (unweighted: inv), stored at the
-// edge's bin-major place.  Also finds the largest |v| of the sources (the scale of pass 2;
-// sinks contribute nothing), as the bits of a non-negative float, which order like unsigned
-// integers.
+// edge's bin-major place.  Also finds the largest |v| of the sources (sinks contribute nothing).
 __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
     const uint4 *items, uint32_t nitems, const float *v, const float *inv, uint32_t n,
-    const uint16_t *bsrc, const float *bcoef, const uint32_t *pos, float *bval,
-    uint32_t *vmax_bits, uint32_t *queue) {
+    const uint16_t *bsrc, const float *bcoef, const uint32_t *pos, float *bval, BinCtl *ctl) {
+    constexpr uint32_t kSpan = 1u << kSpan32Shift;
     __shared__ float tab[kSpan + 1];
     __shared__ uint32_t slot;
     if (threadIdx.x == 0) tab[kSpan] = 0.f;  // read by the padding slots
     float vmax = 0.f;
-    for (uint32_t it = next_item(queue, &slot); it < nitems; it = next_item(queue, &slot)) {
+    for (uint32_t it = next_item(&ctl->q1, &slot); it < nitems; it = next_item(&ctl->q1, &slot)) {
         const uint4 item = items[it];
-        const uint32_t base = item.x << kSpanShift;
+        const uint32_t base = item.x << kSpan32Shift;
         for (uint32_t i = threadIdx.x; i < kSpan; i += kPassThreads) {
             const uint32_t u = base + i;
             float x = 0.f;
@@ -276,8 +326,7 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
         }
         __syncthreads();
     }
-    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(vmax));
+    note_vmax(ctl, (double)vmax);
 }
 
 // Adds the quads f0, f0 + 64, f0 + 128, ... below `end` (quads from the start of bval) into
@@ -321,22 +370,21 @@ __device__ __forceinline__ void stream_quads(unsigned long long *acc, const uint
 // Pass 2: item = {bin, first edge, end edge, flags | slot << 2} (bin-major, quad-aligned).  Each
 // wave streams an equal, contiguous share of the item's quads (64 consecutive quads per load,
 // one per lane) into the bin's LDS accumulator.  A bin held by one item is written out as
-// w = v - (d y + st) at once; a split bin's items write their accumulators to slots of their
-// own.  Without `minus` the output is B v = d y + st instead (the operator whose Krylov space
-// GMRES builds).
+// w = v - (d y + st) (minus) or w = B v = d y + st at once; a split bin's items write their
+// accumulators to slots of their own.
 __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
     const uint4 *items, uint32_t nitems, const uint16_t *bdst, const float *bval, long long *part,
-    uint32_t n, const uint32_t *vmax_bits, double insum, const float *v, bool minus, float d,
-    const double *sink_sum, float *w, uint32_t *queue) {
+    uint32_t n, BinCtl *ctl, double insum, const float *v, bool minus, float d,
+    const double *sink_sum, float *w) {
     __shared__ unsigned long long acc[kBin];
     __shared__ uint32_t slot;
-    const int ex = fixed_exponent(vmax_bits, insum);
+    const int ex = fixed_exponent(ctl, insum);
     const float scale = ldexpf(1.f, ex);
     const double unscale = ldexp(1.0, -ex);
     const float st = sink_sum ? (float)(d * *sink_sum / (double)n) : 0.f;
     constexpr uint32_t kWaves = kPassThreads / 64;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t it = next_item(queue, &slot); it < nitems; it = next_item(queue, &slot)) {
+    for (uint32_t it = next_item(&ctl->q2, &slot); it < nitems; it = next_item(&ctl->q2, &slot)) {
         const uint4 item = items[it];
         for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) acc[i] = 0ull;
         __syncthreads();
@@ -361,11 +409,11 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
 }
 
 // Bins split over several items: y = the items' partial accumulators added in item order, then
-// w = v - (d y + st) (minus) or w = d y + st.  mb[j] = {bin, first slot, slot count}.
+// w = v - (d y + st) (minus) or w = d y + st.  mb[j] = {bin, first slot, slot count, -}.
 __global__ void bin_reduce_kernel(const uint4 *mb, uint32_t nmb, const long long *part, uint32_t n,
-                                  const uint32_t *vmax_bits, double insum, const float *v, bool minus,
+                                  const BinCtl *ctl, double insum, const float *v, bool minus,
                                   float d, const double *sink_sum, float *w) {
-    const double unscale = ldexp(1.0, -fixed_exponent(vmax_bits, insum));
+    const double unscale = ldexp(1.0, -fixed_exponent(ctl, insum));
     const float st = sink_sum ? (float)(d * *sink_sum / (double)n) : 0.f;
     const uint64_t total = (uint64_t)nmb * kBin;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
@@ -381,47 +429,240 @@ __global__ void bin_reduce_kernel(const uint4 *mb, uint32_t nmb, const long long
     }
 }
 
-// ctl[0] = largest |v| (bits), ctl[1], ctl[2] = queue counters of the two passes
-__global__ void bin_clear_kernel(uint32_t *ctl) {
-    if (threadIdx.x < 3) ctl[threadIdx.x] = 0u;
+// ---- FP64 sweep ---------------------------------------------------------------------------
+
+// What the row epilogue of a sweep needs (the fields of SweepArgs<double> it reads).
+struct Epi64 {
+    const double *xold;
+    double *xnew;
+    double *xhat_new;        // prescaled new iterate (unweighted) or nullptr
+    const double *inv;       // 1/deg (unweighted) or nullptr
+    const double *base_vec;  // per-node right-hand side or nullptr
+    float *r32;              // FP32 residual or nullptr
+    double *partials;        // [slots * 4] (l1, l2, sum, sink)
+    const PhaseCtl *pctl;
+    double damp, tele;
+    uint32_t n, sink_begin;
+    int32_t distribute;
+    int lo_shift;            // fraction bits of the low accumulator word
+};
+
+// The finished row u from its accumulated product y (the pull sweep's epilogue_store).
+__device__ __forceinline__ void finish_row64(const Epi64 &e, uint32_t u, double y, double c,
+                                             double (&s)[4]) {
+    double x = e.damp * y + c;
+    if (e.base_vec) x += e.base_vec[u];
+    __builtin_nontemporal_store(x, e.xnew + u);
+    if (e.xhat_new != nullptr && u < e.sink_begin) __builtin_nontemporal_store(x * e.inv[u], e.xhat_new + u);
+    const double dif = x - e.xold[u];
+    if (e.r32) __builtin_nontemporal_store((float)dif, e.r32 + u);
+    s[0] += fabs(dif);
+    s[1] += dif * dif;
+    s[2] += x;
+    if (u >= e.sink_begin) s[3] += x;
 }
 
-// largest in-coefficient sum of a row: sum_in w/deg(src) (weighted) or sum_in 1/deg(src); one
-// wave per row, the block's largest sum goes to `best` with one atomic
-__global__ void bin_insum_kernel(const uint32_t *rowptr, uint32_t n, const int32_t *src,
-                                 const float *coef, const float *inv, unsigned long long *best) {
-    __shared__ double wbest[4];
-    const int lane = threadIdx.x & 63;
-    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    double mine = 0.0;
-    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
-        double sum = 0.0;
-        for (uint32_t e = rowptr[w] + lane; e < rowptr[w + 1]; e += 64)
-            sum += coef ? (double)coef[e] : (double)inv[src[e]];
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-        mine = fmax(mine, sum);
+// Block sums of the four reductions into partials[slot * 4 + k] (fixed order: deterministic).
+template <int kThreads>
+__device__ __forceinline__ void block_partials(double (&s)[4], double *partials, uint32_t slot) {
+    __shared__ double red[4][kThreads / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double v = s[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[k][wave] = v;
     }
-    if (lane == 0) wbest[threadIdx.x >> 6] = mine;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = wbest[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) b = fmax(b, wbest[q]);
-        atomicMax(best, (unsigned long long)__double_as_longlong(b));
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int q = 0; q < kThreads / 64; ++q) v += red[threadIdx.x][q];
+        partials[(size_t)slot * 4 + threadIdx.x] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double rhs_const64(const Epi64 &e) {
+    double c = e.tele;
+    if (e.distribute) c += e.damp * e.pctl->sink_in / (double)e.n;
+    return c;
+}
+
+// Pass 1 (FP64): as the FP32 pass with an FP64 table of 16384 sources and FP64 contributions.
+__global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
+    const uint4 *items, uint32_t nitems, const double *x, const double *inv, uint32_t n,
+    const uint16_t *bsrc, const double *bcoef, const uint32_t *pos, double *bval, BinCtl *ctl,
+    const PhaseCtl *pctl) {
+    constexpr uint32_t kSpan = 1u << kSpan64Shift;
+    if (pctl->done) return;
+    __shared__ double tab[kSpan + 1];
+    __shared__ uint32_t slot;
+    if (threadIdx.x == 0) tab[kSpan] = 0.0;  // read by the padding slots
+    double vmax = 0.0;
+    for (uint32_t it = next_item(&ctl->q1, &slot); it < nitems; it = next_item(&ctl->q1, &slot)) {
+        const uint4 item = items[it];
+        const uint32_t base = item.x << kSpan64Shift;
+        for (uint32_t i = threadIdx.x; i < kSpan; i += kPassThreads) {
+            const uint32_t u = base + i;
+            double t = 0.0;
+            if (u < n) {
+                const double xu = x[u];
+                vmax = fmax(vmax, fabs(xu));
+                t = inv ? xu * inv[u] : xu;
+            }
+            tab[i] = t;
+        }
+        __syncthreads();
+        // one quad per thread and step, the next step's loads issued before this step's stores
+        const uint32_t q1 = item.z / kQuad;
+        const f64x2 one = {1.0, 1.0};
+        uint32_t q = item.y / kQuad + threadIdx.x;
+        u32x2 sp = {kSpan | (kSpan << 16), kSpan | (kSpan << 16)};
+        uint32_t at = 0;
+        f64x2 c0 = one, c1 = one;
+        if (q < q1) {
+            sp = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + q * kQuad));
+            at = __builtin_nontemporal_load(pos + q);
+            if (bcoef) {
+                c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + q * kQuad));
+                c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + q * kQuad + 2));
+            }
+        }
+        for (; q < q1; q += kPassThreads) {
+            const uint32_t qn = q + kPassThreads;
+            u32x2 spn = sp;
+            uint32_t atn = 0;
+            f64x2 c0n = one, c1n = one;
+            if (qn < q1) {
+                spn = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + qn * kQuad));
+                atn = __builtin_nontemporal_load(pos + qn);
+                if (bcoef) {
+                    c0n = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + qn * kQuad));
+                    c1n = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + qn * kQuad + 2));
+                }
+            }
+            f64x2 r0 = {tab[sp.x & 0xFFFFu], tab[sp.x >> 16]};
+            f64x2 r1 = {tab[sp.y & 0xFFFFu], tab[sp.y >> 16]};
+            r0 *= c0;
+            r1 *= c1;
+            __builtin_nontemporal_store(r0, reinterpret_cast<f64x2 *>(bval + at));
+            __builtin_nontemporal_store(r1, reinterpret_cast<f64x2 *>(bval + at + 2));
+            sp = spn;
+            at = atn;
+            c0 = c0n;
+            c1 = c1n;
+        }
+        __syncthreads();
+    }
+    note_vmax(ctl, vmax);
+}
+
+// One contribution t (already scaled by 2^e) into the two accumulator words: the whole units
+// hi = trunc(t) and the fraction t - hi (exact) in units of 2^-lo_shift.
+__device__ __forceinline__ void fx_add64(unsigned long long *hi, unsigned long long *lo, uint32_t d,
+                                         double t, double lscale) {
+    const double h = trunc(t);
+    atomicAdd(&hi[d], (unsigned long long)(long long)h);
+    atomicAdd(&lo[d], (unsigned long long)__double2ll_rn((t - h) * lscale));
+}
+
+// Pass 2 (FP64): the bin's contributions into 128-bit accumulators; a bin held by one item is
+// finished here (the sweep epilogue, the block's reductions into partials slot w >> 2), a split
+// bin's items write both words to part slots of their own.
+__global__ __launch_bounds__(kPassThreads) void bin_gather64_kernel(
+    const uint4 *items, uint32_t nitems, const uint16_t *bdst, const double *bval,
+    long long *part, BinCtl *ctl, double insum, Epi64 e) {
+    if (e.pctl->done) return;
+    __shared__ unsigned long long acc_hi[kBin];
+    __shared__ unsigned long long acc_lo[kBin];
+    __shared__ uint32_t slot;
+    const int ex = fixed_exponent(ctl, insum);
+    const double scale = ldexp(1.0, ex), unscale = ldexp(1.0, -ex);
+    const double lscale = ldexp(1.0, e.lo_shift), lunscale = ldexp(1.0, -e.lo_shift);
+    const double c = rhs_const64(e);
+    const uint32_t n = e.n;
+    for (uint32_t it = next_item(&ctl->q2, &slot); it < nitems; it = next_item(&ctl->q2, &slot)) {
+        const uint4 item = items[it];
+        for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) acc_hi[i] = acc_lo[i] = 0ull;
+        __syncthreads();
+        for (uint32_t q = item.y / kQuad + threadIdx.x; q < item.z / kQuad; q += kPassThreads) {
+            const u32x2 dd = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bdst + q * kQuad));
+            const f64x2 a0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bval + q * kQuad));
+            const f64x2 a1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bval + q * kQuad + 2));
+            fx_add64(acc_hi, acc_lo, dd.x & 0xFFFFu, a0.x * scale, lscale);
+            fx_add64(acc_hi, acc_lo, dd.x >> 16, a0.y * scale, lscale);
+            fx_add64(acc_hi, acc_lo, dd.y & 0xFFFFu, a1.x * scale, lscale);
+            fx_add64(acc_hi, acc_lo, dd.y >> 16, a1.y * scale, lscale);
+        }
+        __syncthreads();
+        const uint32_t base = item.x << kBinShift;
+        if (item.w & kItemSole) {
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
+            for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) {
+                const uint32_t u = base + i;
+                if (u >= n) break;
+                const double y = ((double)(long long)acc_hi[i] + (double)(long long)acc_lo[i] * lunscale) * unscale;
+                finish_row64(e, u, y, c, s);
+            }
+            block_partials<kPassThreads>(s, e.partials, item.w >> 2);
+        } else {
+            for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) {
+                if (base + i >= n) break;
+                long long *p = part + ((size_t)(item.w >> 2) * kBin + i) * 2;
+                p[0] = (long long)acc_hi[i];
+                p[1] = (long long)acc_lo[i];
+            }
+        }
+        __syncthreads();
     }
 }
+
+// Split bins (FP64): kReduce64Blocks blocks per bin, one row per thread, add the items' words in
+// item order and finish the rows; block j of a bin writes partials slot mb.w + j.
+// mb[i] = {bin, first slot, slot count, first partials slot}.
+constexpr int kReduce64Threads = 1024;
+constexpr uint32_t kReduce64Blocks = kBin / kReduce64Threads;
+__global__ __launch_bounds__(kReduce64Threads) void bin_reduce64_kernel(
+    const uint4 *mb, const long long *part, const BinCtl *ctl, double insum, Epi64 e) {
+    if (e.pctl->done) return;
+    const uint4 b = mb[blockIdx.x / kReduce64Blocks];
+    const uint32_t j = blockIdx.x % kReduce64Blocks;
+    const double unscale = ldexp(1.0, -fixed_exponent(ctl, insum));
+    const double lunscale = ldexp(1.0, -e.lo_shift);
+    const double c = rhs_const64(e);
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const uint32_t i = j * kReduce64Threads + threadIdx.x;
+    const uint32_t u = (b.x << kBinShift) + i;
+    if (u < e.n) {
+        long long hi = 0, lo = 0;
+        for (uint32_t q = 0; q < b.z; ++q) {
+            const long long *p = part + ((size_t)(b.y + q) * kBin + i) * 2;
+            hi += p[0];
+            lo += p[1];
+        }
+        finish_row64(e, u, ((double)hi + (double)lo * lunscale) * unscale, c, s);
+    }
+    block_partials<kReduce64Threads>(s, e.partials, b.w + j);
+}
+
+// ---- host side ----------------------------------------------------------------------------
 
 // Bound of the fixed-point sums: the largest in-coefficient sum of a row (x 1.01 for the FP32
-// rounding of the coefficients).
-void binned_insum(nkpr_graph *g) {
+// rounding of the coefficients; FP64 products round by at most 2^-52).
+void binned_insum(nkpr_graph *g, BinLayout &L) {
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     DevBuf best;
     best.alloc(8);
     NKPR_HIP(hipMemsetAsync(best.p, 0, 8, s));
-    if (n)
-        hipLaunchKernelGGL(bin_insum_kernel, dim3((unsigned)std::min<uint64_t>(((uint64_t)n * 64 + 255) / 256, 4096)),
-                           dim3(256), 0, s, g->b_rowptr.as<uint32_t>(), n, g->b_src.as<int32_t>(),
-                           g->weighted ? g->b_coef32.as<float>() : nullptr,
+    const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)n * 64 + 255) / 256, 4096);
+    if (n && L.fp64)
+        hipLaunchKernelGGL(bin_insum_kernel<double>, dim3(grid), dim3(256), 0, s, g->b_rowptr.as<uint32_t>(), n,
+                           g->b_src.as<int32_t>(), g->weighted ? g->b_coef64.as<double>() : nullptr,
+                           g->weighted ? nullptr : g->inv64.as<double>(), best.as<unsigned long long>());
+    else if (n)
+        hipLaunchKernelGGL(bin_insum_kernel<float>, dim3(grid), dim3(256), 0, s, g->b_rowptr.as<uint32_t>(), n,
+                           g->b_src.as<int32_t>(), g->weighted ? g->b_coef32.as<float>() : nullptr,
                            g->weighted ? nullptr : g->inv32.as<float>(), best.as<unsigned long long>());
     NKPR_HIP(hipGetLastError());
     unsigned long long hb = 0;
@@ -429,7 +670,7 @@ void binned_insum(nkpr_graph *g) {
     NKPR_HIP(hipStreamSynchronize(s));
     double v;
     std::memcpy(&v, &hb, 8);
-    g->bn_insum = v * 1.01;
+    L.insum = v * (L.fp64 ? 1.0 + 1e-9 : 1.01);
 }
 
 // Items sorted by decreasing cost (stable: equal costs keep their order).
@@ -442,30 +683,32 @@ void sort_by_cost(std::vector<uint4> &items, const std::vector<uint64_t> &cost) 
     items.swap(out);
 }
 
-}  // namespace
-
-// The binned operator replaces the pull sweep in the GMRES corrections of graphs with at least
-// kBinnedMinEdges edges (on R-MAT scale 26: 3.8 ms per application against 8.4 ms; on graphs
-// whose gathers stay in the caches the pull sweep is as fast and needs no second edge copy).
-// NKPR_BINNED=0 / 1 forces either operator.
-constexpr uint64_t kBinnedMinEdges = 1ull << 26;
-
-bool binned_enabled(const nkpr_graph *g) {
-    if (g->bn_off) return false;
-    if (const char *e = std::getenv("NKPR_BINNED"); e && *e) return std::atoi(e) != 0;
-    return g->m >= kBinnedMinEdges;
+void emit(nkpr_graph *g, BinLayout &L, const uint32_t *row) {
+    const hipStream_t s = g->stream;
+    if (!L.mp) return;
+    if (L.fp64)
+        hipLaunchKernelGGL(bin_emit_kernel<double>, dim3(nb(L.mp)), dim3(256), 0, s, L.order.as<uint32_t>(),
+                           L.pos.as<uint32_t>(), L.mp, L.span_shift, g->b_src.as<int32_t>(), row,
+                           g->weighted ? g->b_coef64.as<double>() : nullptr, L.src.as<uint16_t>(),
+                           L.dst.as<uint16_t>(), g->weighted ? L.coef.as<double>() : nullptr);
+    else
+        hipLaunchKernelGGL(bin_emit_kernel<float>, dim3(nb(L.mp)), dim3(256), 0, s, L.order.as<uint32_t>(),
+                           L.pos.as<uint32_t>(), L.mp, L.span_shift, g->b_src.as<int32_t>(), row,
+                           g->weighted ? g->b_coef32.as<float>() : nullptr, L.src.as<uint16_t>(),
+                           L.dst.as<uint16_t>(), g->weighted ? L.coef.as<float>() : nullptr);
+    NKPR_HIP(hipGetLastError());
 }
 
-// Builds the binned edge copy (once per graph; coefficients again after new weights).  Returns
-// false when the padded copy would not fit 32-bit positions (the pull sweep is used instead).
-bool binned_layout(nkpr_graph *g) {
-    batch_layout(g);
-    if (g->bn_off) return false;
-    if (g->bn_ready && g->bn_coef_ready) return true;
+// Builds the binned copy L (once per graph; coefficients again after new weights).  Returns
+// false when the padded copy would not fit 32-bit positions.
+bool build_layout(nkpr_graph *g, BinLayout &L) {
+    if (L.ready && L.coef_ready) return true;
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    const uint32_t nchunks = std::max<uint32_t>(1, (g->sink_begin + kSpan - 1) / kSpan);
+    const int ss = L.span_shift;
+    const uint32_t span = 1u << ss;
+    const uint32_t nchunks = std::max<uint32_t>(1, (g->sink_begin + span - 1) / span);
     const uint32_t nbins = (n + kBin - 1) / kBin;
     const uint64_t nseg = (uint64_t)nchunks * nbins;
     DevBuf row, k1, k2, v1, v2, tmp, rseg, cseg, bseg;
@@ -473,17 +716,12 @@ bool binned_layout(nkpr_graph *g) {
     if (n)
         hipLaunchKernelGGL(bin_row_kernel, dim3((unsigned)std::min<uint64_t>(((uint64_t)n * 64 + 255) / 256, 1u << 18)), dim3(256), 0, s,
                            g->b_rowptr.as<uint32_t>(), n, row.as<uint32_t>());
-    if (g->bn_ready) {
+    if (L.ready) {
         // new coefficients, same order: re-emit from the kept order
-        if (g->bn_mp && g->weighted)
-            hipLaunchKernelGGL(bin_emit_kernel, dim3(nb(g->bn_mp)), dim3(256), 0, s, g->bn_order.as<uint32_t>(),
-                               g->bn_pos.as<uint32_t>(), g->bn_mp, g->b_src.as<int32_t>(), row.as<uint32_t>(),
-                               g->b_coef32.as<float>(), g->bn_src.as<uint16_t>(), g->bn_dst.as<uint16_t>(),
-                               g->bn_coef.as<float>());
-        NKPR_HIP(hipGetLastError());
+        if (g->weighted) emit(g, L, row.as<uint32_t>());
         NKPR_HIP(hipStreamSynchronize(s));
-        binned_insum(g);
-        g->bn_coef_ready = true;
+        binned_insum(g, L);
+        L.coef_ready = true;
         return true;
     }
     // sort the edges by key and find the unpadded segment starts (chunk-major)
@@ -495,12 +733,12 @@ bool binned_layout(nkpr_graph *g) {
         v1.alloc(4 * m);
         v2.alloc(4 * m);
         hipLaunchKernelGGL(bin_key_kernel, dim3(nb(m)), dim3(256), 0, s, g->b_src.as<int32_t>(),
-                           row.as<uint32_t>(), m, nbins, k1.as<uint64_t>(), v1.as<uint32_t>());
+                           row.as<uint32_t>(), m, nbins, ss, k1.as<uint64_t>(), v1.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
         hipcub::DoubleBuffer<uint64_t> kb(k1.as<uint64_t>(), k2.as<uint64_t>());
         hipcub::DoubleBuffer<uint32_t> vb(v1.as<uint32_t>(), v2.as<uint32_t>());
-        int bits = kSpanShift + 1;
-        while (bits < 64 && (1ull << (bits - kSpanShift)) < nseg) ++bits;
+        int bits = ss + 1;
+        while (bits < 64 && (1ull << (bits - ss)) < nseg) ++bits;
         size_t tb = 0;
         NKPR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (size_t)m, 0, bits, s));
         tmp.alloc(tb);
@@ -513,7 +751,7 @@ bool binned_layout(nkpr_graph *g) {
         v2.reset();
         tmp.reset();
         hipLaunchKernelGGL(bin_seg_kernel, dim3(nb(nseg + 1)), dim3(256), 0, s,
-                           (const uint64_t *)k1.as<uint64_t>(), m, nseg, rseg.as<uint32_t>());
+                           (const uint64_t *)k1.as<uint64_t>(), m, nseg, ss, rseg.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
         NKPR_HIP(hipMemcpyAsync(seg.data(), rseg.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, s));
         NKPR_HIP(hipStreamSynchronize(s));
@@ -527,7 +765,7 @@ bool binned_layout(nkpr_graph *g) {
         chost[q] = (uint32_t)mp;
         mp += padded(q);
         if (mp > kMaxPadded) {
-            g->bn_off = true;
+            L.off = true;
             if (std::getenv("NKPR_DEBUG"))
                 std::fprintf(stderr, "[nkpr] binned layout: padded copy exceeds 32-bit positions\n");
             return false;
@@ -546,34 +784,30 @@ bool binned_layout(nkpr_graph *g) {
     }
     bin_start[nbins] = (uint32_t)at;
     bhost[nseg] = (uint32_t)mp;
-    g->bn_mp = mp;
+    L.mp = mp;
     cseg.alloc(4 * (nseg + 1));
     bseg.alloc(4 * (nseg + 1));
     NKPR_HIP(hipMemcpyAsync(cseg.p, chost.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, s));
     NKPR_HIP(hipMemcpyAsync(bseg.p, bhost.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, s));
-    g->bn_order.alloc(4 * std::max<uint64_t>(mp, 1));
-    NKPR_HIP(hipMemsetAsync(g->bn_order.p, 0xFF, 4 * std::max<uint64_t>(mp, 1), s));
+    L.order.alloc(4 * std::max<uint64_t>(mp, 1));
+    NKPR_HIP(hipMemsetAsync(L.order.p, 0xFF, 4 * std::max<uint64_t>(mp, 1), s));
     if (m)
         hipLaunchKernelGGL(bin_place_kernel, dim3(nb(m)), dim3(256), 0, s, (const uint64_t *)k1.as<uint64_t>(),
-                           (const uint32_t *)v1.as<uint32_t>(), m, (const uint32_t *)rseg.as<uint32_t>(),
-                           (const uint32_t *)cseg.as<uint32_t>(), g->bn_order.as<uint32_t>());
+                           (const uint32_t *)v1.as<uint32_t>(), m, ss, (const uint32_t *)rseg.as<uint32_t>(),
+                           (const uint32_t *)cseg.as<uint32_t>(), L.order.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
-    g->bn_pos.alloc(4 * std::max<uint64_t>(mp / kQuad, 1));
+    L.pos.alloc(4 * std::max<uint64_t>(mp / kQuad, 1));
     if (mp)
         hipLaunchKernelGGL(bin_pos_kernel, dim3((unsigned)std::min<uint64_t>((nseg * 64 + 255) / 256, 1u << 18)),
                            dim3(256), 0, s, (const uint32_t *)cseg.as<uint32_t>(),
-                           (const uint32_t *)bseg.as<uint32_t>(), nseg, g->bn_pos.as<uint32_t>());
+                           (const uint32_t *)bseg.as<uint32_t>(), nseg, L.pos.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
-    g->bn_src.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
-    g->bn_dst.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
-    g->bn_val.alloc(4 * std::max<uint64_t>(mp, 1) + 64);
-    if (g->weighted) g->bn_coef.alloc(4 * std::max<uint64_t>(mp, 1) + 64);
-    if (mp)
-        hipLaunchKernelGGL(bin_emit_kernel, dim3(nb(mp)), dim3(256), 0, s, g->bn_order.as<uint32_t>(),
-                           g->bn_pos.as<uint32_t>(), mp, g->b_src.as<int32_t>(), row.as<uint32_t>(),
-                           g->weighted ? g->b_coef32.as<float>() : nullptr, g->bn_src.as<uint16_t>(),
-                           g->bn_dst.as<uint16_t>(), g->weighted ? g->bn_coef.as<float>() : nullptr);
-    NKPR_HIP(hipGetLastError());
+    const size_t vb = L.fp64 ? 8 : 4;
+    L.src.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
+    L.dst.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
+    L.val.alloc(vb * std::max<uint64_t>(mp, 1) + 64);
+    if (g->weighted) L.coef.alloc(vb * std::max<uint64_t>(mp, 1) + 64);
+    emit(g, L, row.as<uint32_t>());
     NKPR_HIP(hipStreamSynchronize(s));
 
     // work items (NKPR_BINNED_ITEM overrides both sizes, e.g. to exercise split bins); sizes in
@@ -583,93 +817,213 @@ bool binned_layout(nkpr_graph *g) {
         const uint64_t v = std::strtoull(ev, nullptr, 10);
         if (v > 0) e1max = e2max = (v + kQuad - 1) / kQuad * kQuad;
     }
-    std::vector<uint4> p1, p2, mb;  // mb: {bin, first slot, slot count} of the split bins
+    std::vector<uint4> p1, p2, mb;  // mb: {bin, first slot, slot count, partials slot}
     std::vector<uint64_t> cost1, cost2;
     for (uint32_t c = 0; c < nchunks; ++c) {
         const uint32_t b0 = chost[(uint64_t)c * nbins], b1 = chost[(uint64_t)(c + 1) * nbins];
         for (uint64_t e = b0; e < b1; e += e1max) {
             const uint64_t e1 = std::min<uint64_t>(e + e1max, b1);
             p1.push_back(make_uint4(c, (uint32_t)e, (uint32_t)e1, 0));
-            cost1.push_back(e1 - e + kSpan);  // edges + the LDS table fill
+            cost1.push_back(e1 - e + span);  // edges + the LDS table fill
         }
     }
-    uint32_t slots = 0;
+    // every bin finished in one place (a sole item or the reduce kernel) has a partials slot
+    uint32_t slots = 0, pslots = 0;
     for (uint32_t b = 0; b < nbins; ++b) {
         const uint64_t s0 = bin_start[b], s1 = bin_start[b + 1];
         const uint32_t pieces = (uint32_t)std::max<uint64_t>(1, (s1 - s0 + e2max - 1) / e2max);
         for (uint32_t i = 0; i < pieces; ++i) {
             const uint64_t e0 = s0 + i * e2max, e1 = std::min<uint64_t>(e0 + e2max, s1);
-            const uint32_t flags = pieces == 1 ? kItemSole : (slots + i) << 2;
+            const uint32_t flags = pieces == 1 ? kItemSole | (pslots << 2) : (slots + i) << 2;
             p2.push_back(make_uint4(b, (uint32_t)e0, (uint32_t)e1, flags));
             cost2.push_back(e1 - e0 + kBin);  // edges + the accumulator clear and flush
         }
         if (pieces > 1) {
-            // every item of a split bin writes its accumulator to a slot of its own
-            mb.push_back(make_uint4(b, slots, pieces, 0));
+            // every item of a split bin writes its accumulator to a slot of its own; the FP64
+            // reduce kernel finishes the bin in kReduce64Blocks blocks with a partials slot each
+            mb.push_back(make_uint4(b, slots, pieces, pslots));
             slots += pieces;
+            pslots += L.fp64 ? kReduce64Blocks : 1;
+        } else {
+            ++pslots;
         }
     }
     sort_by_cost(p1, cost1);
     sort_by_cost(p2, cost2);
-    g->bn_p1.alloc(sizeof(uint4) * std::max<size_t>(p1.size(), 1));
-    g->bn_p2.alloc(sizeof(uint4) * std::max<size_t>(p2.size(), 1));
-    if (!p1.empty()) NKPR_HIP(hipMemcpyAsync(g->bn_p1.p, p1.data(), sizeof(uint4) * p1.size(), hipMemcpyHostToDevice, s));
-    if (!p2.empty()) NKPR_HIP(hipMemcpyAsync(g->bn_p2.p, p2.data(), sizeof(uint4) * p2.size(), hipMemcpyHostToDevice, s));
-    g->bn_np1 = (uint32_t)p1.size();
-    g->bn_np2 = (uint32_t)p2.size();
-    g->bn_multi = slots;
-    g->bn_nmb = (uint32_t)mb.size();
-    g->bn_mb.alloc(sizeof(uint4) * std::max<size_t>(mb.size(), 1));
-    if (!mb.empty()) NKPR_HIP(hipMemcpyAsync(g->bn_mb.p, mb.data(), sizeof(uint4) * mb.size(), hipMemcpyHostToDevice, s));
-    g->bn_part.alloc(8 * (size_t)std::max<uint32_t>(slots, 1) * kBin);
-    binned_insum(g);
-    g->bn_vmax.alloc(16);
+    L.p1.alloc(sizeof(uint4) * std::max<size_t>(p1.size(), 1));
+    L.p2.alloc(sizeof(uint4) * std::max<size_t>(p2.size(), 1));
+    if (!p1.empty()) NKPR_HIP(hipMemcpyAsync(L.p1.p, p1.data(), sizeof(uint4) * p1.size(), hipMemcpyHostToDevice, s));
+    if (!p2.empty()) NKPR_HIP(hipMemcpyAsync(L.p2.p, p2.data(), sizeof(uint4) * p2.size(), hipMemcpyHostToDevice, s));
+    L.np1 = (uint32_t)p1.size();
+    L.np2 = (uint32_t)p2.size();
+    L.multi = slots;
+    L.nmb = (uint32_t)mb.size();
+    L.nslots = pslots;
+    L.mb.alloc(sizeof(uint4) * std::max<size_t>(mb.size(), 1));
+    if (!mb.empty()) NKPR_HIP(hipMemcpyAsync(L.mb.p, mb.data(), sizeof(uint4) * mb.size(), hipMemcpyHostToDevice, s));
+    L.part.alloc((L.fp64 ? 16 : 8) * (size_t)std::max<uint32_t>(slots, 1) * kBin);
+    if (L.fp64) L.partials.alloc(32 * (size_t)std::max<uint32_t>(pslots, 1));
+    binned_insum(g, L);
+    L.ctl.alloc(sizeof(BinCtl));
     NKPR_HIP(hipStreamSynchronize(s));
     if (std::getenv("NKPR_DEBUG"))
-        std::fprintf(stderr, "[nkpr] binned layout: %u chunks x %u bins, %llu padded edges, %u + %u items (%u split)\n",
-                     nchunks, nbins, (unsigned long long)mp, g->bn_np1, g->bn_np2, g->bn_multi);
-    g->bn_ready = true;
-    g->bn_coef_ready = true;
+        std::fprintf(stderr, "[nkpr] binned %s layout: %u chunks x %u bins, %llu padded edges, %u + %u items (%u split)\n",
+                     L.fp64 ? "FP64" : "FP32", nchunks, nbins, (unsigned long long)mp, L.np1, L.np2, L.multi);
+    L.ready = true;
+    L.coef_ready = true;
     return true;
+}
+
+void release_layout(BinLayout &L) {
+    for (DevBuf *b : {&L.order, &L.pos, &L.src, &L.dst, &L.coef, &L.val, &L.p1, &L.p2, &L.mb,
+                      &L.part, &L.ctl, &L.partials})
+        b->reset();
+    L.ready = L.coef_ready = false;
+}
+
+// Builds L if needed.  False when it cannot be built (too many edges for 32-bit positions, or
+// not enough device memory for the copy and its build: about 17 + 26 bytes per edge for the
+// FP32 copy, 25 + 26 for the FP64 one); the pull sweep is used instead.
+bool layout_ready(nkpr_graph *g, BinLayout &L) {
+    batch_layout(g);
+    if (L.off) return false;
+    try {
+        return build_layout(g, L);
+    } catch (const Error &e) {
+        if (e.code != NKPR_ERR_NOMEM) throw;
+        NKPR_HIP(hipStreamSynchronize(g->stream));
+        release_layout(L);
+        L.off = true;
+        if (std::getenv("NKPR_DEBUG"))
+            std::fprintf(stderr, "[nkpr] binned layout: %s; the pull sweep stays\n", e.what());
+        return false;
+    }
+}
+
+// persistent grids: every block that fits on the device at once
+template <typename K1, typename K2>
+void persistent_grids(nkpr_graph *g, BinLayout &L, K1 k1, K2 k2) {
+    if (L.grid1) return;
+    int cus = 0, b1 = 0, b2 = 0;
+    NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    NKPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k1, kPassThreads, 0));
+    NKPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k2, kPassThreads, 0));
+    L.grid1 = (uint32_t)std::max(1, b1 * cus);
+    L.grid2 = (uint32_t)std::max(1, b2 * cus);
+}
+
+// The binned copies serve graphs with at least kBinnedMinEdges edges (on R-MAT scale 26 the
+// FP32 operator takes 3.8 ms per application against 8.4 ms for the pull sweep; on graphs
+// whose gathers stay in the caches the pull sweep is as fast and needs no second copy).
+// NKPR_BINNED=0 / 1 (FP32 operator) and NKPR_BINNED64=0 / 1 (FP64 sweep) force either way.
+constexpr uint64_t kBinnedMinEdges = 1ull << 26;
+
+bool policy(const nkpr_graph *g, const char *env) {
+    if (const char *e = std::getenv(env); e && *e) return std::atoi(e) != 0;
+    return g->m >= kBinnedMinEdges;
+}
+
+}  // namespace
+
+bool binned_enabled(const nkpr_graph *g) { return !g->bn32.off && policy(g, "NKPR_BINNED"); }
+
+bool binned_layout(nkpr_graph *g) {
+    g->bn32.span_shift = kSpan32Shift;
+    g->bn32.fp64 = false;
+    return layout_ready(g, g->bn32);
 }
 
 // w = B v = d P v (+ d S(v)/n) through the binned passes, or w = A v = v - B v (minus).
 void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
                   bool minus, hipEvent_t ev0, hipEvent_t ev1) {
+    BinLayout &L = g->bn32;
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     if (!n) return;
-    int cus = 0;
-    NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
     const double *sink_sum = nullptr;
     if (distribute && g->sink_begin < n) {
         launch_range_sum<float>(v, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
         sink_sum = g->red_out.as<double>();
     }
-    if (!g->bn_grid1) {
-        // persistent grids: every block that fits on the device at once
-        int b1 = 0, b2 = 0;
-        NKPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, bin_scatter_kernel, kPassThreads, 0));
-        NKPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, bin_gather_kernel, kPassThreads, 0));
-        g->bn_grid1 = (uint32_t)std::max(1, b1 * cus);
-        g->bn_grid2 = (uint32_t)std::max(1, b2 * cus);
-    }
+    persistent_grids(g, L, bin_scatter_kernel, bin_gather_kernel);
     if (ev0) NKPR_HIP(hipEventRecord(ev0, s));
-    uint32_t *ctl = g->bn_vmax.as<uint32_t>();
+    BinCtl *ctl = L.ctl.as<BinCtl>();
     hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(g->bn_np1, 1), g->bn_grid1)),
-                       dim3(kPassThreads), 0, s, g->bn_p1.as<uint4>(), g->bn_np1, v,
-                       g->weighted ? nullptr : g->inv32.as<float>(), n, g->bn_src.as<uint16_t>(),
-                       g->weighted ? g->bn_coef.as<float>() : nullptr, g->bn_pos.as<uint32_t>(),
-                       g->bn_val.as<float>(), ctl, ctl + 1);
-    hipLaunchKernelGGL(bin_gather_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(g->bn_np2, 1), g->bn_grid2)),
-                       dim3(kPassThreads), 0, s, g->bn_p2.as<uint4>(), g->bn_np2,
-                       g->bn_dst.as<uint16_t>(), g->bn_val.as<float>(), g->bn_part.as<long long>(), n,
-                       (const uint32_t *)ctl, g->bn_insum, v, minus, (float)damp, sink_sum, w, ctl + 2);
-    if (g->bn_nmb)
-        hipLaunchKernelGGL(bin_reduce_kernel, dim3(nb((uint64_t)g->bn_nmb * kBin)), dim3(256), 0, s,
-                           g->bn_mb.as<uint4>(), g->bn_nmb, (const long long *)g->bn_part.as<long long>(), n,
-                           (const uint32_t *)ctl, g->bn_insum, v, minus, (float)damp, sink_sum, w);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1)),
+                       dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, v,
+                       g->weighted ? nullptr : g->inv32.as<float>(), n, L.src.as<uint16_t>(),
+                       g->weighted ? L.coef.as<float>() : nullptr, L.pos.as<uint32_t>(), L.val.as<float>(),
+                       ctl);
+    hipLaunchKernelGGL(bin_gather_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
+                       dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2, L.dst.as<uint16_t>(),
+                       L.val.as<float>(), L.part.as<long long>(), n, ctl, L.insum, v, minus, (float)damp,
+                       sink_sum, w);
+    if (L.nmb)
+        hipLaunchKernelGGL(bin_reduce_kernel, dim3(nb((uint64_t)L.nmb * kBin)), dim3(256), 0, s,
+                           L.mb.as<uint4>(), L.nmb, (const long long *)L.part.as<long long>(), n,
+                           (const BinCtl *)ctl, L.insum, v, minus, (float)damp, sink_sum, w);
+    if (ev1) NKPR_HIP(hipEventRecord(ev1, s));
+    NKPR_HIP(hipGetLastError());
+}
+
+bool binned64_active(nkpr_graph *g) {
+    if (g->bn64.off || g->n_groups != 0 || !policy(g, "NKPR_BINNED64")) return false;
+    g->bn64.span_shift = kSpan64Shift;
+    g->bn64.fp64 = true;
+    return layout_ready(g, g->bn64);
+}
+
+// One FP64 sweep x' = d P x + b through the binned passes, with the epilogue, reductions and
+// stopping rule of launch_sweep<double> (the finalize kernel adds the per-bin partials in bin
+// order).  Every kernel returns at once when the phase has stopped.
+void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArgs &f, int kind,
+                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    BinLayout &L = g->bn64;
+    const uint32_t n = g->n;
+    persistent_grids(g, L, bin_scatter64_kernel, bin_gather64_kernel);
+    const bool basevec = kind == kSweepBaseVec || kind == kSweepBaseResid;
+    const bool resid = kind == kSweepResid || kind == kSweepBaseResid;
+    Epi64 e{};
+    e.xold = a.xold;
+    e.xnew = a.xnew;
+    e.xhat_new = a.xhat_new;
+    e.inv = g->weighted ? nullptr : g->inv64.as<double>();
+    e.base_vec = basevec ? a.base_vec : nullptr;
+    e.r32 = resid ? a.r32 : nullptr;
+    e.partials = L.partials.as<double>();
+    e.pctl = a.ctl;
+    e.damp = a.damp;
+    e.tele = a.tele;
+    e.n = n;
+    e.sink_begin = a.sink_begin;
+    e.distribute = a.distribute;
+    // fraction bits of the low word: a row's fractions (each below 2^lo_shift in magnitude)
+    // must sum below 2^63
+    int deg_bits = 0;
+    while (deg_bits < 40 && (1ull << deg_bits) <= (uint64_t)g->max_in_degree) ++deg_bits;
+    e.lo_shift = std::min(40, 62 - deg_bits);
+    if (ev0) NKPR_HIP(hipEventRecord(ev0, s));
+    BinCtl *ctl = L.ctl.as<BinCtl>();
+    hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
+    if (n) {
+        hipLaunchKernelGGL(bin_scatter64_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1)),
+                           dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, a.xold,
+                           g->weighted ? nullptr : g->inv64.as<double>(), n, L.src.as<uint16_t>(),
+                           g->weighted ? L.coef.as<double>() : nullptr, L.pos.as<uint32_t>(),
+                           L.val.as<double>(), ctl, (const PhaseCtl *)a.ctl);
+        hipLaunchKernelGGL(bin_gather64_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
+                           dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2, L.dst.as<uint16_t>(),
+                           L.val.as<double>(), L.part.as<long long>(), ctl, L.insum, e);
+        if (L.nmb)
+            hipLaunchKernelGGL(bin_reduce64_kernel, dim3(L.nmb * kReduce64Blocks), dim3(kReduce64Threads), 0, s, L.mb.as<uint4>(),
+                               (const long long *)L.part.as<long long>(), (const BinCtl *)ctl, L.insum, e);
+    }
+    SweepArgs<double> fa = a;
+    fa.partials = L.partials.as<double>();
+    FinalizeArgs ff = f;
+    ff.n_split_rows = 0;
+    ff.n_blocks = n ? L.nslots : 0;
+    launch_finalize64(fa, ff, kind, s);
     if (ev1) NKPR_HIP(hipEventRecord(ev1, s));
     NKPR_HIP(hipGetLastError());
 }

@@ -258,10 +258,19 @@ struct BatchTune {
 void batch_layout(nkpr_graph *g);
 void batch_split(nkpr_graph *g, uint32_t heavy);
 void ensure_buf(DevBuf &b, size_t bytes);
+// Binned FP32 operator of the GMRES corrections (nkpr_binned.hip).
 bool binned_enabled(const nkpr_graph *g);
 bool binned_layout(nkpr_graph *g);
 void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool distribute, bool minus,
                   hipEvent_t ev0, hipEvent_t ev1);
+// Binned FP64 sweep: true when FP64 sweeps of this graph run binned (builds the copy on first
+// use); binned_sweep64 is one sweep with the epilogue, reductions and stop rule of
+// launch_sweep<double>.
+bool binned64_active(nkpr_graph *g);
+void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArgs &f, int kind,
+                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+// The finalize step of launch_sweep<double> alone (nkpr_sweep.hip).
+void launch_finalize64(const SweepArgs<double> &a, const FinalizeArgs &f, int kind, hipStream_t s);
 void set_weights(nkpr_graph *g, const float *w, const double *w64, const double *out_wdeg,
                  int device_ptrs);
 void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
@@ -276,6 +285,20 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
 }  // namespace nkpr
 
 // The graph handle.
+namespace nkpr {
+// One binned copy of the edges (nkpr_binned.hip): segments of (source chunk, destination bin),
+// padded to quads, in chunk-major order for pass 1 and bin-major order for pass 2.
+struct BinLayout {
+    bool ready = false, coef_ready = false, off = false;
+    int span_shift = 15;        // ids per source chunk = 2^span_shift
+    bool fp64 = false;          // FP64 coefficients and contributions
+    uint64_t mp = 0;            // padded edges
+    DevBuf order, pos, src, dst, coef, val, p1, p2, mb, part, ctl, partials;
+    double insum = 0.0;         // largest in-coefficient sum of a row (fixed-point bound)
+    uint32_t np1 = 0, np2 = 0, multi = 0, nmb = 0, nslots = 0, grid1 = 0, grid2 = 0;
+};
+}  // namespace nkpr
+
 struct nkpr_graph {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -346,14 +369,9 @@ struct nkpr_graph {
     nkpr::DevBuf q_chunks, q_part, q_ctl, q_add, q_factor, q_flag;
     uint32_t q_nchunks = 0;
 
-    // binned FP32 operator for the GMRES correction solves (nkpr_binned.hip, built on first use)
-    bool bn_ready = false, bn_coef_ready = false, bn_off = false;
-    uint64_t bn_mp = 0;  // edges of the binned copy, padding included
-    nkpr::DevBuf bn_order, bn_pos, bn_src, bn_dst, bn_coef, bn_val, bn_p1, bn_p2, bn_mb, bn_part,
-        bn_vmax;
-    double bn_insum = 0.0;
-    uint32_t bn_np1 = 0, bn_np2 = 0, bn_multi = 0, bn_nmb = 0;
-    uint32_t bn_grid1 = 0, bn_grid2 = 0;  // persistent grid sizes of the two passes
+    // binned copies of the edges (nkpr_binned.hip, built on first use): the FP32 operator of
+    // the GMRES corrections and the FP64 sweep
+    nkpr::BinLayout bn32, bn64;
 
     // adjoint: the transposed operator d P^T as a graph of its own (built on first use; it
     // shares this graph's stream and device)

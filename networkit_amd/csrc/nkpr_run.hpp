@@ -84,6 +84,9 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
     a.partials = g->partials.as<double>();
     a.ctl = g->ctl.as<PhaseCtl>();
 
+    // FP64 sweeps of large graphs run through the binned copy (a handle with per-graph hooks
+    // keeps the pull sweep)
+    const bool binned64 = dbl && !post && binned64_active(g);
     uint64_t launched = 0, done_sweeps = 0;
     uint64_t batch = (max_launch <= 2) ? max_launch : 4;
     const size_t need_ev = 2 * 64;
@@ -107,7 +110,12 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
             a.xhat_new = folded ? nullptr : v.xh[nxt];
             hipEvent_t e0 = timing ? g->events[2 * j] : nullptr;
             hipEvent_t e1 = timing ? g->events[2 * j + 1] : nullptr;
-            launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
+            if constexpr (sizeof(T) == 8) {
+                if (binned64) binned_sweep64(g, a, f, kind, g->stream, e0, e1);
+                else launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
+            } else {
+                launch_sweep<T>(a, f, kind, folded, rc.combine_grid, g->stream, e0, e1);
+            }
             if (post) post(v.x[cur], v.x[nxt]);
         }
         NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,

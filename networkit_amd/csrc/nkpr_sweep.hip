@@ -608,6 +608,21 @@ template void launch_sweep<float>(const SweepArgs<float> &, const FinalizeArgs &
 template void launch_sweep<double>(const SweepArgs<double> &, const FinalizeArgs &, int, bool,
                                    int, hipStream_t, hipEvent_t, hipEvent_t);
 
+void launch_finalize64(const SweepArgs<double> &a, const FinalizeArgs &f, int kind, hipStream_t s) {
+    // the template flags only matter for split rows, which a binned sweep does not have
+    const bool basevec = kind == kSweepBaseVec || kind == kSweepBaseResid;
+    const bool resid = kind == kSweepResid || kind == kSweepBaseResid;
+    if (basevec && resid)
+        hipLaunchKernelGGL((finalize_kernel<double, true, true>), dim3(1), dim3(kFinalizeThreads), 0, s, a, f);
+    else if (basevec)
+        hipLaunchKernelGGL((finalize_kernel<double, true, false>), dim3(1), dim3(kFinalizeThreads), 0, s, a, f);
+    else if (resid)
+        hipLaunchKernelGGL((finalize_kernel<double, false, true>), dim3(1), dim3(kFinalizeThreads), 0, s, a, f);
+    else
+        hipLaunchKernelGGL((finalize_kernel<double, false, false>), dim3(1), dim3(kFinalizeThreads), 0, s, a, f);
+    NKPR_HIP(hipGetLastError());
+}
+
 int sweep_resident_blocks(int device, bool dbl, bool folded, int kind) {
     const void *fn;
     if (dbl) {

@@ -411,3 +411,54 @@ def test_mixed_binned_operator_edge_cases(monkeypatch, case):
     assert st["l1_error_bound"] <= 1e-10
     ex = np.ones(z, bool) if exists is None else exists.astype(bool)
     assert np.abs(x[ex] - ref[ex]).sum() <= 1e-10
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+@pytest.mark.parametrize("sinks", [0, 1])
+@pytest.mark.parametrize("normalized", [False, True])
+@pytest.mark.parametrize("item", ["0", "2000"])
+def test_fp64_binned_sweep_parity(monkeypatch, weighted, sinks, normalized, item):
+    """The reference FP64 iteration with every sweep through the binned FP64 passes (forced on
+    a graph of several source chunks and destination bins; item=2000 splits bins over work
+    items, finished by the reduce kernel): the oracle's iteration count, L1 difference
+    <= 1e-12, maximum() to 1e-12 and the same ranking, with deleted nodes."""
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    monkeypatch.setenv("NKPR_BINNED_ITEM", item)
+    z, u, v, w = skewed_graph(50000, 400000, 65, weighted=weighted)
+    check_fp64_parity(z, u, v, w, True, remove=(3, 4000, 49999), sinks=sinks,
+                      normalized=normalized, tol=1e-10)
+
+
+def test_fp64_binned_sweep_undirected_and_l1(monkeypatch):
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    z, u, v, w = random_graph(30000, 150000, 66, directed=False, weighted=True)
+    check_fp64_parity(z, u, v, w, False, norm=0, tol=1e-11)
+
+
+def test_fp64_binned_sweep_teleport_and_mixed(monkeypatch):
+    """Sweeps with a per-node right-hand side (teleport vector) and the FP64 residual sweeps of
+    the mixed path through the binned FP64 passes."""
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    z, u, v, w = skewed_graph(40000, 300000, 67)
+    g = oracle.in_lists(z, u, v, w, True)
+    rng = np.random.default_rng(8)
+    t = np.zeros(z)
+    np.add.at(t, rng.choice(z, 50), rng.uniform(0.5, 2.0, 50))
+    t /= t.sum()
+    dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), g.in_w.astype(np.float32),
+                          weight64=g.in_w)
+    x = np.full(z, 1.0 / z)
+    st = dg.run(x, capi.default_params(teleport=t.ctypes.data, sink_handling=1))
+    ref, it, _ = oracle.pagerank(g, sink_mode=1, teleport=t)
+    assert st["iterations"] == it
+    assert np.abs(x - ref).sum() <= 1e-12
+    ref2, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=20000)
+    for binned32 in ("0", "1"):
+        monkeypatch.setenv("NKPR_BINNED", binned32)
+        dg2 = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), g.in_w.astype(np.float32),
+                               weight64=g.in_w)
+        x2 = np.full(z, 1.0 / z)
+        st2 = dg2.run(x2, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-10,
+                                              stop=capi.NKPR_STOP_L1_ERROR))
+        assert st2["l1_error_bound"] <= 1e-10
+        assert np.abs(x2 - ref2).sum() <= 1e-10
