@@ -134,8 +134,18 @@ def main():
         a_dst.copy_(a_src)
         torch.cuda.synchronize()
         del a_src, a_dst
-        for name, prec, per_edge, per_node in (("fp32", capi.NKPR_FP32, 12, 16),
-                                                ("fp64", capi.NKPR_FP64, 20, 24)):
+        # plain FP32 and FP64 pull sweeps, then the FP64 sweeps as the library runs them on
+        # this graph (binned from 2^26 edges on)
+        env64 = os.environ.get("NKPR_BINNED64")
+        for name, prec, per_edge, per_node, pull in (("fp32 pull", capi.NKPR_FP32, 12, 16, True),
+                                                      ("fp64 pull", capi.NKPR_FP64, 20, 24, True),
+                                                      ("fp64", capi.NKPR_FP64, 20, 24, False)):
+            if pull:
+                os.environ["NKPR_BINNED64"] = "0"
+            elif env64 is None:
+                os.environ.pop("NKPR_BINNED64", None)
+            else:
+                os.environ["NKPR_BINNED64"] = env64
             torch.cuda.current_stream().synchronize()
             with torch.cuda.stream(stream):
                 x.fill_(1.0 / z)
