@@ -133,6 +133,15 @@ def main():
         a_dst = torch.empty_like(a_src)
         a_dst.copy_(a_src)
         torch.cuda.synchronize()
+        # the streaming rate a plain device copy reaches here, for comparison with the passes
+        t0e, t1e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0e.record()
+        for _ in range(4):
+            a_dst.copy_(a_src)
+        t1e.record()
+        torch.cuda.synchronize()
+        cms = t0e.elapsed_time(t1e) / 4
+        log("copy 1 GiB: %.3f ms = %.0f GB/s (read + write)" % (cms, 2 * 2 ** 30 / (cms / 1e3) / 1e9))
         del a_src, a_dst
         # plain FP32 and FP64 pull sweeps, then the FP64 sweeps as the library runs them on
         # this graph (binned from 2^26 edges on)

@@ -703,6 +703,9 @@ void emit(nkpr_graph *g, BinLayout &L, const uint32_t *row) {
 // false when the padded copy would not fit 32-bit positions.
 bool build_layout(nkpr_graph *g, BinLayout &L) {
     if (L.ready && L.coef_ready) return true;
+    // test hook: the allocation failure of a device that is too small for the copy
+    if (const char *t = std::getenv("NKPR_TEST_BINNED_NOMEM"); t && std::atoi(t) != 0)
+        throw Error(NKPR_ERR_NOMEM, "binned copy: allocation refused (NKPR_TEST_BINNED_NOMEM)");
     const hipStream_t s = g->stream;
     const uint32_t n = g->n;
     const uint64_t m = g->m;

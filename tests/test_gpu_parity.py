@@ -462,3 +462,28 @@ def test_fp64_binned_sweep_teleport_and_mixed(monkeypatch):
                                               stop=capi.NKPR_STOP_L1_ERROR))
         assert st2["l1_error_bound"] <= 1e-10
         assert np.abs(x2 - ref2).sum() <= 1e-10
+
+
+def test_binned_copies_fall_back_when_memory_is_short(monkeypatch):
+    """A device without room for the binned copies (forced from the host: the build's first
+    allocation is refused) keeps the pull sweep: the FP64 run still has the oracle's iteration
+    count and the mixed run still certifies; later runs on the handle do not retry the build."""
+    monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    monkeypatch.setenv("NKPR_TEST_BINNED_NOMEM", "1")
+    z, u, v, w = skewed_graph(20000, 150000, 68)
+    g = oracle.in_lists(z, u, v, w, True)
+    dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), g.in_w.astype(np.float32),
+                          weight64=g.in_w)
+    for _ in range(2):
+        x = np.full(z, 1.0 / z)
+        st = dg.run(x, capi.default_params())
+        ref, it, _ = oracle.pagerank(g)
+        assert st["iterations"] == it
+        assert np.abs(x - ref).sum() <= 1e-12
+    ref2, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=20000)
+    x2 = np.full(z, 1.0 / z)
+    st2 = dg.run(x2, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-10,
+                                         stop=capi.NKPR_STOP_L1_ERROR))
+    assert st2["l1_error_bound"] <= 1e-10
+    assert np.abs(x2 - ref2).sum() <= 1e-10
