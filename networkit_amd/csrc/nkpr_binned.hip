@@ -22,8 +22,9 @@
 //       possible row sum just below 2^61 units; about 17 bytes per edge moved.
 //   FP64 copy (chunks of 16384 ids, FP64 table 128 KB): 128-bit accumulators as two 64-bit
 //       integers (whole units and 2^-lo_shift fractions of a unit), so the rows carry more
-//       than FP64 precision; about 29 bytes per edge moved.  The row epilogue, the reductions
-//       and the stopping rule are those of the pull sweep (nkpr_sweep.hip).
+//       than FP64 precision; about 29 bytes per edge moved (25 when every weight is an FP32
+//       number and the copy keeps 4-byte weights).  The row epilogue, the reductions and the
+//       stopping rule are those of the pull sweep (nkpr_sweep.hip).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -199,6 +200,29 @@ __global__ void bin_emit_kernel(const uint32_t *order, const uint32_t *pos, uint
         bsrc[j] = (uint16_t)((uint32_t)src[e] & ((1u << ss) - 1));
         bdst[jb] = (uint16_t)(row[e] & (kBin - 1));
         if (bcoef) bcoef[j] = coef[e];
+    }
+}
+
+// The FP64 copy of a graph whose weights are all FP32 numbers keeps the weights instead of the
+// folded coefficients: w = coef64 / (1/deg) recovers the weight exactly (its FP64 rounding
+// error is far below half an FP32 unit), and pass 1 stages x / deg in its table.
+__global__ void bin_emit_w32_kernel(const uint32_t *order, const uint32_t *pos, uint64_t mp, int ss,
+                                    const int32_t *src, const uint32_t *row, const double *coef64,
+                                    const double *invw, uint16_t *bsrc, uint16_t *bdst, float *bw) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < mp;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = order[j];
+        const uint32_t jb = pos[j / kQuad] + (uint32_t)(j % kQuad);
+        if (e == kPadEdge) {
+            bsrc[j] = (uint16_t)(1u << ss);
+            bdst[jb] = 0;
+            bw[j] = 0.f;
+            continue;
+        }
+        const uint32_t v = (uint32_t)src[e];
+        bsrc[j] = (uint16_t)(v & ((1u << ss) - 1));
+        bdst[jb] = (uint16_t)(row[e] & (kBin - 1));
+        bw[j] = (float)(coef64[e] / invw[v]);
     }
 }
 
@@ -488,10 +512,24 @@ __device__ __forceinline__ double rhs_const64(const Epi64 &e) {
     return c;
 }
 
+// four coefficients of an FP64 pass-1 quad, from FP64 coefficients or FP32 weights
+__device__ __forceinline__ void load_coef4(const double *c, uint32_t e, f64x2 &c0, f64x2 &c1) {
+    c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(c + e));
+    c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(c + e + 2));
+}
+__device__ __forceinline__ void load_coef4(const float *c, uint32_t e, f64x2 &c0, f64x2 &c1) {
+    const f32x4 w = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(c + e));
+    c0 = f64x2{(double)w.x, (double)w.y};
+    c1 = f64x2{(double)w.z, (double)w.w};
+}
+
 // Pass 1 (FP64): as the FP32 pass with an FP64 table of 16384 sources and FP64 contributions.
+// inv scales the table (1/deg: unweighted graphs, and weighted ones whose copy keeps FP32
+// weights); bcoef is the per-edge factor (FP64 coefficient or FP32 weight) or nullptr.
+template <typename CT>
 __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
     const uint4 *items, uint32_t nitems, const double *x, const double *inv, uint32_t n,
-    const uint16_t *bsrc, const double *bcoef, const uint32_t *pos, double *bval, BinCtl *ctl,
+    const uint16_t *bsrc, const CT *bcoef, const uint32_t *pos, double *bval, BinCtl *ctl,
     const PhaseCtl *pctl) {
     constexpr uint32_t kSpan = 1u << kSpan64Shift;
     if (pctl->done) return;
@@ -523,10 +561,7 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
         if (q < q1) {
             sp = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + q * kQuad));
             at = __builtin_nontemporal_load(pos + q);
-            if (bcoef) {
-                c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + q * kQuad));
-                c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + q * kQuad + 2));
-            }
+            if (bcoef) load_coef4(bcoef, q * kQuad, c0, c1);
         }
         for (; q < q1; q += kPassThreads) {
             const uint32_t qn = q + kPassThreads;
@@ -536,10 +571,7 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
             if (qn < q1) {
                 spn = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + qn * kQuad));
                 atn = __builtin_nontemporal_load(pos + qn);
-                if (bcoef) {
-                    c0n = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + qn * kQuad));
-                    c1n = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bcoef + qn * kQuad + 2));
-                }
+                if (bcoef) load_coef4(bcoef, qn * kQuad, c0n, c1n);
             }
             f64x2 r0 = {tab[sp.x & 0xFFFFu], tab[sp.x >> 16]};
             f64x2 r1 = {tab[sp.y & 0xFFFFu], tab[sp.y >> 16]};
@@ -686,7 +718,12 @@ void sort_by_cost(std::vector<uint4> &items, const std::vector<uint64_t> &cost) 
 void emit(nkpr_graph *g, BinLayout &L, const uint32_t *row) {
     const hipStream_t s = g->stream;
     if (!L.mp) return;
-    if (L.fp64)
+    if (L.fp64 && L.w32)
+        hipLaunchKernelGGL(bin_emit_w32_kernel, dim3(nb(L.mp)), dim3(256), 0, s, L.order.as<uint32_t>(),
+                           L.pos.as<uint32_t>(), L.mp, L.span_shift, g->b_src.as<int32_t>(), row,
+                           g->b_coef64.as<double>(), g->invw64.as<double>(), L.src.as<uint16_t>(),
+                           L.dst.as<uint16_t>(), L.coef.as<float>());
+    else if (L.fp64)
         hipLaunchKernelGGL(bin_emit_kernel<double>, dim3(nb(L.mp)), dim3(256), 0, s, L.order.as<uint32_t>(),
                            L.pos.as<uint32_t>(), L.mp, L.span_shift, g->b_src.as<int32_t>(), row,
                            g->weighted ? g->b_coef64.as<double>() : nullptr, L.src.as<uint16_t>(),
@@ -806,10 +843,11 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
                            (const uint32_t *)bseg.as<uint32_t>(), nseg, L.pos.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
     const size_t vb = L.fp64 ? 8 : 4;
+    L.w32 = L.fp64 && g->weighted && g->w32_exact;
     L.src.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
     L.dst.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
     L.val.alloc(vb * std::max<uint64_t>(mp, 1) + 64);
-    if (g->weighted) L.coef.alloc(vb * std::max<uint64_t>(mp, 1) + 64);
+    if (g->weighted) L.coef.alloc((L.w32 ? 4 : vb) * std::max<uint64_t>(mp, 1) + 64);
     emit(g, L, row.as<uint32_t>());
     NKPR_HIP(hipStreamSynchronize(s));
 
@@ -983,7 +1021,8 @@ void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArg
                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     BinLayout &L = g->bn64;
     const uint32_t n = g->n;
-    persistent_grids(g, L, bin_scatter64_kernel, bin_gather64_kernel);
+    if (L.w32) persistent_grids(g, L, bin_scatter64_kernel<float>, bin_gather64_kernel);
+    else persistent_grids(g, L, bin_scatter64_kernel<double>, bin_gather64_kernel);
     const bool basevec = kind == kSweepBaseVec || kind == kSweepBaseResid;
     const bool resid = kind == kSweepResid || kind == kSweepBaseResid;
     Epi64 e{};
@@ -1009,11 +1048,17 @@ void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArg
     BinCtl *ctl = L.ctl.as<BinCtl>();
     hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
     if (n) {
-        hipLaunchKernelGGL(bin_scatter64_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1)),
-                           dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, a.xold,
-                           g->weighted ? nullptr : g->inv64.as<double>(), n, L.src.as<uint16_t>(),
-                           g->weighted ? L.coef.as<double>() : nullptr, L.pos.as<uint32_t>(),
-                           L.val.as<double>(), ctl, (const PhaseCtl *)a.ctl);
+        const dim3 g1(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1));
+        if (L.w32)
+            hipLaunchKernelGGL(bin_scatter64_kernel<float>, g1, dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1,
+                               a.xold, (const double *)g->invw64.as<double>(), n, L.src.as<uint16_t>(),
+                               (const float *)L.coef.as<float>(), L.pos.as<uint32_t>(), L.val.as<double>(), ctl,
+                               (const PhaseCtl *)a.ctl);
+        else
+            hipLaunchKernelGGL(bin_scatter64_kernel<double>, g1, dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1,
+                               a.xold, g->weighted ? nullptr : (const double *)g->inv64.as<double>(), n,
+                               L.src.as<uint16_t>(), g->weighted ? (const double *)L.coef.as<double>() : nullptr,
+                               L.pos.as<uint32_t>(), L.val.as<double>(), ctl, (const PhaseCtl *)a.ctl);
         hipLaunchKernelGGL(bin_gather64_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
                            dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2, L.dst.as<uint16_t>(),
                            L.val.as<double>(), L.part.as<long long>(), ctl, L.insum, e);

@@ -177,7 +177,20 @@ __global__ void inv_degree_kernel(const int32_t *perm, uint32_t n, const uint32_
     if (r >= n) return;
     const double v = (outcnt[perm[r]] > 0) ? 1.0 / wdeg[perm[r]] : 0.0;
     inv64[r] = v;
-    inv32[r] = (float)v;
+    if (inv32) inv32[r] = (float)v;
+}
+
+// flag = 1 when some FP64 weight is not the FP32 weight given beside it
+__global__ void fp32_exact_kernel(const float *w, const double *w64, uint64_t m, uint32_t *flag) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        if ((double)w[i] != w64[i]) atomicOr(flag, 1u);
+}
+
+// invw64[u] = 1 / deg[u] (compact order; 0 for nodes without out-weight)
+__global__ void inv_of_kernel(const double *deg, uint32_t n, double *invw64) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) invw64[r] = deg[r] > 0.0 ? 1.0 / deg[r] : 0.0;
 }
 
 // Per sliced row j: number of in-edges whose source falls in each slice (slice-major output).
@@ -749,6 +762,26 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
                                g->perm.as<int32_t>(), n, outcnt_b.as<uint32_t>(), wdeg, g->inv64.as<double>(),
                                g->inv32.as<float>());
         NKPR_HIP(hipGetLastError());
+    } else {
+        // 1 / weighted out-degree, and whether every weight is an FP32 number (then the binned
+        // FP64 sweep reads 4-byte weights instead of 8-byte coefficients)
+        g->invw64.alloc(8 * std::max<uint64_t>(n, 1));
+        if (n)
+            hipLaunchKernelGGL(inv_degree_kernel, dim3(blocks_for(n)), dim3(256), 0, s,
+                               g->perm.as<int32_t>(), n, outcnt_b.as<uint32_t>(), wdeg, g->invw64.as<double>(),
+                               (float *)nullptr);
+        NKPR_HIP(hipGetLastError());
+        g->w32_exact = true;
+        if (w64_in && m) {
+            NKPR_HIP(hipMemsetAsync(flags_b.p, 0, 4, s));
+            hipLaunchKernelGGL(fp32_exact_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)), dim3(256), 0, s,
+                               w_in, w64_in, m, flags_b.as<uint32_t>());
+            NKPR_HIP(hipGetLastError());
+            uint32_t hf = 0;
+            NKPR_HIP(hipMemcpyAsync(&hf, flags_b.p, 4, hipMemcpyDeviceToHost, s));
+            NKPR_HIP(hipStreamSynchronize(s));
+            g->w32_exact = hf == 0;
+        }
     }
 
     // --- schedule ---------------------------------------------------------------------------
@@ -840,6 +873,27 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
                            dim3(256), 0, s, g->src.as<int32_t>(), g->emap.as<uint32_t>(), m, w, w64,
                            deg.as<double>(), g->coef32.as<float>(), g->coef64.as<double>());
     NKPR_HIP(hipGetLastError());
+    if (n)
+        hipLaunchKernelGGL(inv_of_kernel, dim3(blocks_for(n)), dim3(256), 0, s, (const double *)deg.as<double>(), n,
+                           g->invw64.as<double>());
+    NKPR_HIP(hipGetLastError());
+    bool exact = true;
+    if (w && w64 && m) {
+        DevBuf flag;
+        flag.alloc(4);
+        NKPR_HIP(hipMemsetAsync(flag.p, 0, 4, s));
+        hipLaunchKernelGGL(fp32_exact_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)), dim3(256), 0, s,
+                           w, w64, m, flag.as<uint32_t>());
+        NKPR_HIP(hipGetLastError());
+        uint32_t hf = 0;
+        NKPR_HIP(hipMemcpyAsync(&hf, flag.p, 4, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+        exact = hf == 0;
+    } else if (!w) {
+        exact = false;  // FP64 weights alone: not checked
+    }
+    if (exact != g->w32_exact) g->bn64.ready = false;  // the FP64 copy changes its coefficient type
+    g->w32_exact = exact;
     NKPR_HIP(hipStreamSynchronize(s));
     // derived copies follow the coefficients
     g->b_coef_ready = false;

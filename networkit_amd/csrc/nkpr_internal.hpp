@@ -292,6 +292,7 @@ struct BinLayout {
     bool ready = false, coef_ready = false, off = false;
     int span_shift = 15;        // ids per source chunk = 2^span_shift
     bool fp64 = false;          // FP64 coefficients and contributions
+    bool w32 = false;           // FP64 copy: 4-byte weights (x / deg staged in the table)
     uint64_t mp = 0;            // padded edges
     DevBuf order, pos, src, dst, coef, val, p1, p2, mb, part, ctl, partials;
     double insum = 0.0;         // largest in-coefficient sum of a row (fixed-point bound)
@@ -315,6 +316,8 @@ struct nkpr_graph {
     nkpr::DevBuf coef64;    // double [m + kCap]  (weighted)
     nkpr::DevBuf inv32;     // float [n]   (unweighted)
     nkpr::DevBuf inv64;     // double [n]  (unweighted)
+    nkpr::DevBuf invw64;    // double [n]  1 / weighted out-degree (weighted)
+    bool w32_exact = false; // every weight is an FP32 number (weighted)
     nkpr::DevBuf emap;      // uint32 [m]  input position of every edge in sweep order (weighted)
     std::vector<uint8_t> exists_host;  // empty: every slot exists
 

@@ -413,18 +413,21 @@ def test_mixed_binned_operator_edge_cases(monkeypatch, case):
     assert np.abs(x[ex] - ref[ex]).sum() <= 1e-10
 
 
-@pytest.mark.parametrize("weighted", [True, False])
+@pytest.mark.parametrize("weights", ["fp64", "fp32", None])
 @pytest.mark.parametrize("sinks", [0, 1])
 @pytest.mark.parametrize("normalized", [False, True])
 @pytest.mark.parametrize("item", ["0", "2000"])
-def test_fp64_binned_sweep_parity(monkeypatch, weighted, sinks, normalized, item):
+def test_fp64_binned_sweep_parity(monkeypatch, weights, sinks, normalized, item):
     """The reference FP64 iteration with every sweep through the binned FP64 passes (forced on
     a graph of several source chunks and destination bins; item=2000 splits bins over work
     items, finished by the reduce kernel): the oracle's iteration count, L1 difference
-    <= 1e-12, maximum() to 1e-12 and the same ranking, with deleted nodes."""
+    <= 1e-12, maximum() to 1e-12 and the same ranking, with deleted nodes.  Weights that are
+    all FP32 numbers take the copy with 4-byte weights (x / deg staged in the table)."""
     monkeypatch.setenv("NKPR_BINNED64", "1")
     monkeypatch.setenv("NKPR_BINNED_ITEM", item)
-    z, u, v, w = skewed_graph(50000, 400000, 65, weighted=weighted)
+    z, u, v, w = skewed_graph(50000, 400000, 65, weighted=weights is not None)
+    if weights == "fp32":
+        w = w.astype(np.float32).astype(np.float64)
     check_fp64_parity(z, u, v, w, True, remove=(3, 4000, 49999), sinks=sinks,
                       normalized=normalized, tol=1e-10)
 
