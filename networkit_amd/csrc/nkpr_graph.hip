@@ -180,11 +180,15 @@ __global__ void inv_degree_kernel(const int32_t *perm, uint32_t n, const uint32_
     if (inv32) inv32[r] = (float)v;
 }
 
-// flag = 1 when some FP64 weight is not the FP32 weight given beside it
+// flag = 1 when some FP64 weight is not the FP32 weight given beside it.  Each thread keeps its
+// own verdict over a grid-stride loop; one atomic per block at most (the grid is capped by the
+// launcher), so a graph whose weights all differ does not serialise on one address.
 __global__ void fp32_exact_kernel(const float *w, const double *w64, uint64_t m, uint32_t *flag) {
+    int diff = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
          i += (uint64_t)gridDim.x * blockDim.x)
-        if ((double)w[i] != w64[i]) atomicOr(flag, 1u);
+        diff |= (double)w[i] != w64[i];
+    if (__syncthreads_or(diff) && threadIdx.x == 0) atomicOr(flag, 1u);
 }
 
 // invw64[u] = 1 / deg[u] (compact order; 0 for nodes without out-weight)
@@ -774,7 +778,7 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
         g->w32_exact = true;
         if (w64_in && m) {
             NKPR_HIP(hipMemsetAsync(flags_b.p, 0, 4, s));
-            hipLaunchKernelGGL(fp32_exact_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)), dim3(256), 0, s,
+            hipLaunchKernelGGL(fp32_exact_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 2048)), dim3(256), 0, s,
                                w_in, w64_in, m, flags_b.as<uint32_t>());
             NKPR_HIP(hipGetLastError());
             uint32_t hf = 0;
@@ -882,7 +886,7 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
         DevBuf flag;
         flag.alloc(4);
         NKPR_HIP(hipMemsetAsync(flag.p, 0, 4, s));
-        hipLaunchKernelGGL(fp32_exact_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)), dim3(256), 0, s,
+        hipLaunchKernelGGL(fp32_exact_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 2048)), dim3(256), 0, s,
                            w, w64, m, flag.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
         uint32_t hf = 0;
