@@ -414,7 +414,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
             launch_init_ctl(a->ctl.as<PhaseCtl>(), nullptr, a->claim.as<uint32_t>(), s);
             SweepArgs<double> sa = sweep_args();
             sa.base_vec = a->base64.as<double>();
-            sa.r32 = a->r32.as<float>();
+            sa.r32 = kr.V;  // the residual is the first Krylov vector
             PhaseRule rr{};
             rr.kind = kRuleStep;
             rr.norm = NKPR_L1;
@@ -435,7 +435,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
             ++st->refinements;
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / l1));
             const uint64_t left = p->max_iterations > total + 1 ? p->max_iterations - total - 1 : 0;
-            const uint64_t kc = gmres_f32(kr, apply_b, a->r32.as<float>(), kr.e, rtol, left, s);
+            const uint64_t kc = gmres_f32(kr, apply_b, kr.V, kr.e, rtol, left, s);
             st->sweeps_fp32 += kc;
             st->sweeps_fp32_correction += kc;
             total += kc;

@@ -82,10 +82,11 @@ struct BinCtl {
 };
 
 // The fixed-point scale is 2^e with every row sum below 2^61 units:
-// |sum| <= max|x| * max_u sum_in coef(u) = bound.
+// |sum| <= max|x| * max_u sum_in coef(u) = bound.  e stays within [-126, 126] so that 2^e is a
+// normal FP32 number (a smaller e only keeps fewer fraction bits of very small sums).
 __device__ __forceinline__ int fixed_exponent(const BinCtl *ctl, double insum) {
     const double bound = __longlong_as_double((long long)ctl->vmax) * insum;
-    return bound > 0.0 ? 60 - ilogb(bound) : 0;
+    return bound > 0.0 ? max(-126, min(126, 60 - ilogb(bound))) : 0;
 }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));

@@ -208,7 +208,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             rr.max_iterations = 1;
             rr.damp = damp;
             SweepArgs<double> a64 = base_args64();
-            a64.r32 = g->r32.as<float>();
+            a64.r32 = kr.V;  // the residual is the first Krylov vector
             Vecs<double> vr{{xcur, ynew}, {xh, g->xh64b.as<double>()}};
             const uint64_t k = run_phase<double>(rc, a64, vr, kind_resid, rr, 1, timing, &st->ms_sweeps_fp64);
             st->sweeps_fp64 += k;
@@ -226,7 +226,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
             const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
-            const uint64_t kc = gmres_f32(kr, apply_b, g->r32.as<float>(), kr.e, rtol, left, s);
+            const uint64_t kc = gmres_f32(kr, apply_b, kr.V, kr.e, rtol, left, s);
             if (std::getenv("NKPR_DEBUG"))
                 std::fprintf(stderr, "[nkpr] refinement %d: residual l1 %.3e, bound %.3e, rtol %.3e, %llu applications\n",
                              round, c.l1, c.bound, rtol, (unsigned long long)kc);

@@ -416,7 +416,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                 if (!folded) launch_prescale(xcur, g->inv64.as<double>(), g->xh64a.as<double>(), nullptr, nullptr, nullptr, n, n, s);
                 launch_init_ctl(ctl, nullptr, g->claim.as<uint32_t>(), s);
                 SweepArgs<double> a64 = args64();
-                a64.r32 = g->r32.as<float>();
+                a64.r32 = kr.V;  // the residual is the first Krylov vector
                 Vecs<double> vr{{xcur, ynew}, {g->xh64a.as<double>(), g->xh64b.as<double>()}};
                 std::function<void(const double *, double *)> post = [&](const double *xo, double *xn) {
                     post_sweep<double>(q, xo, xn, cr, 0, nullptr);
@@ -439,7 +439,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                 const double target = 0.5 * p->tol * min_sum * (1.0 - damp) / (2.0 * damp);
                 const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(l1, 1e-300)));
                 const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;
-                const uint64_t kc = gmres_f32(kr, apply_b, g->r32.as<float>(), kr.e, rtol, left, s);
+                const uint64_t kc = gmres_f32(kr, apply_b, kr.V, kr.e, rtol, left, s);
                 st->sweeps_fp32 += kc;
                 st->sweeps_fp32_correction += kc;
                 total += kc;

@@ -176,9 +176,19 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
 // the same, so the Arnoldi process runs on B, B V_j = V_{j+1} H_B, and the least-squares
 // problem uses A V_j = V_{j+1} (I~ - H_B).  On B the Gram-Schmidt pass cancels little: A v_j
 // is mostly v_j itself (B is a contraction), so Arnoldi on A would cancel most of every new
-// vector and need the second pass each step.  Classical Gram-Schmidt with selective
-// reorthogonalisation; the Hessenberg least-squares problem is solved on the host with Givens
-// rotations.  Returns the number of applications.
+// vector and need the second pass each step.
+//
+// The basis is kept unnormalised: slot i holds u_i and the orthonormal vector is
+// v_i = sig[i] u_i, with the scales on the host (no normalisation pass over the vector).  A
+// step is one operator application, one multi-dot <u_i, w>, <w, w> (the only host read), and
+// one projection.  The projected norm comes from ||w'||^2 = ||w||^2 - sum_i <v_i, w>^2
+// (accumulated in FP64 from the stored FP32 vectors); when that cancels to eta^2 ||w||^2 or
+// less, the projection is repeated with measured dots and norms (selective reorthogonalisation,
+// the criterion of Daniel, Gragg, Kaufman and Stewart, eta = 0.1).  A vector whose norm leaves
+// [2^-50, 2^50] is rescaled on the device, so the FP32 entries stay far from the ends of the
+// range.  `rhs` may be basis slot 0 itself; it is overwritten then.  The Hessenberg
+// least-squares problem is solved on the host with Givens rotations.  Returns the number of
+// applications.
 template <typename Apply>
 uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, double rtol,
                    uint64_t cap, hipStream_t s) {
@@ -191,12 +201,13 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * cnt, hipMemcpyDeviceToHost, s));
         NKPR_HIP(hipStreamSynchronize(s));
     };
-    // w -= V c (cnt vectors), then host[i] = <V[i], w> for i < cnt and host[cnt] = <w, w>
+    // w -= V c (cnt vectors), then host[i] = <u_i, w> for i < cnt and host[cnt] = <w, w>
     auto project = [&](float *w, uint32_t cnt, const double *c, double *host) {
         launch_axpy_dot(w, V, ld, cnt, n, c, k.dpart, k.dout, s);
         NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * (cnt + 1), hipMemcpyDeviceToHost, s));
         NKPR_HIP(hipStreamSynchronize(s));
     };
+    const auto in_range = [](double nrm) { return nrm >= 0x1p-50 && nrm <= 0x1p50; };
     double beta2 = 0.0;
     dots(rhs, rhs, 1, &beta2);
     const double beta = std::sqrt(beta2);
@@ -204,7 +215,12 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         launch_axpby(rhs, nullptr, e, n, 0.0f, 0.0f, s);
         return 0;
     }
-    launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
+    std::vector<double> sig(mres + 1, 1.0);
+    if (rhs == V && in_range(beta)) {
+        sig[0] = 1.0 / beta;
+    } else {
+        launch_axpby(rhs, nullptr, V, n, (float)(1.0 / beta), 0.0f, s);
+    }
     std::vector<double> R((size_t)mres * mres, 0.0), cs(mres), sn(mres), gv(mres + 1, 0.0);
     gv[0] = beta;
     const bool debug = std::getenv("NKPR_DEBUG") != nullptr;
@@ -213,33 +229,47 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
     if (const char *ev = std::getenv("NKPR_GMRES_REORTH")) reorth_eta2 = std::atof(ev);
     uint64_t applies = 0;
     int kk = 0;
+    std::vector<double> h(mres + 1), d(mres + 2), c(mres + 1);
     for (int j = 0; j < mres && applies < cap; ++j) {
-        float *vj = V + (size_t)j * ld;
+        float *uj = V + (size_t)j * ld;
         float *w = V + (size_t)(j + 1) * ld;
-        apply(vj, w);
+        apply(uj, w);  // w = B u_j, so B v_j = sig[j] w
         ++applies;
-        // one classical Gram-Schmidt pass; w is stored as basis vector j+1, so the same
-        // multi-dot also returns w.w.  The projection kernel also returns the dots of the
-        // projected w with the basis and its squared norm, which a second pass needs: that
-        // pass runs only when the first cancelled all but a tenth of w's norm (selective
-        // reorthogonalisation, the criterion of Daniel, Gragg, Kaufman and Stewart with
-        // eta = 0.1; the FP64 residual sweeps check every correction anyway).
-        std::vector<double> h(j + 2), h2(j + 2), h3(j + 2);
-        dots(w, V, j + 2, h.data());
-        const double ww = h[j + 1];
-        project(w, j + 1, h.data(), h2.data());
-        double hn2 = h2[j + 1];
-        const bool reorth = !(hn2 > reorth_eta2 * ww);
-        if (reorth) {
-            project(w, j + 1, h2.data(), h3.data());
-            for (int i = 0; i <= j; ++i) h[i] += h2[i];
-            hn2 = h3[j + 1];
+        // <u_i, w> for i <= j and <w, w> (w is slot j + 1)
+        dots(w, V, j + 2, d.data());
+        const double ww = d[j + 1];
+        double proj2 = 0.0;
+        for (int i = 0; i <= j; ++i) {
+            h[i] = sig[i] * d[i];  // <v_i, w>
+            c[i] = sig[i] * h[i];  // w -= sum_i <v_i, w> v_i = sum_i c_i u_i
+            proj2 += h[i] * h[i];
         }
-        const double hn = std::sqrt(std::max(hn2, 0.0));
-        // column j of the Hessenberg matrix of A: I~ - H_B
-        for (int i = 0; i <= j; ++i) h[i] = -h[i];
+        double hn2 = ww - proj2;
+        const bool reorth = !(hn2 > reorth_eta2 * ww);
+        if (!reorth) {
+            launch_multiaxpy(w, V, ld, j + 1, n, c.data(), false, s);
+        } else {
+            // both passes with measured dots and norms
+            project(w, j + 1, c.data(), d.data());
+            for (int i = 0; i <= j; ++i) {
+                const double h2 = sig[i] * d[i];
+                h[i] += h2;
+                c[i] = sig[i] * h2;
+            }
+            project(w, j + 1, c.data(), d.data());
+            hn2 = d[j + 1];
+        }
+        double hn = std::sqrt(std::max(hn2, 0.0));
+        // u_{j+1} = w (projected), v_{j+1} = w / hn
+        sig[j + 1] = hn > 0.0 ? 1.0 / hn : 1.0;
+        if (hn > 0.0 && !in_range(hn)) {
+            launch_axpby(w, nullptr, w, n, (float)(1.0 / hn), 0.0f, s);
+            sig[j + 1] = 1.0;
+        }
+        // column j of the Hessenberg matrix of A: I~ - H_B, with H_B's column sig[j] (h, hn)
+        for (int i = 0; i <= j; ++i) h[i] = -sig[j] * h[i];
         h[j] += 1.0;
-        h[j + 1] = -hn;
+        h[j + 1] = -sig[j] * hn;
         for (int i = 0; i < j; ++i) {
             const double t = cs[i] * h[i] + sn[i] * h[i + 1];
             h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
@@ -253,7 +283,6 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         gv[j] = cs[j] * gv[j];
         for (int i = 0; i <= j; ++i) R[(size_t)i * mres + j] = h[i];
         kk = j + 1;
-        if (hn > 0.0) launch_axpby(w, nullptr, w, n, (float)(1.0 / hn), 0.0f, s);
         if (debug)
             std::fprintf(stderr, "[nkpr]   gmres %d: relative residual %.3e%s\n", j,
                          std::fabs(gv[j + 1]) / beta, reorth ? " (reorthogonalised)" : "");
@@ -265,6 +294,7 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         for (int q = i + 1; q < kk; ++q) t -= R[(size_t)i * mres + q] * y[q];
         y[i] = R[(size_t)i * mres + i] != 0.0 ? t / R[(size_t)i * mres + i] : 0.0;
     }
+    for (int i = 0; i < kk; ++i) y[i] *= sig[i];  // e = sum_i y_i v_i = sum_i (y_i sig_i) u_i
     launch_multiaxpy(e, V, ld, kk, n, y.data(), true, s);
     return applies;
 }

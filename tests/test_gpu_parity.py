@@ -345,8 +345,13 @@ def test_mixed_binned_operator(monkeypatch, weighted, sinks, item):
         assert st["l1_error_bound"] <= 1e-9
         assert np.abs(x - ref).sum() <= 1e-9
         stats[binned] = st
-    assert stats["1"]["refinements"] == stats["0"]["refinements"] <= 4
-    assert abs(int(stats["1"]["sweeps_fp32"]) - int(stats["0"]["sweeps_fp32"])) <= 1
+    # GMRES stops on the L2 norm of its residual while the certificate is in L1, so a
+    # refinement that ends close to its target can leave one of the two solves just short of
+    # the bound (one more refinement of a few applications); a wrong operator costs many.
+    r0, r1 = int(stats["0"]["refinements"]), int(stats["1"]["refinements"])
+    assert abs(r1 - r0) <= 1 and max(r0, r1) <= 4
+    slack = 1 if r0 == r1 else 8
+    assert abs(int(stats["1"]["sweeps_fp32"]) - int(stats["0"]["sweeps_fp32"])) <= slack
 
 
 
