@@ -59,15 +59,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
-    ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "adjoint"],
+    ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "graphs", "adjoint"],
                     default="pagerank",
                     help="pagerank: one solve on R-MAT (the benchmark line); rhg: the same on "
                          "an undirected random hyperbolic graph; ppr: batched personalised "
-                         "PageRank on a Barabasi-Albert graph; adjoint: forward and backward "
+                         "PageRank on a Barabasi-Albert graph; graphs: one batched run over 512 "
+                         "Erdos-Renyi graphs of uneven size; adjoint: forward and backward "
                          "of differentiable PageRank on R-MAT scale 20")
     ap.add_argument("--rhg-nodes", type=int, default=10_000_000)
     ap.add_argument("--ppr-nodes", type=int, default=2_000_000)
     ap.add_argument("--ppr-columns", type=int, default=256)
+    ap.add_argument("--graphs", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -75,6 +77,9 @@ def main():
         sys.exit(2)
     if args.workload == "ppr":
         bench_ppr(args)
+        return
+    if args.workload == "graphs":
+        bench_graphs(args)
         return
     if args.workload == "rhg":
         args.no_cpu = True
@@ -369,6 +374,126 @@ def bench_ppr(args):
         "fp64": {"ms_per_solve": sec_d * 1e3, "sweeps": d0["sweeps_fp64"], "max_l1_error": err_d,
                  "ms_per_sweep": ms64, "achieved_GBs": ach64,
                  "tile_columns": d0["tile_columns"]},
+        "speedup_vs_fp64": sec_d / sec_m,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_graphs(args):
+    """A batch of graphs in one handle: G undirected Erdos-Renyi graphs G(n_i, p_i), n_i ~
+    U{1000..200000}, p_i = 12 / (n_i - 1) (average degree 12), seed 4 (sizes and per-graph
+    generator seeds drawn from numpy's default_rng(4)); one nkpr_run_batch to a certified L1
+    error <= tol for every graph, mixed next to plain FP64.  Edge-updates count every arc of
+    the union once per sweep."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    G = args.graphs
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(1000, 200001, size=G)
+    gseeds = rng.integers(1, 1 << 62, size=G)
+    t0 = time.time()
+
+    def make(i):
+        n = int(sizes[i])
+        return gen.erdos_renyi(n, 12.0 / (n - 1), seed=int(gseeds[i]), directed=False)
+
+    with ThreadPoolExecutor(16) as pool:
+        edges = list(pool.map(make, range(G)))
+    offs = np.zeros(G + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(sizes)
+    z = int(offs[-1])
+    # undirected in-lists of the union (edge (u, v) adds v to row u, then u to row v), sorted
+    # stably by row on the GPU
+    tu = torch.cat([torch.from_numpy((u + offs[i]).astype(np.int32)) for i, (u, _) in enumerate(edges)]).to(dev)
+    tv = torch.cat([torch.from_numpy((v + offs[i]).astype(np.int32)) for i, (_, v) in enumerate(edges)]).to(dev)
+    del edges
+    rows = torch.stack([tu, tv], dim=1).reshape(-1)
+    srcs = torch.stack([tv, tu], dim=1).reshape(-1)
+    del tu, tv
+    order = torch.argsort(rows, stable=True)
+    colptr = torch.zeros(z + 1, dtype=torch.int64, device=dev)
+    colptr[1:] = torch.cumsum(torch.bincount(rows, minlength=z), 0)
+    src = srcs[order].contiguous()
+    del rows, srcs, order
+    m = int(colptr[-1])
+    torch.cuda.synchronize()
+    log("%d Erdos-Renyi graphs: z=%d m=%d arcs, built in %.1f s" % (G, z, m, time.time() - t0))
+    t0 = time.time()
+    dg = capi.DeviceGraph(z, colptr, src, device=True, graph_offsets=offs.astype(np.uint64))
+    del colptr, src
+    torch.cuda.synchronize()
+    log("batch handle in %.1f s" % (time.time() - t0))
+    stream = torch.cuda.ExternalStream(dg.stream())
+    x = torch.empty(z, dtype=torch.float64, device=dev)
+    start = torch.repeat_interleave(torch.from_numpy(1.0 / sizes.astype(np.float64)).to(dev),
+                                    torch.from_numpy(sizes).to(dev))
+    seg = torch.repeat_interleave(torch.arange(G, device=dev), torch.from_numpy(sizes).to(dev))
+
+    def solve(precision, tol):
+        torch.cuda.current_stream().synchronize()
+        with torch.cuda.stream(stream):
+            x.copy_(start)
+        p = capi.default_params(scores_on_device=1, precision=precision, tol=tol,
+                                stop=capi.NKPR_STOP_L1_ERROR, timing=1)
+        st, its, mx = dg.run_batch(x, params=p)
+        return st
+
+    solve(capi.NKPR_FP64, 1e-13)
+    torch.cuda.synchronize()
+    ref = x.clone()
+
+    def timed(precision):
+        for _ in range(args.warmup):
+            solve(precision, args.tol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sts = [solve(precision, args.tol) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t) / args.steps
+        err = torch.zeros(G, dtype=torch.float64, device=dev).index_add_(0, seg, (x - ref).abs())
+        return sec, sts, float(err.max())
+
+    sec_m, st_m, err_m = timed(capi.NKPR_MIXED)
+    sec_d, st_d, err_d = timed(capi.NKPR_FP64)
+    s0, d0 = st_m[0], st_d[0]
+    b32 = algorithmic_bytes(m, z, False, "fp32")
+    b64 = algorithmic_bytes(m, z, False, "fp64")
+    ms32 = sum(st["ms_sweeps_fp32"] for st in st_m) / max(1, sum(st["sweeps_fp32"] for st in st_m))
+    ms64 = sum(st["ms_sweeps_fp64"] for st in st_d) / max(1, sum(st["sweeps_fp64"] for st in st_d))
+    ach32 = b32 / (ms32 / 1e3) / 1e9
+    ach64 = b64 / (ms64 / 1e3) / 1e9
+    log("mixed: %.1f ms, %d fp32 + %d fp64 sweeps, %d refinements, max L1 err %.2e, bound %.2e; "
+        "fp32 sweep %.3f ms = %.0f GB/s" % (sec_m * 1e3, s0["sweeps_fp32"], s0["sweeps_fp64"],
+                                             s0["refinements"], err_m, s0["l1_error_bound"], ms32, ach32))
+    log("fp64: %.1f ms, %d sweeps, max L1 err %.2e; fp64 sweep %.3f ms = %.0f GB/s" % (
+        sec_d * 1e3, d0["sweeps_fp64"], err_d, ms64, ach64))
+    sweeps_m = s0["sweeps_fp32"] + s0["sweeps_fp64"]
+    out = {
+        "metric": "PageRank edge-updates/s on a batch of graphs, mixed vs FP64 at equal L1 error",
+        "value": m * sweeps_m / sec_m, "unit": "edge-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec_m * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "one batched PageRank run over %d undirected Erdos-Renyi graphs, "
+                               "n ~ U{1000..200000}, average degree 12, seed 4, certified L1 "
+                               "error <= %g per graph (damp 0.85, no sink handling)"
+                               % (G, args.tol), "graphs": G, "z": z, "m": m},
+        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "FP32 pull sweep (GMRES operator)", "bytes_per_sweep": b32,
+                     "ms_per_sweep": ms32},
+        "mixed": {"ms_per_solve": sec_m * 1e3, "sweeps_fp32": s0["sweeps_fp32"],
+                  "sweeps_fp64": s0["sweeps_fp64"], "refinements": s0["refinements"],
+                  "max_l1_error": err_m, "l1_error_bound": s0["l1_error_bound"]},
+        "fp64": {"ms_per_solve": sec_d * 1e3, "sweeps": d0["sweeps_fp64"], "max_l1_error": err_d,
+                 "ms_per_sweep": ms64, "achieved_GBs": ach64},
         "speedup_vs_fp64": sec_d / sec_m,
     }
     print(json.dumps(out), flush=True)

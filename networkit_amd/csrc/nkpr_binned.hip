@@ -1009,7 +1009,7 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
 }
 
 bool binned64_active(nkpr_graph *g) {
-    if (g->bn64.off || g->n_groups != 0 || !policy(g, "NKPR_BINNED64")) return false;
+    if (g->bn64.off || !policy(g, "NKPR_BINNED64")) return false;
     g->bn64.span_shift = kSpan64Shift;
     g->bn64.fp64 = true;
     return layout_ready(g, g->bn64);

@@ -263,6 +263,27 @@ Krylov krylov_setup(nkpr_graph *g) {
     return k;
 }
 
+// w = B v (+ d S(v)/n when `distribute`) or w = A v = v - B v (minus) through the binned copy,
+// when the graph uses one (false otherwise: nothing launched).
+bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
+                  bool minus, bool timing, double *ms_acc) {
+    if (!(binned_enabled(g) && binned_layout(g))) return false;
+    while (timing && g->events.size() < 2) {
+        hipEvent_t e;
+        NKPR_HIP(hipEventCreate(&e));
+        g->events.push_back(e);
+    }
+    binned_apply(g, v, w, damp, distribute, minus, timing ? g->events[0] : nullptr,
+                 timing ? g->events[1] : nullptr);
+    if (timing) {
+        float ms = 0.f;
+        NKPR_HIP(hipEventSynchronize(g->events[1]));
+        NKPR_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
+        *ms_acc += ms;
+    }
+    return true;
+}
+
 // w = A v = v - B v (minus) or w = B v, B v = d P v: one FP32 sweep with no teleport term
 // (sinks per the run's rule).
 void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
@@ -272,22 +293,7 @@ void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool
     const uint32_t n = g->n;
     const bool folded = g->weighted;
     const bool dist = rc.p->sink_handling == NKPR_SINK_DISTRIBUTE;
-    if (binned_enabled(g) && binned_layout(g)) {
-        while (timing && g->events.size() < 2) {
-            hipEvent_t e;
-            NKPR_HIP(hipEventCreate(&e));
-            g->events.push_back(e);
-        }
-        binned_apply(g, v, w, rc.p->damp, dist, minus, timing ? g->events[0] : nullptr,
-                     timing ? g->events[1] : nullptr);
-        if (timing) {
-            float ms = 0.f;
-            NKPR_HIP(hipEventSynchronize(g->events[1]));
-            NKPR_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
-            *ms_acc += ms;
-        }
-        return;
-    }
+    if (apply_binned(g, v, w, rc.p->damp, dist, minus, timing, ms_acc)) return;
     if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kr.vh, n, g->sink_begin, s);
     const double *sink_sum = nullptr;
     if (dist && g->sink_begin < n) {

@@ -385,13 +385,16 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
             // A v = v - d P v - (per graph) d S_q(v) / n_q
             // w = B v = d P v (+ per-graph sink terms), the GMRES Krylov operator (A = I - B)
             auto apply_b = [&](const float *v, float *w) {
-                if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kr.vh, n, n, s);
-                launch_init_ctl(ctl, nullptr, g->claim.as<uint32_t>(), s);
-                SweepArgs<float> a = args32();
-                a.base_vec = nullptr;
-                a.tele = 0.0;
-                Vecs<float> vv{{const_cast<float *>(v), dist ? kr.y : w}, {kr.vh, kr.yh}};
-                run_phase<float>(rc, a, vv, kSweepPlain, counting_rule(1), 1, timing, &st->ms_sweeps_fp32);
+                // d P v through the binned copy of the union on large batches, else one pull sweep
+                if (!apply_binned(g, v, dist ? kr.y : w, damp, false, false, timing, &st->ms_sweeps_fp32)) {
+                    if (!folded) launch_prescale(nullptr, nullptr, nullptr, v, g->inv32.as<float>(), kr.vh, n, n, s);
+                    launch_init_ctl(ctl, nullptr, g->claim.as<uint32_t>(), s);
+                    SweepArgs<float> a = args32();
+                    a.base_vec = nullptr;
+                    a.tele = 0.0;
+                    Vecs<float> vv{{const_cast<float *>(v), dist ? kr.y : w}, {kr.vh, kr.yh}};
+                    run_phase<float>(rc, a, vv, kSweepPlain, counting_rule(1), 1, timing, &st->ms_sweeps_fp32);
+                }
                 if (dist) {
                     PhaseRule r0{};
                     r0.damp = damp;
@@ -426,18 +429,19 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                 result = ynew;
                 NKPR_HIP(hipMemcpyAsync(hc.data(), q.gc, sizeof(GroupCtl) * G, hipMemcpyDeviceToHost, s));
                 NKPR_HIP(hipStreamSynchronize(s));
-                double l1 = 0.0, min_sum = INFINITY;
+                // the correction must bring every graph's residual to its own target: the ratio
+                // of the worst graph sets the relative tolerance of the union's solve
+                double worst = 0.0;
                 uint32_t open = 0;
                 for (uint32_t k = 0; k < G; ++k) {
                     if (g->goff[k + 1] == g->goff[k]) continue;
-                    l1 += hc[k].l1;
-                    min_sum = std::min(min_sum, hc[k].sum);
+                    const double target = 0.5 * p->tol * hc[k].sum * (1.0 - damp) / (2.0 * damp);
+                    worst = std::max(worst, hc[k].l1 / std::max(target, 1e-300));
                     open += hc[k].bound <= p->tol ? 0u : 1u;
                 }
                 if (open == 0 || total >= max_it || round >= kMaxRefinements) break;
                 ++st->refinements;
-                const double target = 0.5 * p->tol * min_sum * (1.0 - damp) / (2.0 * damp);
-                const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(l1, 1e-300)));
+                const double rtol = std::min(0.5, std::max(1e-7, 0.5 / std::max(worst, 1e-300)));
                 const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;
                 const uint64_t kc = gmres_f32(kr, apply_b, kr.V, kr.e, rtol, left, s);
                 st->sweeps_fp32 += kc;

@@ -84,9 +84,9 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
     a.partials = g->partials.as<double>();
     a.ctl = g->ctl.as<PhaseCtl>();
 
-    // FP64 sweeps of large graphs run through the binned copy (a handle with per-graph hooks
-    // keeps the pull sweep)
-    const bool binned64 = dbl && !post && binned64_active(g);
+    // FP64 sweeps of large graphs run through the binned copy (per-graph hooks of a batch
+    // handle run after either sweep)
+    const bool binned64 = dbl && binned64_active(g);
     uint64_t launched = 0, done_sweeps = 0;
     uint64_t batch = (max_launch <= 2) ? max_launch : 4;
     const size_t need_ev = 2 * 64;
@@ -170,6 +170,8 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
 Krylov krylov_setup(nkpr_graph *g);
 void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
                     double *ms_acc, bool minus);
+bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
+                  bool minus, bool timing, double *ms_acc);
 
 // GMRES(m) from e = 0 on A e = rhs, A = I - B, until ||A e - rhs||_2 <= rtol ||rhs||_2 or `cap`
 // operator applications.  `apply(v, w)` computes w = B v.  The Krylov spaces of A and B are

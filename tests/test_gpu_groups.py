@@ -121,6 +121,57 @@ def test_batch_graphs_fp32_certified():
         assert np.abs(x[offs[q]:offs[q + 1]][ex] - ref[ex]).sum() <= 1e-5 + 2e-6, q
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("sinks", [0, 1])
+def test_batch_graphs_binned_copies(monkeypatch, weighted, sinks):
+    """Both binned copies forced on a batch handle (the default on unions of 2^26+ edges): the
+    FP64 run keeps every graph's oracle iteration count and L1 <= 1e-12 with deleted slots and
+    both sink rules, and the mixed run (binned GMRES operator, binned residual sweeps)
+    certifies every graph."""
+    monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    graphs, offs, colptr, src, w, exists = make_batch(SPECS, weighted, 17)
+    x, st, its, mx = run_batch(offs, colptr, src, w, exists, norm=oracle.L1, sink_handling=sinks)
+    for q, g in enumerate(graphs):
+        ref, it, rmx = oracle.pagerank(g, norm=oracle.L1, sink_mode=sinks)
+        ex = np.ones(g.z, bool) if g.exists is None else g.exists.astype(bool)
+        assert its[q] == it, (q, its[q], it)
+        assert np.abs(x[offs[q]:offs[q + 1]][ex] - ref[ex]).sum() <= 1e-12, q
+        assert mx[q] == pytest.approx(rmx, rel=1e-12)
+    x, st, its, mx = run_batch(offs, colptr, src, w, exists, precision=capi.NKPR_MIXED,
+                               tol=1e-9, sink_handling=sinks)
+    assert st["l1_error_bound"] <= 1e-9
+    for q, g in enumerate(graphs):
+        ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=20000,
+                                    sink_mode=sinks)
+        ex = np.ones(g.z, bool) if g.exists is None else g.exists.astype(bool)
+        assert np.abs(x[offs[q]:offs[q + 1]][ex] - ref[ex]).sum() <= 1e-9, q
+
+
+@pytest.mark.parametrize("item", ["0", "3000"])
+def test_batch_graphs_binned_union_spans_chunks(monkeypatch, item):
+    """A union of 48 graphs over many source chunks and destination bins (item=3000 splits bins
+    over work items) through both forced binned copies: FP64 per-graph parity with the oracle
+    and a certified mixed run."""
+    monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    monkeypatch.setenv("NKPR_BINNED_ITEM", item)
+    rng = np.random.default_rng(12)
+    specs = [(int(n), int(n) * 6, True, ()) for n in rng.integers(5000, 40000, 48)]
+    graphs, offs, colptr, src, w, exists = make_batch(specs, True, 13)
+    x, st, its, mx = run_batch(offs, colptr, src, w, exists)
+    for q, g in enumerate(graphs):
+        ref, it, _ = oracle.pagerank(g)
+        assert its[q] == it, q
+        assert np.abs(x[offs[q]:offs[q + 1]] - ref).sum() <= 1e-12, q
+    x, st, its, mx = run_batch(offs, colptr, src, w, exists, precision=capi.NKPR_MIXED,
+                               tol=1e-9)
+    assert st["l1_error_bound"] <= 1e-9
+    for q, g in enumerate(graphs):
+        ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=20000)
+        assert np.abs(x[offs[q]:offs[q + 1]] - ref).sum() <= 1e-9, q
+
+
 def test_batch_graphs_invalid():
     graphs, offs, colptr, src, w, exists = make_batch(SPECS[:3], False, 16)
     bad = src.copy()
