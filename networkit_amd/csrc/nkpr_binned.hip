@@ -275,6 +275,55 @@ __device__ __forceinline__ void note_vmax(BinCtl *ctl, double vmax) {
     if ((threadIdx.x & 63) == 0) atomicMax(&ctl->vmax, (unsigned long long)__double_as_longlong(vmax));
 }
 
+// ---- pass-1 tables --------------------------------------------------------------------------
+
+// tab[i] = v[base + i] (* inv[base + i]) for i < kSpan (0 past n), and vmax = max |v|.  Every
+// load of a thread is issued before its first LDS store, so a work item waits for one memory
+// round trip here instead of one per element: 16-byte loads when the whole chunk lies below n
+// and both vectors are 16-byte aligned, guarded element loads otherwise.
+template <typename T, uint32_t kSpan>
+__device__ __forceinline__ void fill_table(T *tab, const T *v, const T *inv, uint32_t base,
+                                           uint32_t n, T &vmax) {
+    typedef T t2 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr uint32_t kLanes = 16 / sizeof(T);
+    constexpr uint32_t kVec = kSpan / kLanes / kPassThreads;  // 16-byte loads per thread
+    constexpr uint32_t kPer = kSpan / kPassThreads;           // elements per thread
+    static_assert(kVec * kLanes * kPassThreads == kSpan, "table fill covers the chunk");
+    const bool aligned = (((uintptr_t)v | (uintptr_t)inv) & 15) == 0;
+    if (base + kSpan <= n && aligned) {
+        const t2 *v2 = reinterpret_cast<const t2 *>(v + base);
+        const t2 *i2 = reinterpret_cast<const t2 *>(inv + base);
+        t2 a[kVec], b[kVec];
+#pragma unroll
+        for (uint32_t k = 0; k < kVec; ++k) a[k] = v2[threadIdx.x + k * kPassThreads];
+        if (inv) {
+#pragma unroll
+            for (uint32_t k = 0; k < kVec; ++k) b[k] = i2[threadIdx.x + k * kPassThreads];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kVec; ++k) {
+            t2 t = a[k];
+#pragma unroll
+            for (uint32_t l = 0; l < kLanes; ++l) vmax = fmax(vmax, fabs(a[k][l]));
+            if (inv) t = a[k] * b[k];
+            reinterpret_cast<t2 *>(tab)[threadIdx.x + k * kPassThreads] = t;
+        }
+    } else {
+        T a[kPer], b[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t u = base + threadIdx.x + k * kPassThreads;
+            a[k] = u < n ? v[u] : T(0);
+            b[k] = (inv && u < n) ? inv[u] : T(1);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            vmax = fmax(vmax, fabs(a[k]));
+            tab[threadIdx.x + k * kPassThreads] = inv ? a[k] * b[k] : a[k];
+        }
+    }
+}
+
 // ---- FP32 operator ------------------------------------------------------------------------
 
 // Pass 1: item = {chunk, first edge, end edge} (chunk-major, quad-aligned).  The contribution
@@ -284,23 +333,14 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
     const uint4 *items, uint32_t nitems, const float *v, const float *inv, uint32_t n,
     const uint16_t *bsrc, const float *bcoef, const uint32_t *pos, float *bval, BinCtl *ctl) {
     constexpr uint32_t kSpan = 1u << kSpan32Shift;
-    __shared__ float tab[kSpan + 1];
+    __shared__ __align__(16) float tab[kSpan + 1];
     __shared__ uint32_t slot;
     if (threadIdx.x == 0) tab[kSpan] = 0.f;  // read by the padding slots
     float vmax = 0.f;
     for (uint32_t it = next_item(&ctl->q1, &slot); it < nitems; it = next_item(&ctl->q1, &slot)) {
         const uint4 item = items[it];
         const uint32_t base = item.x << kSpan32Shift;
-        for (uint32_t i = threadIdx.x; i < kSpan; i += kPassThreads) {
-            const uint32_t u = base + i;
-            float x = 0.f;
-            if (u < n) {
-                const float vu = v[u];
-                vmax = fmaxf(vmax, fabsf(vu));
-                x = inv ? vu * inv[u] : vu;
-            }
-            tab[i] = x;
-        }
+        fill_table<float, kSpan>(tab, v, inv, base, n, vmax);
         __syncthreads();
         // thread t handles quads q0 + t + j * kPassThreads, so every load of a wave is
         // contiguous (8-byte source offsets, 16-byte coefficients, 4-byte places); kP1Quads
@@ -419,14 +459,33 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
         stream_quads(acc, bdst, bval, scale, q0 + wb + lane, q0 + we);
         __syncthreads();
         const uint32_t base = item.x << kBinShift;
-        for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) {
-            const uint32_t u = base + i;
-            if (u >= n) break;
-            if (item.w & kItemSole) {
+        if ((item.w & kItemSole) && minus) {
+            // v of all rows of this thread loaded first
+            constexpr uint32_t kRows = kBin / kPassThreads;
+            float vv[kRows];
+#pragma unroll
+            for (uint32_t k = 0; k < kRows; ++k) {
+                const uint32_t u = base + threadIdx.x + k * kPassThreads;
+                vv[k] = u < n ? v[u] : 0.f;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kRows; ++k) {
+                const uint32_t i = threadIdx.x + k * kPassThreads;
+                const uint32_t u = base + i;
+                if (u >= n) break;
                 const float y = (float)((double)(long long)acc[i] * unscale);
-                w[u] = minus ? v[u] - (d * y + st) : d * y + st;
-            } else {
-                part[(size_t)(item.w >> 2) * kBin + i] = (long long)acc[i];
+                w[u] = vv[k] - (d * y + st);
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) {
+                const uint32_t u = base + i;
+                if (u >= n) break;
+                if (item.w & kItemSole) {
+                    const float y = (float)((double)(long long)acc[i] * unscale);
+                    w[u] = d * y + st;
+                } else {
+                    part[(size_t)(item.w >> 2) * kBin + i] = (long long)acc[i];
+                }
             }
         }
         __syncthreads();
@@ -472,14 +531,29 @@ struct Epi64 {
     int lo_shift;            // fraction bits of the low accumulator word
 };
 
+// Inputs of row u's epilogue, loaded ahead of the row (xold, the right-hand side entry and
+// 1/deg where the prescaled copy is written).
+struct Row64 {
+    double xo, bv, iv;
+};
+__device__ __forceinline__ Row64 load_row64(const Epi64 &e, uint32_t u) {
+    Row64 r{0.0, 0.0, 0.0};
+    if (u < e.n) {
+        r.xo = e.xold[u];
+        if (e.base_vec) r.bv = e.base_vec[u];
+        if (e.xhat_new != nullptr && u < e.sink_begin) r.iv = e.inv[u];
+    }
+    return r;
+}
+
 // The finished row u from its accumulated product y (the pull sweep's epilogue_store).
 __device__ __forceinline__ void finish_row64(const Epi64 &e, uint32_t u, double y, double c,
-                                             double (&s)[4]) {
+                                             const Row64 &in, double (&s)[4]) {
     double x = e.damp * y + c;
-    if (e.base_vec) x += e.base_vec[u];
+    if (e.base_vec) x += in.bv;
     __builtin_nontemporal_store(x, e.xnew + u);
-    if (e.xhat_new != nullptr && u < e.sink_begin) __builtin_nontemporal_store(x * e.inv[u], e.xhat_new + u);
-    const double dif = x - e.xold[u];
+    if (e.xhat_new != nullptr && u < e.sink_begin) __builtin_nontemporal_store(x * in.iv, e.xhat_new + u);
+    const double dif = x - in.xo;
     if (e.r32) __builtin_nontemporal_store((float)dif, e.r32 + u);
     s[0] += fabs(dif);
     s[1] += dif * dif;
@@ -534,23 +608,14 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
     const PhaseCtl *pctl) {
     constexpr uint32_t kSpan = 1u << kSpan64Shift;
     if (pctl->done) return;
-    __shared__ double tab[kSpan + 1];
+    __shared__ __align__(16) double tab[kSpan + 1];
     __shared__ uint32_t slot;
     if (threadIdx.x == 0) tab[kSpan] = 0.0;  // read by the padding slots
     double vmax = 0.0;
     for (uint32_t it = next_item(&ctl->q1, &slot); it < nitems; it = next_item(&ctl->q1, &slot)) {
         const uint4 item = items[it];
         const uint32_t base = item.x << kSpan64Shift;
-        for (uint32_t i = threadIdx.x; i < kSpan; i += kPassThreads) {
-            const uint32_t u = base + i;
-            double t = 0.0;
-            if (u < n) {
-                const double xu = x[u];
-                vmax = fmax(vmax, fabs(xu));
-                t = inv ? xu * inv[u] : xu;
-            }
-            tab[i] = t;
-        }
+        fill_table<double, kSpan>(tab, x, inv, base, n, vmax);
         __syncthreads();
         // one quad per thread and step, the next step's loads issued before this step's stores
         const uint32_t q1 = item.z / kQuad;
@@ -630,12 +695,19 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather64_kernel(
         __syncthreads();
         const uint32_t base = item.x << kBinShift;
         if (item.w & kItemSole) {
+            // rows threadIdx.x + k * kPassThreads: all inputs loaded first, rows finished in order
+            constexpr uint32_t kRows = kBin / kPassThreads;
+            Row64 in[kRows];
+#pragma unroll
+            for (uint32_t k = 0; k < kRows; ++k) in[k] = load_row64(e, base + threadIdx.x + k * kPassThreads);
             double s[4] = {0.0, 0.0, 0.0, 0.0};
-            for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) {
+#pragma unroll
+            for (uint32_t k = 0; k < kRows; ++k) {
+                const uint32_t i = threadIdx.x + k * kPassThreads;
                 const uint32_t u = base + i;
                 if (u >= n) break;
                 const double y = ((double)(long long)acc_hi[i] + (double)(long long)acc_lo[i] * lunscale) * unscale;
-                finish_row64(e, u, y, c, s);
+                finish_row64(e, u, y, c, in[k], s);
             }
             block_partials<kPassThreads>(s, e.partials, item.w >> 2);
         } else {
@@ -673,7 +745,7 @@ __global__ __launch_bounds__(kReduce64Threads) void bin_reduce64_kernel(
             hi += p[0];
             lo += p[1];
         }
-        finish_row64(e, u, ((double)hi + (double)lo * lunscale) * unscale, c, s);
+        finish_row64(e, u, ((double)hi + (double)lo * lunscale) * unscale, c, load_row64(e, u), s);
     }
     block_partials<kReduce64Threads>(s, e.partials, b.w + j);
 }
