@@ -587,6 +587,24 @@ __device__ __forceinline__ double rhs_const64(const Epi64 &e) {
     return c;
 }
 
+// A contribution c in 6 bytes: head h = float(c) and tail q = rint((c - h) * 2^(165 - E)),
+// E the biased exponent of h (1 for zero and subnormal heads).  |c - h| <= ulp(h) / 2 =
+// 2^(E - 151), so |q| <= 2^14; c is recovered as h + q * 2^(E - 165) to within 2^(E - 166),
+// i.e. 2^-39 of |c| for a normal head.
+__device__ __forceinline__ double exp2_of(int e) {
+    return __longlong_as_double((long long)(1023 + e) << 52);
+}
+__device__ __forceinline__ int head_exponent(float h) {
+    return max(1, (int)((__float_as_uint(h) >> 23) & 0xFFu));
+}
+__device__ __forceinline__ void pack6(double c, float &h, short &q) {
+    h = (float)c;
+    q = (short)__double2int_rn((c - (double)h) * exp2_of(165 - head_exponent(h)));
+}
+__device__ __forceinline__ double unpack6(float h, short q) {
+    return (double)h + (double)q * exp2_of(head_exponent(h) - 165);
+}
+
 // four coefficients of an FP64 pass-1 quad, from FP64 coefficients or FP32 weights
 __device__ __forceinline__ void load_coef4(const double *c, uint32_t e, f64x2 &c0, f64x2 &c1) {
     c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(c + e));
@@ -601,11 +619,13 @@ __device__ __forceinline__ void load_coef4(const float *c, uint32_t e, f64x2 &c0
 // Pass 1 (FP64): as the FP32 pass with an FP64 table of 16384 sources and FP64 contributions.
 // inv scales the table (1/deg: unweighted graphs, and weighted ones whose copy keeps FP32
 // weights); bcoef is the per-edge factor (FP64 coefficient or FP32 weight) or nullptr.
-template <typename CT>
+// COMPACT: the contributions go to 6-byte places (FP32 heads at bval as float[mp], 16-bit
+// tails after them) instead of FP64 values.
+template <typename CT, bool COMPACT>
 __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
     const uint4 *items, uint32_t nitems, const double *x, const double *inv, uint32_t n,
-    const uint16_t *bsrc, const CT *bcoef, const uint32_t *pos, double *bval, BinCtl *ctl,
-    const PhaseCtl *pctl) {
+    const uint16_t *bsrc, const CT *bcoef, const uint32_t *pos, double *bval, uint64_t mp,
+    BinCtl *ctl, const PhaseCtl *pctl) {
     constexpr uint32_t kSpan = 1u << kSpan64Shift;
     if (pctl->done) return;
     __shared__ __align__(16) double tab[kSpan + 1];
@@ -643,8 +663,24 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
             f64x2 r1 = {tab[sp.y & 0xFFFFu], tab[sp.y >> 16]};
             r0 *= c0;
             r1 *= c1;
-            __builtin_nontemporal_store(r0, reinterpret_cast<f64x2 *>(bval + at));
-            __builtin_nontemporal_store(r1, reinterpret_cast<f64x2 *>(bval + at + 2));
+            if constexpr (COMPACT) {
+                float h0, h1, h2, h3;
+                short q0, q1, q2, q3;
+                pack6(r0.x, h0, q0);
+                pack6(r0.y, h1, q1);
+                pack6(r1.x, h2, q2);
+                pack6(r1.y, h3, q3);
+                const f32x4 h = {h0, h1, h2, h3};
+                float *heads = reinterpret_cast<float *>(bval);
+                unsigned short *tails = reinterpret_cast<unsigned short *>(heads + mp);
+                const u32x2 t = {(uint32_t)(unsigned short)q0 | ((uint32_t)(unsigned short)q1 << 16),
+                                 (uint32_t)(unsigned short)q2 | ((uint32_t)(unsigned short)q3 << 16)};
+                __builtin_nontemporal_store(h, reinterpret_cast<f32x4 *>(heads + at));
+                __builtin_nontemporal_store(t, reinterpret_cast<u32x2 *>(tails + at));
+            } else {
+                __builtin_nontemporal_store(r0, reinterpret_cast<f64x2 *>(bval + at));
+                __builtin_nontemporal_store(r1, reinterpret_cast<f64x2 *>(bval + at + 2));
+            }
             sp = spn;
             at = atn;
             c0 = c0n;
@@ -667,8 +703,9 @@ __device__ __forceinline__ void fx_add64(unsigned long long *hi, unsigned long l
 // Pass 2 (FP64): the bin's contributions into 128-bit accumulators; a bin held by one item is
 // finished here (the sweep epilogue, the block's reductions into partials slot w >> 2), a split
 // bin's items write both words to part slots of their own.
+template <bool COMPACT>
 __global__ __launch_bounds__(kPassThreads) void bin_gather64_kernel(
-    const uint4 *items, uint32_t nitems, const uint16_t *bdst, const double *bval,
+    const uint4 *items, uint32_t nitems, const uint16_t *bdst, const double *bval, uint64_t mp,
     long long *part, BinCtl *ctl, double insum, Epi64 e) {
     if (e.pctl->done) return;
     __shared__ unsigned long long acc_hi[kBin];
@@ -685,8 +722,18 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather64_kernel(
         __syncthreads();
         for (uint32_t q = item.y / kQuad + threadIdx.x; q < item.z / kQuad; q += kPassThreads) {
             const u32x2 dd = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bdst + q * kQuad));
-            const f64x2 a0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bval + q * kQuad));
-            const f64x2 a1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bval + q * kQuad + 2));
+            f64x2 a0, a1;
+            if constexpr (COMPACT) {
+                const float *heads = reinterpret_cast<const float *>(bval);
+                const unsigned short *tails = reinterpret_cast<const unsigned short *>(heads + mp);
+                const f32x4 h = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(heads + q * kQuad));
+                const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(tails + q * kQuad));
+                a0 = f64x2{unpack6(h.x, (short)(t.x & 0xFFFFu)), unpack6(h.y, (short)(t.x >> 16))};
+                a1 = f64x2{unpack6(h.z, (short)(t.y & 0xFFFFu)), unpack6(h.w, (short)(t.y >> 16))};
+            } else {
+                a0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bval + q * kQuad));
+                a1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(bval + q * kQuad + 2));
+            }
             fx_add64(acc_hi, acc_lo, dd.x & 0xFFFFu, a0.x * scale, lscale);
             fx_add64(acc_hi, acc_lo, dd.x >> 16, a0.y * scale, lscale);
             fx_add64(acc_hi, acc_lo, dd.y & 0xFFFFu, a1.x * scale, lscale);
@@ -1094,10 +1141,11 @@ void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArg
                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     BinLayout &L = g->bn64;
     const uint32_t n = g->n;
-    if (L.w32) persistent_grids(g, L, bin_scatter64_kernel<float>, bin_gather64_kernel);
-    else persistent_grids(g, L, bin_scatter64_kernel<double>, bin_gather64_kernel);
+    if (L.w32) persistent_grids(g, L, bin_scatter64_kernel<float, false>, bin_gather64_kernel<false>);
+    else persistent_grids(g, L, bin_scatter64_kernel<double, false>, bin_gather64_kernel<false>);
     const bool basevec = kind == kSweepBaseVec || kind == kSweepBaseResid;
     const bool resid = kind == kSweepResid || kind == kSweepBaseResid;
+    const bool compact = resid && a.compact64 != 0;
     Epi64 e{};
     e.xold = a.xold;
     e.xnew = a.xnew;
@@ -1122,19 +1170,26 @@ void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArg
     hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
     if (n) {
         const dim3 g1(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1));
-        if (L.w32)
-            hipLaunchKernelGGL(bin_scatter64_kernel<float>, g1, dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1,
-                               a.xold, (const double *)g->invw64.as<double>(), n, L.src.as<uint16_t>(),
-                               (const float *)L.coef.as<float>(), L.pos.as<uint32_t>(), L.val.as<double>(), ctl,
+        const dim3 g2(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2));
+        const double *inv = L.w32 ? (const double *)g->invw64.as<double>()
+                                  : (g->weighted ? nullptr : (const double *)g->inv64.as<double>());
+        auto pass1 = [&](auto kernel, const auto *coef) {
+            hipLaunchKernelGGL(kernel, g1, dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, a.xold, inv, n,
+                               L.src.as<uint16_t>(), coef, L.pos.as<uint32_t>(), L.val.as<double>(), L.mp, ctl,
                                (const PhaseCtl *)a.ctl);
+        };
+        const float *w32 = (const float *)L.coef.as<float>();
+        const double *c64 = g->weighted ? (const double *)L.coef.as<double>() : nullptr;
+        if (L.w32 && compact) pass1(bin_scatter64_kernel<float, true>, w32);
+        else if (L.w32) pass1(bin_scatter64_kernel<float, false>, w32);
+        else if (compact) pass1(bin_scatter64_kernel<double, true>, c64);
+        else pass1(bin_scatter64_kernel<double, false>, c64);
+        if (compact)
+            hipLaunchKernelGGL(bin_gather64_kernel<true>, g2, dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2,
+                               L.dst.as<uint16_t>(), L.val.as<double>(), L.mp, L.part.as<long long>(), ctl, L.insum, e);
         else
-            hipLaunchKernelGGL(bin_scatter64_kernel<double>, g1, dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1,
-                               a.xold, g->weighted ? nullptr : (const double *)g->inv64.as<double>(), n,
-                               L.src.as<uint16_t>(), g->weighted ? (const double *)L.coef.as<double>() : nullptr,
-                               L.pos.as<uint32_t>(), L.val.as<double>(), ctl, (const PhaseCtl *)a.ctl);
-        hipLaunchKernelGGL(bin_gather64_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
-                           dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2, L.dst.as<uint16_t>(),
-                           L.val.as<double>(), L.part.as<long long>(), ctl, L.insum, e);
+            hipLaunchKernelGGL(bin_gather64_kernel<false>, g2, dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2,
+                               L.dst.as<uint16_t>(), L.val.as<double>(), L.mp, L.part.as<long long>(), ctl, L.insum, e);
         if (L.nmb)
             hipLaunchKernelGGL(bin_reduce64_kernel, dim3(L.nmb * kReduce64Blocks), dim3(kReduce64Threads), 0, s, L.mb.as<uint4>(),
                                (const long long *)L.part.as<long long>(), (const BinCtl *)ctl, L.insum, e);

@@ -164,7 +164,14 @@ template <typename T> struct SweepArgs {
     uint32_t n;
     uint32_t sink_begin;     // nodes [sink_begin, n) have no out-edges
     int32_t distribute;      // add damp * sink_in / n to every node
+    // FP64 residual sweeps through the binned copy: store each contribution in 6 bytes (an
+    // FP32 head and a 16-bit tail, about 40 significant bits) instead of 8
+    int32_t compact64;
 };
+
+// Relative error of a contribution stored as an FP32 head and a 16-bit tail (at most
+// 2^-39 of its magnitude); a sweep that uses them may be off by this much times sum |c x|.
+constexpr double kCompact64Rel = 0x1p-39;
 
 struct FinalizeArgs {
     const uint32_t *split_rows;   // [n_split_rows] sliced-row index j
