@@ -203,7 +203,9 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         // error enters the certificate twice, through r and through the returned y: the bound
         // grows by (2 d / (1 - d) + 1) d kCompact64Rel (1 + ||r||_1 / sum(y)), about 2e-11 at
         // d = 0.85 (1% added for the rounding of these FP64 formulas).
-        const bool compact = binned64_active(g);
+        // NKPR_COMPACT64=0 keeps 8-byte contributions
+        const char *cenv = std::getenv("NKPR_COMPACT64");
+        const bool compact = binned64_active(g) && !(cenv && cenv[0] == '0');
         const double slack = compact ? (2.0 * damp / (1.0 - damp) + 1.0) * damp * kCompact64Rel * 1.01 : 0.0;
         for (int round = 0;; ++round) {
             // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)

@@ -472,6 +472,41 @@ def test_fp64_binned_sweep_teleport_and_mixed(monkeypatch):
         assert np.abs(x2 - ref2).sum() <= 1e-10
 
 
+@pytest.mark.parametrize("weights", ["fp32", "fp64", None])
+def test_mixed_residual_sweeps_with_6_byte_contributions(monkeypatch, weights):
+    """The mixed solve's residual sweeps through the binned FP64 copy store every contribution
+    as an FP32 head and a 16-bit tail (within 2^-39 of it).  Against the same solve with 8-byte
+    contributions (NKPR_COMPACT64=0): both certify and match the converged oracle at tol, with
+    the same refinements (within one), and the compact run's bound carries the slack."""
+    monkeypatch.setenv("NKPR_BINNED64", "1")
+    z, u, v, w = skewed_graph(50000, 400000, 71, weighted=weights is not None)
+    if weights == "fp32":
+        w = w.astype(np.float32).astype(np.float64)
+    elif weights == "fp64":
+        w = w * (1.0 + 1e-9)  # not FP32 numbers: the copy keeps FP64 coefficients
+    g = oracle.in_lists(z, u, v, w, True)
+    ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=20000)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("NKPR_COMPACT64", flag)
+        w32 = None if g.in_w is None else g.in_w.astype(np.float32)
+        dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, weight64=g.in_w)
+        x = np.full(z, 1.0 / z)
+        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-10,
+                                           stop=capi.NKPR_STOP_L1_ERROR))
+        assert st["l1_error_bound"] <= 1e-10, flag
+        assert np.abs(x - ref).sum() <= 1e-10, flag
+        out[flag] = (x, st)
+    (xc, sc), (x8, s8) = out["1"], out["0"]
+    # a residual differs by at most d 2^-39 ||x||_1 (~1.5e-12 here), below the FP32 rounding
+    # that the correction solves add anyway: the solves take the same path up to a refinement
+    # that ends close to the bound, and both results are certified within tol of the solution
+    assert abs(int(sc["refinements"]) - int(s8["refinements"])) <= 1
+    assert np.abs(xc - x8).sum() <= 2e-10
+    slack = (2 * 0.85 / 0.15 + 1) * 0.85 * 2.0**-39 * 1.01
+    assert sc["l1_error_bound"] >= slack
+
+
 def test_binned_copies_fall_back_when_memory_is_short(monkeypatch):
     """A device without room for the binned copies (forced from the host: the build's first
     allocation is refused) keeps the pull sweep: the FP64 run still has the oracle's iteration
