@@ -972,8 +972,22 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
     NKPR_HIP(hipStreamSynchronize(s));
 
     // work items (NKPR_BINNED_ITEM overrides both sizes, e.g. to exercise split bins); sizes in
-    // whole quads so that every item boundary stays quad-aligned
-    uint64_t e1max = kPass1Edges, e2max = kPass2Edges;
+    // whole quads so that every item boundary stays quad-aligned.  Items shrink on mid-size
+    // graphs so that each persistent block still draws a few from the queue: about m / (2 CUs)
+    // edges, a power of two within [2^16, 2^19] (pass 1) and [2^18, 2^20] (pass 2).  Measured:
+    // R-MAT scale 26 (1.06e9 edges) keeps 2^19 / 2^20 (2^18 for both: 3.68 instead of 3.38 ms
+    // per FP32 application); the RHG graph of 2.0e8 edges takes 2^18 / 2^18 (0.61 instead of
+    // 0.66 ms).
+    int cus = 0;
+    NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    const uint64_t per_block = mp / (2 * (uint64_t)std::max(cus, 1));
+    auto pow2_within = [](uint64_t v, uint64_t lo, uint64_t hi) {
+        uint64_t p = lo;
+        while (p * 2 <= v && p * 2 <= hi) p *= 2;
+        return p;
+    };
+    uint64_t e1max = pow2_within(per_block, 1ull << 16, kPass1Edges);
+    uint64_t e2max = pow2_within(per_block, 1ull << 18, kPass2Edges);
     if (const char *ev = std::getenv("NKPR_BINNED_ITEM")) {
         const uint64_t v = std::strtoull(ev, nullptr, 10);
         if (v > 0) e1max = e2max = (v + kQuad - 1) / kQuad * kQuad;
