@@ -207,10 +207,34 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         const char *cenv = std::getenv("NKPR_COMPACT64");
         const bool compact = binned64_active(g) && !(cenv && cenv[0] == '0');
         const double slack = compact ? (2.0 * damp / (1.0 - damp) + 1.0) * damp * kCompact64Rel * 1.01 : 0.0;
+        // A run that starts from a constant vector x = c (every first run does: 1/n) begins
+        // with y = d P x + b = d c rowsum + b in closed form, from the cached FP64 row sums of
+        // P: an O(n) pass instead of an O(m) sweep.  It is not counted as a sweep, and it never
+        // certifies (the next residual is a full FP64 sweep).
+        const char *ccenv = std::getenv("NKPR_CLOSED_START");
+        bool closed = !(ccenv && ccenv[0] == '0') && n > 0;
+        double cval = 0.0;
+        if (closed) {
+            ensure(g->flag_buf, 4);
+            uint32_t *flag = g->flag_buf.as<uint32_t>();
+            launch_constant_check(xcur, n, flag, s);
+            uint32_t hf = 1;
+            NKPR_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, s));
+            NKPR_HIP(hipMemcpyAsync(&cval, xcur, 8, hipMemcpyDeviceToHost, s));
+            NKPR_HIP(hipStreamSynchronize(s));
+            closed = hf == 0;
+        }
+        if (closed && !g->rowsum_ready) {
+            batch_layout(g);
+            g->rowsum64.alloc(8 * std::max<uint64_t>(n, 1));
+            launch_rowsum64(g->b_rowptr.as<uint32_t>(), n, g->b_src.as<int32_t>(),
+                            folded ? g->b_coef64.as<double>() : nullptr,
+                            folded ? nullptr : g->inv64.as<double>(), g->rowsum64.as<double>(), s);
+            NKPR_HIP(hipStreamSynchronize(s));
+            g->rowsum_ready = true;
+        }
         for (int round = 0;; ++round) {
             // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
-            if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
-            init_phase(xcur, true);
             PhaseRule rr{};
             rr.kind = kRuleErr;
             rr.norm = NKPR_L1;
@@ -220,10 +244,32 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             SweepArgs<double> a64 = base_args64();
             a64.r32 = kr.V;  // the residual is the first Krylov vector
             a64.compact64 = compact ? 1 : 0;
-            Vecs<double> vr{{xcur, ynew}, {xh, g->xh64b.as<double>()}};
-            const uint64_t k = run_phase<double>(rc, a64, vr, kind_resid, rr, 1, timing, &st->ms_sweeps_fp64);
-            st->sweeps_fp64 += k;
-            total += k;
+            const bool closed_round = closed && round == 0;
+            if (closed_round) {
+                init_phase(xcur, true);
+                const double sinks = (double)(n - g->sink_begin);
+                const double rhs = tele + (dist ? damp * cval * sinks / (double)n : 0.0);
+                ensure(g->partials, 8 * 4 * 2048);
+                const unsigned blocks = launch_const_resid(g->rowsum64.as<double>(), n, damp * cval, rhs,
+                                                           base64, cval, g->sink_begin, ynew, kr.V,
+                                                           g->partials.as<double>(), s);
+                SweepArgs<double> fa = a64;
+                fa.partials = g->partials.as<double>();
+                fa.ctl = ctl;
+                FinalizeArgs ff{};
+                ff.n_blocks = blocks;
+                ff.rule = rr;
+                launch_finalize64(fa, ff, kind_resid, s);
+                NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost, s));
+                NKPR_HIP(hipStreamSynchronize(s));
+            } else {
+                if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
+                init_phase(xcur, true);
+                Vecs<double> vr{{xcur, ynew}, {xh, g->xh64b.as<double>()}};
+                const uint64_t k = run_phase<double>(rc, a64, vr, kind_resid, rr, 1, timing, &st->ms_sweeps_fp64);
+                st->sweeps_fp64 += k;
+                total += k;
+            }
             const PhaseCtl c = *g->ctl_host;
             const double bound = c.bound + (c.sum > 0.0 ? slack * (1.0 + c.l1 / c.sum) : 0.0);
             st->last_step_l1 = c.l1;
@@ -232,7 +278,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             res.sum = c.sum;
             res.sink = c.sink_out;
             res.x = ynew;
-            if (bound <= p->tol || total >= max_it || round >= kMaxRefinements) break;
+            if ((bound <= p->tol && !closed_round) || total >= max_it || round >= kMaxRefinements) break;
             // FP32 correction: A e = r, tightened so that the next residual certifies
             ++st->refinements;
             const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);

@@ -901,6 +901,7 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
     NKPR_HIP(hipStreamSynchronize(s));
     // derived copies follow the coefficients
     g->b_coef_ready = false;
+    g->rowsum_ready = false;
     g->bn32.coef_ready = false;
     g->bn64.coef_ready = false;
     g->adj_stale = true;

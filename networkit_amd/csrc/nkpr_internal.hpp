@@ -209,6 +209,16 @@ void launch_gather_scaled(const double *in_z, const int32_t *perm, uint32_t n, d
 void launch_to_double(const float *x32, double *x64, uint32_t n, hipStream_t s);
 void launch_add_correction(double *x64, const float *e32, uint32_t n, hipStream_t s);
 void launch_copy_f32(const float *in, float *out, uint32_t n, hipStream_t s);
+// rowsum[u] = sum of the in-coefficients of u (FP64), from the plain CSC copy
+void launch_rowsum64(const uint32_t *rowptr, uint32_t n, const int32_t *src, const double *coef,
+                     const double *inv, double *rowsum, hipStream_t s);
+// flag (device) = 1 when x[0, n) is not constant
+void launch_constant_check(const double *x, uint32_t n, uint32_t *flag, hipStream_t s);
+// residual sweep from a constant vector c in closed form (ynew, r32 and the block partials
+// (l1, l2, sum, sink) of a sweep); returns the number of partial blocks
+unsigned launch_const_resid(const double *rowsum, uint32_t n, double dc, double rhs,
+                            const double *base, double c, uint32_t sink_begin, double *ynew,
+                            float *r32, double *partials, hipStream_t s);
 void launch_scatter_result(const double *y, const int32_t *perm, uint32_t n, double factor,
                            double *scores_z, hipStream_t s);
 // Deterministic two-stage sums / max over x[begin, end).
@@ -324,6 +334,9 @@ struct nkpr_graph {
     nkpr::DevBuf inv32;     // float [n]   (unweighted)
     nkpr::DevBuf inv64;     // double [n]  (unweighted)
     nkpr::DevBuf invw64;    // double [n]  1 / weighted out-degree (weighted)
+    nkpr::DevBuf rowsum64;  // double [n]  FP64 sum of each row's in-coefficients (built on use)
+    bool rowsum_ready = false;
+    nkpr::DevBuf flag_buf;  // uint32 [1]  device-side yes/no answers read back by the host
     bool w32_exact = false; // every weight is an FP32 number (weighted)
     nkpr::DevBuf emap;      // uint32 [m]  input position of every edge in sweep order (weighted)
     std::vector<uint8_t> exists_host;  // empty: every slot exists

@@ -425,7 +425,8 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(SweepArgs<T>
             if (it >= 4 && l1 > r.stall_ratio * prev) stop = true;
         }
         ctl->done = stop ? 1u : 0u;
-        for (int s = 0; s < kSlices; ++s) a.sc.claim[s] = 0u;
+        if (a.sc.claim)
+            for (int s = 0; s < kSlices; ++s) a.sc.claim[s] = 0u;
     }
 }
 
@@ -480,6 +481,64 @@ __global__ void to_double_kernel(const float *x32, double *x64, uint32_t n) {
 __global__ void add_correction_kernel(double *x64, const float *e32, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x64[i] += (double)e32[i];
+}
+
+// rowsum[u] = the FP64 sum of row u's in-coefficients, c_vu = w_vu / deg_v (coef) or 1 / deg_v
+// (inv[v]): one wave per row, lanes strided over the row, then a fixed shuffle tree.
+__global__ void rowsum_kernel(const uint32_t *rowptr, uint32_t n, const int32_t *src,
+                              const double *coef, const double *inv, double *rowsum) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; u < n; u += nw) {
+        double sum = 0.0;
+        for (uint32_t e = rowptr[u] + lane; e < rowptr[u + 1]; e += 64)
+            sum += coef ? coef[e] : inv[src[e]];
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane == 0) rowsum[u] = sum;
+    }
+}
+
+// flag = 1 when some x[u], u < n, differs from x[0]
+__global__ void constant_check_kernel(const double *x, uint32_t n, uint32_t *flag) {
+    const double x0 = x[0];
+    int diff = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x)
+        diff |= x[u] != x0;
+    if (__syncthreads_or(diff) && threadIdx.x == 0) atomicOr(flag, 1u);
+}
+
+// The residual sweep from the constant vector x = c in closed form: y = d c rowsum + rhs
+// (+ base), ynew = y, r32 = float(y - c), and the sweep's block partials (l1, l2, sum, sink).
+__global__ __launch_bounds__(256) void const_resid_kernel(const double *rowsum, uint32_t n,
+                                                          double dc, double rhs,
+                                                          const double *base, double c,
+                                                          uint32_t sink_begin, double *ynew,
+                                                          float *r32, double *partials) {
+    __shared__ double s_red[256 / 64];
+    double l1 = 0.0, l2 = 0.0, sum = 0.0, sink = 0.0;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        double y = dc * rowsum[u] + rhs;
+        if (base) y += base[u];
+        ynew[u] = y;
+        const double dif = y - c;
+        r32[u] = (float)dif;
+        l1 += fabs(dif);
+        l2 += dif * dif;
+        sum += y;
+        if (u >= sink_begin) sink += y;
+    }
+    l1 = block_sum<256>(l1, s_red);
+    l2 = block_sum<256>(l2, s_red);
+    sum = block_sum<256>(sum, s_red);
+    sink = block_sum<256>(sink, s_red);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x * 4 + 0] = l1;
+        partials[blockIdx.x * 4 + 1] = l2;
+        partials[blockIdx.x * 4 + 2] = sum;
+        partials[blockIdx.x * 4 + 3] = sink;
+    }
 }
 
 __global__ void copy_f32_kernel(const float *in, float *out, uint32_t n) {
@@ -677,6 +736,32 @@ void launch_add_correction(double *x64, const float *e32, uint32_t n, hipStream_
     if (!n) return;
     hipLaunchKernelGGL(add_correction_kernel, dim3(grid1d(n)), dim3(256), 0, s, x64, e32, n);
     NKPR_HIP(hipGetLastError());
+}
+
+void launch_rowsum64(const uint32_t *rowptr, uint32_t n, const int32_t *src, const double *coef,
+                     const double *inv, double *rowsum, hipStream_t s) {
+    if (!n) return;
+    const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)n * 64 + 255) / 256, 1u << 16);
+    hipLaunchKernelGGL(rowsum_kernel, dim3(grid), dim3(256), 0, s, rowptr, n, src, coef, inv, rowsum);
+    NKPR_HIP(hipGetLastError());
+}
+
+void launch_constant_check(const double *x, uint32_t n, uint32_t *flag, hipStream_t s) {
+    NKPR_HIP(hipMemsetAsync(flag, 0, 4, s));
+    if (!n) return;
+    hipLaunchKernelGGL(constant_check_kernel, dim3(std::min<unsigned>(grid1d(n), 2048)), dim3(256), 0, s,
+                       x, n, flag);
+    NKPR_HIP(hipGetLastError());
+}
+
+unsigned launch_const_resid(const double *rowsum, uint32_t n, double dc, double rhs,
+                            const double *base, double c, uint32_t sink_begin, double *ynew,
+                            float *r32, double *partials, hipStream_t s) {
+    const unsigned grid = std::max(1u, std::min<unsigned>(grid1d(n), 2048));
+    hipLaunchKernelGGL(const_resid_kernel, dim3(grid), dim3(256), 0, s, rowsum, n, dc, rhs, base, c,
+                       sink_begin, ynew, r32, partials);
+    NKPR_HIP(hipGetLastError());
+    return grid;
 }
 
 void launch_copy_f32(const float *in, float *out, uint32_t n, hipStream_t s) {
