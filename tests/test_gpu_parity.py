@@ -507,6 +507,48 @@ def test_mixed_residual_sweeps_with_6_byte_contributions(monkeypatch, weights):
     assert sc["l1_error_bound"] >= slack
 
 
+@pytest.mark.parametrize("weighted,sinks,teleport", [(True, 0, False), (False, 1, False),
+                                                     (True, 1, True)])
+def test_mixed_closed_form_start(monkeypatch, weighted, sinks, teleport):
+    """A mixed solve from a constant start takes its first residual in closed form (d c rowsum(P)
+    + b): against the same solve with a first FP64 sweep (NKPR_CLOSED_START=0) it certifies at
+    tol, matches the converged oracle, takes the same refinements and one FP64 sweep fewer.  A
+    start that is not constant takes the sweep."""
+    z, u, v, w = skewed_graph(30000, 250000, 73, weighted=weighted)
+    g = oracle.in_lists(z, u, v, w, True)
+    t = None
+    if teleport:
+        rng = np.random.default_rng(9)
+        t = np.zeros(z)
+        np.add.at(t, rng.choice(z, 40), rng.uniform(0.5, 2.0, 40))
+        t /= t.sum()
+    ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=20000,
+                                sink_mode=sinks, teleport=t)
+    w32 = None if g.in_w is None else g.in_w.astype(np.float32)
+    runs = {}
+    for flag, start in (("1", "const"), ("0", "const"), ("1", "warm")):
+        monkeypatch.setenv("NKPR_CLOSED_START", flag)
+        dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, weight64=g.in_w)
+        if start == "const":
+            x = np.full(z, 1.0 / z)
+        else:
+            x = np.random.default_rng(10).uniform(0.5, 1.5, z)
+            x /= x.sum()
+        p = capi.default_params(precision=capi.NKPR_MIXED, tol=1e-10, sink_handling=sinks,
+                                stop=capi.NKPR_STOP_L1_ERROR)
+        if teleport:
+            p.teleport = t.ctypes.data
+        st = dg.run(x, p)
+        assert st["l1_error_bound"] <= 1e-10, (flag, start)
+        assert np.abs(x - ref).sum() <= 1e-10, (flag, start)
+        runs[(flag, start)] = st
+    closed, swept, warm = runs[("1", "const")], runs[("0", "const")], runs[("1", "warm")]
+    assert abs(int(closed["refinements"]) - int(swept["refinements"])) <= 1
+    if closed["refinements"] == swept["refinements"]:
+        assert closed["sweeps_fp64"] == swept["sweeps_fp64"] - 1
+    assert warm["sweeps_fp64"] == warm["refinements"] + 1  # every residual was a sweep
+
+
 def test_binned_copies_fall_back_when_memory_is_short(monkeypatch):
     """A device without room for the binned copies (forced from the host: the build's first
     allocation is refused) keeps the pull sweep: the FP64 run still has the oracle's iteration
