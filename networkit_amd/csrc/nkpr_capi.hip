@@ -224,15 +224,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             NKPR_HIP(hipStreamSynchronize(s));
             closed = hf == 0;
         }
-        if (closed && !g->rowsum_ready) {
-            batch_layout(g);
-            g->rowsum64.alloc(8 * std::max<uint64_t>(n, 1));
-            launch_rowsum64(g->b_rowptr.as<uint32_t>(), n, g->b_src.as<int32_t>(),
-                            folded ? g->b_coef64.as<double>() : nullptr,
-                            folded ? nullptr : g->inv64.as<double>(), g->rowsum64.as<double>(), s);
-            NKPR_HIP(hipStreamSynchronize(s));
-            g->rowsum_ready = true;
-        }
+        if (closed) ensure_rowsum64(g);
         for (int round = 0;; ++round) {
             // FP64 residual sweep: ynew = d P xcur + b, r32 = float(ynew - xcur)
             PhaseRule rr{};
@@ -298,6 +290,19 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
     st->iterations = total;
     st->score_sum = res.sum;
     return res;
+}
+
+// The FP64 row sums of P (sum of each row's in-coefficients), kept per graph until new weights.
+void ensure_rowsum64(nkpr_graph *g) {
+    if (g->rowsum_ready) return;
+    const uint32_t n = g->n;
+    batch_layout(g);
+    g->rowsum64.alloc(8 * std::max<uint64_t>(n, 1));
+    launch_rowsum64(g->b_rowptr.as<uint32_t>(), n, g->b_src.as<int32_t>(),
+                    g->weighted ? g->b_coef64.as<double>() : nullptr,
+                    g->weighted ? nullptr : g->inv64.as<double>(), g->rowsum64.as<double>(), g->stream);
+    NKPR_HIP(hipStreamSynchronize(g->stream));
+    g->rowsum_ready = true;
 }
 
 Krylov krylov_setup(nkpr_graph *g) {

@@ -168,6 +168,7 @@ void prepare_run(nkpr_graph *g);
 SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64,
                        const float *base32, nkpr_stats *st);
 Krylov krylov_setup(nkpr_graph *g);
+void ensure_rowsum64(nkpr_graph *g);
 void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
                     double *ms_acc, bool minus);
 bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
