@@ -20,6 +20,17 @@ def pytest_collection_modifyitems(config, items):
     pass
 
 
+@pytest.fixture(autouse=True)
+def _collect_after_gpu_test(request):
+    """Device handles caught in reference cycles are destroyed by the garbage collector; collect
+    after every GPU test so that this happens right after the test that made them, where a
+    failure would name the test, and not later inside pytest's own bookkeeping."""
+    yield
+    if request.node.get_closest_marker("gpu") is not None:
+        import gc
+        gc.collect()
+
+
 @pytest.fixture(scope="session")
 def oracle_lib():
     import oracle
