@@ -190,9 +190,9 @@ bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool dis
 // the criterion of Daniel, Gragg, Kaufman and Stewart, with eta = 0.01: only a projection
 // that cancels all but a hundredth of the norm is redone; the FP64 residual sweeps check
 // every correction, and on the R-MAT scale-26, RHG and Erdos-Renyi workloads eta = 0.1
-// repeated projections without changing the applications or the result).  A vector whose norm leaves
-// [2^-50, 2^50] is rescaled on the device, so the FP32 entries stay far from the ends of the
-// range.  `rhs` may be basis slot 0 itself; it is overwritten then.  The Hessenberg
+// repeated projections without changing the applications or the result).  A vector whose
+// norm leaves [2^-50, 2^50] is rescaled on the device, so the FP32 entries stay far from the
+// ends of the range.  `rhs` may be basis slot 0 itself; it is overwritten then.  The Hessenberg
 // least-squares problem is solved on the host with Givens rotations.  Returns the number of
 // applications.
 template <typename Apply>
