@@ -91,3 +91,16 @@ def test_snap_reader_ids_and_dedup(tmp_path):
     assert z == 3 and u.tolist() == [0, 1, 2] and v.tolist() == [1, 0, 0]
     z, u, v = read_snap_edges(str(p), directed=False)
     assert u.tolist() == [0, 2] and v.tolist() == [1, 0]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark measures one GPU: a request for more exits with status 2 and says so,
+    before anything touches a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       cwd=root, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 2
+    assert "measures one GPU" in p.stderr and p.stdout == ""
