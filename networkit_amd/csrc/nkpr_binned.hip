@@ -182,6 +182,30 @@ __global__ void bin_pos_kernel(const uint32_t *cseg, const uint32_t *bseg, uint6
     }
 }
 
+// Bin-major place of chunk-major quad q from the group data: every segment starts on a quad,
+// so the quad's segment is the group's first one plus the segment starts among lanes 1..l of
+// the group, and its place is the quad's own position plus that segment's offset (modulo
+// 2^32).  The segment index is clamped to the offset table.
+__device__ __forceinline__ uint32_t derived_place(const unsigned long long *gmask,
+                                                  const uint32_t *gfirst, const uint32_t *sdelta,
+                                                  uint32_t nsd, uint32_t q) {
+    const uint32_t grp = q >> 6, l = q & 63u;
+    const unsigned long long below = (gmask[grp] & ~1ull) << (63u - l);
+    const uint32_t sidx = min(gfirst[grp] + (uint32_t)__popcll(below), nsd - 1u);
+    return q * kQuad + sdelta[sidx];
+}
+
+// flag = 1 when a derived place differs from pos (checked once per layout)
+__global__ void bin_place_check_kernel(const uint32_t *pos, uint64_t nq,
+                                       const unsigned long long *gmask, const uint32_t *gfirst,
+                                       const uint32_t *sdelta, uint32_t nsd, uint32_t *flag) {
+    int diff = 0;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+         q += (uint64_t)gridDim.x * blockDim.x)
+        diff |= derived_place(gmask, gfirst, sdelta, nsd, (uint32_t)q) != pos[q];
+    if (__syncthreads_or(diff) && threadIdx.x == 0) atomicOr(flag, 1u);
+}
+
 // Source offsets and coefficients in chunk-major order, destination offsets in bin-major
 // order.  A padding slot gets the source offset 2^ss (the zero table entry).
 template <typename C>
@@ -329,9 +353,14 @@ __device__ __forceinline__ void fill_table(T *tab, const T *v, const T *inv, uin
 // Pass 1: item = {chunk, first edge, end edge} (chunk-major, quad-aligned).  The contribution
 // of an edge is v[src] * coef (weighted) or v[src] / deg(src) (unweighted: inv), stored at the
 // edge's bin-major place.  Also finds the largest |v| of the sources (sinks contribute nothing).
+// DERIVED: the places come from the group data (derived_place) instead of pos, and a store
+// whose place is not inside the copy is dropped.
+template <bool DERIVED>
 __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
     const uint4 *items, uint32_t nitems, const float *v, const float *inv, uint32_t n,
-    const uint16_t *bsrc, const float *bcoef, const uint32_t *pos, float *bval, BinCtl *ctl) {
+    const uint16_t *bsrc, const float *bcoef, const uint32_t *pos, const unsigned long long *gmask,
+    const uint32_t *gfirst, const uint32_t *sdelta, uint32_t nsd, uint32_t mp, float *bval,
+    BinCtl *ctl) {
     constexpr uint32_t kSpan = 1u << kSpan32Shift;
     __shared__ __align__(16) float tab[kSpan + 1];
     __shared__ uint32_t slot;
@@ -357,32 +386,48 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
             if (q < q1) {
                 sp[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + q * kQuad));
                 c[k] = bcoef ? __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(bcoef + q * kQuad)) : one;
-                at[k] = __builtin_nontemporal_load(pos + q);
+                at[k] = DERIVED ? derived_place(gmask, gfirst, sdelta, nsd, q)
+                                : __builtin_nontemporal_load(pos + q);
             }
         }
         for (uint32_t b = qt; b < q1; b += kStep) {
             u32x2 spn[kP1Quads];
             f32x4 cn[kP1Quads];
             uint32_t atn[kP1Quads];
+            unsigned long long mkn[kP1Quads];
+            uint32_t fn[kP1Quads];
 #pragma unroll
             for (int k = 0; k < kP1Quads; ++k) {
                 const uint32_t q = b + kStep + k * kPassThreads;
                 if (q < q1) {
+                    // group data first, so that its wait does not include the streams below
+                    if (DERIVED) {
+                        mkn[k] = gmask[q >> 6];
+                        fn[k] = gfirst[q >> 6];
+                    }
                     spn[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + q * kQuad));
                     cn[k] = bcoef ? __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(bcoef + q * kQuad)) : one;
-                    atn[k] = __builtin_nontemporal_load(pos + q);
+                    if (!DERIVED) atn[k] = __builtin_nontemporal_load(pos + q);
                 }
             }
 #pragma unroll
             for (int k = 0; k < kP1Quads; ++k) {
                 const uint32_t q = b + k * kPassThreads;
-                if (q < q1) {
+                if (q < q1 && (!DERIVED || at[k] <= mp - kQuad)) {
                     f32x4 r;
                     r.x = tab[sp[k].x & 0xFFFFu] * c[k].x;
                     r.y = tab[sp[k].x >> 16] * c[k].y;
                     r.z = tab[sp[k].y & 0xFFFFu] * c[k].z;
                     r.w = tab[sp[k].y >> 16] * c[k].w;
                     __builtin_nontemporal_store(r, reinterpret_cast<f32x4 *>(bval + at[k]));
+                }
+                if (DERIVED) {
+                    const uint32_t qn = b + kStep + k * kPassThreads;
+                    if (qn < q1) {
+                        const unsigned long long below = (mkn[k] & ~1ull) << (63u - (qn & 63u));
+                        const uint32_t sidx = min(fn[k] + (uint32_t)__popcll(below), nsd - 1u);
+                        atn[k] = qn * kQuad + sdelta[sidx];
+                    }
                 }
                 sp[k] = spn[k];
                 c[k] = cn[k];
@@ -962,6 +1007,53 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
                            dim3(256), 0, s, (const uint32_t *)cseg.as<uint32_t>(),
                            (const uint32_t *)bseg.as<uint32_t>(), nseg, L.pos.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
+    L.derived = false;
+    if (!L.fp64 && mp) {
+        // group data of pass 1: about 0.1 byte per edge instead of the 1-byte place
+        const uint64_t nq = mp / kQuad, ngrp = (nq + 63) / 64;
+        std::vector<unsigned long long> gm(ngrp, 0ull);
+        std::vector<uint32_t> gf(ngrp), sd;
+        for (uint64_t q = 0; q < nseg; ++q) {
+            if (!padded(q)) continue;
+            const uint64_t qs = chost[q] / kQuad;
+            gm[qs >> 6] |= 1ull << (qs & 63);
+            sd.push_back(bhost[q] - chost[q]);
+        }
+        // quad 0 starts the first non-empty segment, so group 0 has bit 0 set
+        uint32_t cum = 0;
+        for (uint64_t i = 0; i < ngrp; ++i) {
+            gf[i] = cum + (uint32_t)(gm[i] & 1ull) - 1u;
+            cum += (uint32_t)__builtin_popcountll(gm[i]);
+        }
+        L.gmask.alloc(8 * ngrp);
+        L.gfirst.alloc(4 * ngrp);
+        L.sdelta.alloc(4 * sd.size());
+        L.nsdelta = (uint32_t)sd.size();
+        NKPR_HIP(hipMemcpyAsync(L.gmask.p, gm.data(), 8 * ngrp, hipMemcpyHostToDevice, s));
+        NKPR_HIP(hipMemcpyAsync(L.gfirst.p, gf.data(), 4 * ngrp, hipMemcpyHostToDevice, s));
+        NKPR_HIP(hipMemcpyAsync(L.sdelta.p, sd.data(), 4 * sd.size(), hipMemcpyHostToDevice, s));
+        DevBuf chk;
+        chk.alloc(4);
+        NKPR_HIP(hipMemsetAsync(chk.p, 0, 4, s));
+        hipLaunchKernelGGL(bin_place_check_kernel, dim3((unsigned)std::min<uint64_t>((nq + 255) / 256, 2048)),
+                           dim3(256), 0, s, (const uint32_t *)L.pos.as<uint32_t>(), nq,
+                           (const unsigned long long *)L.gmask.as<unsigned long long>(),
+                           (const uint32_t *)L.gfirst.as<uint32_t>(), (const uint32_t *)L.sdelta.as<uint32_t>(),
+                           L.nsdelta, chk.as<uint32_t>());
+        NKPR_HIP(hipGetLastError());
+        uint32_t bad = 1;
+        NKPR_HIP(hipMemcpyAsync(&bad, chk.p, 4, hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipStreamSynchronize(s));
+        L.derived = bad == 0;
+        if (!L.derived) {
+            L.gmask.reset();
+            L.gfirst.reset();
+            L.sdelta.reset();
+        }
+        if (std::getenv("NKPR_DEBUG"))
+            std::fprintf(stderr, "[nkpr] binned FP32 places: %llu groups, %u segments, derived %s\n",
+                         (unsigned long long)ngrp, L.nsdelta, L.derived ? "yes" : "no (places read)");
+    }
     const size_t vb = L.fp64 ? 8 : 4;
     L.w32 = L.fp64 && g->weighted && g->w32_exact;
     L.src.alloc(2 * std::max<uint64_t>(mp, 1) + 64);
@@ -1051,9 +1143,9 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
 
 void release_layout(BinLayout &L) {
     for (DevBuf *b : {&L.order, &L.pos, &L.src, &L.dst, &L.coef, &L.val, &L.p1, &L.p2, &L.mb,
-                      &L.part, &L.ctl, &L.partials})
+                      &L.part, &L.ctl, &L.partials, &L.gmask, &L.gfirst, &L.sdelta})
         b->reset();
-    L.ready = L.coef_ready = false;
+    L.ready = L.coef_ready = L.derived = false;
 }
 
 // Builds L if needed.  False when it cannot be built (too many edges for 32-bit positions, or
@@ -1120,15 +1212,20 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
         launch_range_sum<float>(v, g->sink_begin, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
         sink_sum = g->red_out.as<double>();
     }
-    persistent_grids(g, L, bin_scatter_kernel, bin_gather_kernel);
+    // NKPR_BINNED_POS=1 reads the stored places instead of deriving them
+    const char *pe = std::getenv("NKPR_BINNED_POS");
+    const bool derived = L.derived && !(pe && *pe && std::atoi(pe) != 0);
+    persistent_grids(g, L, bin_scatter_kernel<true>, bin_gather_kernel);
     if (ev0) NKPR_HIP(hipEventRecord(ev0, s));
     BinCtl *ctl = L.ctl.as<BinCtl>();
     hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1)),
+    const auto scatter = derived ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
+    hipLaunchKernelGGL(scatter, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1)),
                        dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, v,
                        g->weighted ? nullptr : g->inv32.as<float>(), n, L.src.as<uint16_t>(),
-                       g->weighted ? L.coef.as<float>() : nullptr, L.pos.as<uint32_t>(), L.val.as<float>(),
-                       ctl);
+                       g->weighted ? L.coef.as<float>() : nullptr, L.pos.as<uint32_t>(),
+                       L.gmask.as<unsigned long long>(), L.gfirst.as<uint32_t>(), L.sdelta.as<uint32_t>(),
+                       L.nsdelta, (uint32_t)L.mp, L.val.as<float>(), ctl);
     hipLaunchKernelGGL(bin_gather_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
                        dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2, L.dst.as<uint16_t>(),
                        L.val.as<float>(), L.part.as<long long>(), n, ctl, L.insum, v, minus, (float)damp,

@@ -319,16 +319,18 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs<T> a) {
         }
     }
 
-    // per-block partials (l1, l2, sum, sink) in FP64
+    // per-worker partials (l1, l2, sum, sink) in FP64: indexed by the claimed worker, whose
+    // rows are fixed, not by the block, whose worker depends on the order of the claims, so
+    // that the finalize kernel adds the same values in the same order on every run
     double v;
     v = block_sum<kBlock>(st.l1, s_red);
-    if (tid == 0) a.partials[blockIdx.x * 4 + 0] = v;
+    if (tid == 0) a.partials[w * 4 + 0] = v;
     v = block_sum<kBlock>(st.l2, s_red);
-    if (tid == 0) a.partials[blockIdx.x * 4 + 1] = v;
+    if (tid == 0) a.partials[w * 4 + 1] = v;
     v = block_sum<kBlock>(st.sum, s_red);
-    if (tid == 0) a.partials[blockIdx.x * 4 + 2] = v;
+    if (tid == 0) a.partials[w * 4 + 2] = v;
     v = block_sum<kBlock>(st.sink, s_red);
-    if (tid == 0) a.partials[blockIdx.x * 4 + 3] = v;
+    if (tid == 0) a.partials[w * 4 + 3] = v;
 }
 
 // Sliced rows: add the per-slice row sums in slice order and run the row epilogue.  Rows

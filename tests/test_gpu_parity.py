@@ -354,6 +354,51 @@ def test_mixed_binned_operator(monkeypatch, weighted, sinks, item):
     assert abs(int(stats["1"]["sweeps_fp32"]) - int(stats["0"]["sweeps_fp32"])) <= slack
 
 
+@pytest.mark.parametrize("weighted", [True, False])
+@pytest.mark.parametrize("item", ["0", "3000"])
+def test_binned_derived_places(monkeypatch, capfd, weighted, item):
+    """Pass 1 of the binned operator derives every quad's bin-major place from per-group data
+    (a segment-start mask and the first segment of each group of 64 quads, one offset per
+    segment) instead of reading the stored place.  The derivation is checked against the stored
+    places when the layout is built and is in use here; the operator then writes every
+    contribution to the same place, so on one graph handle (the same coefficients) the mixed
+    solve is bit-identical to the one that reads the stored places (NKPR_BINNED_POS=1), and
+    to a repeat of itself, and it certifies against the converged oracle."""
+    monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED_ITEM", item)
+    monkeypatch.setenv("NKPR_DEBUG", "1")
+    z, u, v, w = skewed_graph(150000, 1200000, 67, weighted=weighted)
+    g = oracle_graph(z, u, v, w, True)
+    ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=0, max_iterations=5000)
+    w32 = None if g.in_w is None else g.in_w.astype(np.float32)
+    dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, weight64=g.in_w)
+    capfd.readouterr()
+    xs, stats = {}, {}
+    for pos in ("0", "1", "0"):
+        monkeypatch.setenv("NKPR_BINNED_POS", pos)
+        x = np.full(z, 1.0 / z)
+        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
+                                           stop=capi.NKPR_STOP_L1_ERROR))
+        assert st["l1_error_bound"] <= 1e-9
+        assert np.abs(x - ref).sum() <= 1e-9
+        if pos in xs:
+            assert np.array_equal(x, xs[pos])
+        xs[pos], stats[pos] = x, st
+    err = capfd.readouterr().err
+    assert "binned FP32 places:" in err and "derived yes" in err
+    assert np.array_equal(xs["0"], xs["1"])
+    if not weighted:
+        # a second handle has the same coefficients (weighted out-degrees are added with
+        # atomics, in an order that can differ between handles in the last bit)
+        dg2 = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), None)
+        x = np.full(z, 1.0 / z)
+        dg2.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
+                                       stop=capi.NKPR_STOP_L1_ERROR))
+        assert np.array_equal(x, xs["0"])
+    for key in ("refinements", "sweeps_fp32", "sweeps_fp64"):
+        assert stats["0"][key] == stats["1"][key]
+
+
 
 def test_mixed_default_operator_on_large_graph(monkeypatch):
     """R-MAT scale 23 (about 1.3e8 edges, above the 2^26 edges where the GMRES corrections take
