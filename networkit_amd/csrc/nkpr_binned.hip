@@ -666,10 +666,13 @@ __device__ __forceinline__ void load_coef4(const float *c, uint32_t e, f64x2 &c0
 // weights); bcoef is the per-edge factor (FP64 coefficient or FP32 weight) or nullptr.
 // COMPACT: the contributions go to 6-byte places (FP32 heads at bval as float[mp], 16-bit
 // tails after them) instead of FP64 values.
-template <typename CT, bool COMPACT>
+// DERIVED: as in the FP32 pass 1, the places come from the group data and a store whose
+// place is not inside the copy is dropped.
+template <typename CT, bool COMPACT, bool DERIVED>
 __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
     const uint4 *items, uint32_t nitems, const double *x, const double *inv, uint32_t n,
-    const uint16_t *bsrc, const CT *bcoef, const uint32_t *pos, double *bval, uint64_t mp,
+    const uint16_t *bsrc, const CT *bcoef, const uint32_t *pos, const unsigned long long *gmask,
+    const uint32_t *gfirst, const uint32_t *sdelta, uint32_t nsd, double *bval, uint64_t mp,
     BinCtl *ctl, const PhaseCtl *pctl) {
     constexpr uint32_t kSpan = 1u << kSpan64Shift;
     if (pctl->done) return;
@@ -691,19 +694,27 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
         f64x2 c0 = one, c1 = one;
         if (q < q1) {
             sp = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + q * kQuad));
-            at = __builtin_nontemporal_load(pos + q);
+            at = DERIVED ? derived_place(gmask, gfirst, sdelta, nsd, q)
+                         : __builtin_nontemporal_load(pos + q);
             if (bcoef) load_coef4(bcoef, q * kQuad, c0, c1);
         }
         for (; q < q1; q += kPassThreads) {
             const uint32_t qn = q + kPassThreads;
             u32x2 spn = sp;
-            uint32_t atn = 0;
+            uint32_t atn = 0, fn = 0;
+            unsigned long long mkn = 0;
             f64x2 c0n = one, c1n = one;
             if (qn < q1) {
+                // group data first, so that its wait does not include the streams below
+                if (DERIVED) {
+                    mkn = gmask[qn >> 6];
+                    fn = gfirst[qn >> 6];
+                }
                 spn = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(bsrc + qn * kQuad));
-                atn = __builtin_nontemporal_load(pos + qn);
+                if (!DERIVED) atn = __builtin_nontemporal_load(pos + qn);
                 if (bcoef) load_coef4(bcoef, qn * kQuad, c0n, c1n);
             }
+            const bool store = !DERIVED || (uint64_t)at + kQuad <= mp;
             f64x2 r0 = {tab[sp.x & 0xFFFFu], tab[sp.x >> 16]};
             f64x2 r1 = {tab[sp.y & 0xFFFFu], tab[sp.y >> 16]};
             r0 *= c0;
@@ -720,11 +731,17 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter64_kernel(
                 unsigned short *tails = reinterpret_cast<unsigned short *>(heads + mp);
                 const u32x2 t = {(uint32_t)(unsigned short)q0 | ((uint32_t)(unsigned short)q1 << 16),
                                  (uint32_t)(unsigned short)q2 | ((uint32_t)(unsigned short)q3 << 16)};
-                __builtin_nontemporal_store(h, reinterpret_cast<f32x4 *>(heads + at));
-                __builtin_nontemporal_store(t, reinterpret_cast<u32x2 *>(tails + at));
-            } else {
+                if (store) {
+                    __builtin_nontemporal_store(h, reinterpret_cast<f32x4 *>(heads + at));
+                    __builtin_nontemporal_store(t, reinterpret_cast<u32x2 *>(tails + at));
+                }
+            } else if (store) {
                 __builtin_nontemporal_store(r0, reinterpret_cast<f64x2 *>(bval + at));
                 __builtin_nontemporal_store(r1, reinterpret_cast<f64x2 *>(bval + at + 2));
+            }
+            if (DERIVED && qn < q1) {
+                const unsigned long long below = (mkn & ~1ull) << (63u - (qn & 63u));
+                atn = qn * kQuad + sdelta[min(fn + (uint32_t)__popcll(below), nsd - 1u)];
             }
             sp = spn;
             at = atn;
@@ -1008,7 +1025,7 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
                            (const uint32_t *)bseg.as<uint32_t>(), nseg, L.pos.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
     L.derived = false;
-    if (!L.fp64 && mp) {
+    if (mp) {
         // group data of pass 1: about 0.1 byte per edge instead of the 1-byte place
         const uint64_t nq = mp / kQuad, ngrp = (nq + 63) / 64;
         std::vector<unsigned long long> gm(ngrp, 0ull);
@@ -1051,8 +1068,9 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
             L.sdelta.reset();
         }
         if (std::getenv("NKPR_DEBUG"))
-            std::fprintf(stderr, "[nkpr] binned FP32 places: %llu groups, %u segments, derived %s\n",
-                         (unsigned long long)ngrp, L.nsdelta, L.derived ? "yes" : "no (places read)");
+            std::fprintf(stderr, "[nkpr] binned %s places: %llu groups, %u segments, derived %s\n",
+                         L.fp64 ? "FP64" : "FP32", (unsigned long long)ngrp, L.nsdelta,
+                         L.derived ? "yes" : "no (places read)");
     }
     const size_t vb = L.fp64 ? 8 : 4;
     L.w32 = L.fp64 && g->weighted && g->w32_exact;
@@ -1252,8 +1270,10 @@ void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArg
                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     BinLayout &L = g->bn64;
     const uint32_t n = g->n;
-    if (L.w32) persistent_grids(g, L, bin_scatter64_kernel<float, false>, bin_gather64_kernel<false>);
-    else persistent_grids(g, L, bin_scatter64_kernel<double, false>, bin_gather64_kernel<false>);
+    if (L.w32) persistent_grids(g, L, bin_scatter64_kernel<float, false, true>, bin_gather64_kernel<false>);
+    else persistent_grids(g, L, bin_scatter64_kernel<double, false, true>, bin_gather64_kernel<false>);
+    const char *pe = std::getenv("NKPR_BINNED_POS");
+    const bool derived = L.derived && !(pe && *pe && std::atoi(pe) != 0);
     const bool basevec = kind == kSweepBaseVec || kind == kSweepBaseResid;
     const bool resid = kind == kSweepResid || kind == kSweepBaseResid;
     const bool compact = resid && a.compact64 != 0;
@@ -1286,15 +1306,24 @@ void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArg
                                   : (g->weighted ? nullptr : (const double *)g->inv64.as<double>());
         auto pass1 = [&](auto kernel, const auto *coef) {
             hipLaunchKernelGGL(kernel, g1, dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, a.xold, inv, n,
-                               L.src.as<uint16_t>(), coef, L.pos.as<uint32_t>(), L.val.as<double>(), L.mp, ctl,
-                               (const PhaseCtl *)a.ctl);
+                               L.src.as<uint16_t>(), coef, L.pos.as<uint32_t>(),
+                               (const unsigned long long *)L.gmask.as<unsigned long long>(),
+                               (const uint32_t *)L.gfirst.as<uint32_t>(), (const uint32_t *)L.sdelta.as<uint32_t>(),
+                               L.nsdelta, L.val.as<double>(), L.mp, ctl, (const PhaseCtl *)a.ctl);
         };
         const float *w32 = (const float *)L.coef.as<float>();
         const double *c64 = g->weighted ? (const double *)L.coef.as<double>() : nullptr;
-        if (L.w32 && compact) pass1(bin_scatter64_kernel<float, true>, w32);
-        else if (L.w32) pass1(bin_scatter64_kernel<float, false>, w32);
-        else if (compact) pass1(bin_scatter64_kernel<double, true>, c64);
-        else pass1(bin_scatter64_kernel<double, false>, c64);
+        if (derived) {
+            if (L.w32 && compact) pass1(bin_scatter64_kernel<float, true, true>, w32);
+            else if (L.w32) pass1(bin_scatter64_kernel<float, false, true>, w32);
+            else if (compact) pass1(bin_scatter64_kernel<double, true, true>, c64);
+            else pass1(bin_scatter64_kernel<double, false, true>, c64);
+        } else {
+            if (L.w32 && compact) pass1(bin_scatter64_kernel<float, true, false>, w32);
+            else if (L.w32) pass1(bin_scatter64_kernel<float, false, false>, w32);
+            else if (compact) pass1(bin_scatter64_kernel<double, true, false>, c64);
+            else pass1(bin_scatter64_kernel<double, false, false>, c64);
+        }
         if (compact)
             hipLaunchKernelGGL(bin_gather64_kernel<true>, g2, dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2,
                                L.dst.as<uint16_t>(), L.val.as<double>(), L.mp, L.part.as<long long>(), ctl, L.insum, e);

@@ -312,7 +312,7 @@ struct BinLayout {
     bool w32 = false;           // FP64 copy: 4-byte weights (x / deg staged in the table)
     uint64_t mp = 0;            // padded edges
     DevBuf order, pos, src, dst, coef, val, p1, p2, mb, part, ctl, partials;
-    // places derived per group of 64 quads (FP32 copy): segment-start mask, first segment,
+    // pass-1 places derived per group of 64 quads: segment-start mask, first segment,
     // chunk-major to bin-major offset of every non-empty segment; checked against pos
     DevBuf gmask, gfirst, sdelta;
     uint32_t nsdelta = 0;

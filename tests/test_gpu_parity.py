@@ -357,14 +357,17 @@ def test_mixed_binned_operator(monkeypatch, weighted, sinks, item):
 @pytest.mark.parametrize("weighted", [True, False])
 @pytest.mark.parametrize("item", ["0", "3000"])
 def test_binned_derived_places(monkeypatch, capfd, weighted, item):
-    """Pass 1 of the binned operator derives every quad's bin-major place from per-group data
-    (a segment-start mask and the first segment of each group of 64 quads, one offset per
-    segment) instead of reading the stored place.  The derivation is checked against the stored
-    places when the layout is built and is in use here; the operator then writes every
-    contribution to the same place, so on one graph handle (the same coefficients) the mixed
-    solve is bit-identical to the one that reads the stored places (NKPR_BINNED_POS=1), and
-    to a repeat of itself, and it certifies against the converged oracle."""
+    """Pass 1 of both binned copies (the FP32 operator and the FP64 sweep) derives every quad's
+    bin-major place from per-group data (a segment-start mask and the first segment of each
+    group of 64 quads, one offset per segment) instead of reading the stored place.  The
+    derivation is checked against the stored places when a layout is built and is in use here;
+    every contribution then goes to the same place, so on one graph handle (the same
+    coefficients) the mixed solve (FP32 operator, FP64 residual sweeps with 6-byte
+    contributions) and the FP64 power method are bit-identical to the solves that read the
+    stored places (NKPR_BINNED_POS=1) and to repeats of themselves, and they certify against
+    the converged oracle."""
     monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED64", "1")
     monkeypatch.setenv("NKPR_BINNED_ITEM", item)
     monkeypatch.setenv("NKPR_DEBUG", "1")
     z, u, v, w = skewed_graph(150000, 1200000, 67, weighted=weighted)
@@ -376,17 +379,24 @@ def test_binned_derived_places(monkeypatch, capfd, weighted, item):
     xs, stats = {}, {}
     for pos in ("0", "1", "0"):
         monkeypatch.setenv("NKPR_BINNED_POS", pos)
-        x = np.full(z, 1.0 / z)
-        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
-                                           stop=capi.NKPR_STOP_L1_ERROR))
-        assert st["l1_error_bound"] <= 1e-9
-        assert np.abs(x - ref).sum() <= 1e-9
-        if pos in xs:
-            assert np.array_equal(x, xs[pos])
-        xs[pos], stats[pos] = x, st
+        for prec, tol in ((capi.NKPR_MIXED, 1e-9), (capi.NKPR_FP64, 1e-8)):
+            x = np.full(z, 1.0 / z)
+            st = dg.run(x, capi.default_params(precision=prec, tol=tol,
+                                               stop=capi.NKPR_STOP_L1_ERROR))
+            assert st["l1_error_bound"] <= tol
+            assert np.abs(x - ref).sum() <= tol
+            key = (pos, prec)
+            if key in xs:
+                assert np.array_equal(x, xs[key])
+            xs[key], stats[key] = x, st
     err = capfd.readouterr().err
-    assert "binned FP32 places:" in err and "derived yes" in err
-    assert np.array_equal(xs["0"], xs["1"])
+    for copy in ("FP32", "FP64"):
+        assert ("binned %s places:" % copy) in err
+    assert "derived yes" in err and "derived no" not in err
+    for prec in (capi.NKPR_MIXED, capi.NKPR_FP64):
+        assert np.array_equal(xs[("0", prec)], xs[("1", prec)])
+        for key in ("refinements", "sweeps_fp32", "sweeps_fp64"):
+            assert stats[("0", prec)][key] == stats[("1", prec)][key]
     if not weighted:
         # a second handle has the same coefficients (weighted out-degrees are added with
         # atomics, in an order that can differ between handles in the last bit)
@@ -394,10 +404,7 @@ def test_binned_derived_places(monkeypatch, capfd, weighted, item):
         x = np.full(z, 1.0 / z)
         dg2.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
                                        stop=capi.NKPR_STOP_L1_ERROR))
-        assert np.array_equal(x, xs["0"])
-    for key in ("refinements", "sweeps_fp32", "sweeps_fp64"):
-        assert stats["0"][key] == stats["1"][key]
-
+        assert np.array_equal(x, xs[("0", capi.NKPR_MIXED)])
 
 
 def test_mixed_default_operator_on_large_graph(monkeypatch):
