@@ -10,7 +10,9 @@
 // source inside a segment:
 //   pass 1  per source chunk, segments in chunk-major order: x of the chunk is staged in LDS
 //           and every edge writes its contribution x[src] * w/deg(src), each segment's run to
-//           the segment's place in bin-major order;
+//           the segment's place in bin-major order (derived for every quad of 4 edges from a
+//           segment-start mask and the first segment of its group of 64 quads, and one offset
+//           per segment: about 0.1 byte per edge instead of a 1-byte stored place);
 //   pass 2  per destination bin, segments in bin-major order, so one contiguous stream per
 //           bin: the contributions are added into LDS accumulators of fixed-point integers,
 //           from which the rows are finished (a bin split over several work items: the items'
@@ -19,10 +21,10 @@
 // result independent of the order of the atomics (deterministic), and exact: each row is the
 // correctly scaled sum of its rounded products, rounded once.
 //   FP32 copy (chunks of 32768 ids, FP32 table 128 KB): 64-bit accumulators, the largest
-//       possible row sum just below 2^61 units; about 17 bytes per edge moved.
+//       possible row sum just below 2^61 units; about 16 bytes per edge moved.
 //   FP64 copy (chunks of 16384 ids, FP64 table 128 KB): 128-bit accumulators as two 64-bit
 //       integers (whole units and 2^-lo_shift fractions of a unit), so the rows carry more
-//       than FP64 precision; about 29 bytes per edge moved (25 when every weight is an FP32
+//       than FP64 precision; about 28 bytes per edge moved (24 when every weight is an FP32
 //       number and the copy keeps 4-byte weights).  The row epilogue, the reductions and the
 //       stopping rule are those of the pull sweep (nkpr_sweep.hip).
 #include <hip/hip_runtime.h>
@@ -372,8 +374,8 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
         fill_table<float, kSpan>(tab, v, inv, base, n, vmax);
         __syncthreads();
         // thread t handles quads q0 + t + j * kPassThreads, so every load of a wave is
-        // contiguous (8-byte source offsets, 16-byte coefficients, 4-byte places); kP1Quads
-        // quads per step, loaded one step ahead
+        // contiguous (8-byte source offsets, 16-byte coefficients, 4-byte places or the group
+        // data); kP1Quads quads per step, loaded one step ahead
         const uint32_t q1 = item.z / kQuad, qt = item.y / kQuad + threadIdx.x;
         constexpr uint32_t kStep = kP1Quads * kPassThreads;
         const f32x4 one = {1.f, 1.f, 1.f, 1.f};
