@@ -1200,7 +1200,7 @@ void persistent_grids(nkpr_graph *g, BinLayout &L, K1 k1, K2 k2) {
 }
 
 // The binned copies serve graphs with at least kBinnedMinEdges edges (on R-MAT scale 26 the
-// FP32 operator takes 3.4-3.8 ms per application against 8.4 ms for the pull sweep; on graphs
+// FP32 operator takes 3.2-3.5 ms per application against 8.4 ms for the pull sweep; on graphs
 // whose gathers stay in the caches the pull sweep is as fast and needs no second copy).
 // NKPR_BINNED=0 / 1 (FP32 operator) and NKPR_BINNED64=0 / 1 (FP64 sweep) force either way.
 constexpr uint64_t kBinnedMinEdges = 1ull << 26;
