@@ -317,6 +317,7 @@ struct BinLayout {
     DevBuf gmask, gfirst, sdelta;
     uint32_t nsdelta = 0;
     bool derived = false;
+    bool probed = false;        // placement of the contribution array chosen (FP32 copy)
     double insum = 0.0;         // largest in-coefficient sum of a row (fixed-point bound)
     uint32_t np1 = 0, np2 = 0, multi = 0, nmb = 0, nslots = 0, grid1 = 0, grid2 = 0;
 };

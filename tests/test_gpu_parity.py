@@ -407,6 +407,34 @@ def test_binned_derived_places(monkeypatch, capfd, weighted, item):
         assert np.array_equal(x, xs[("0", capi.NKPR_MIXED)])
 
 
+def test_binned_contribution_array_probe(monkeypatch, capfd):
+    """When the FP32 copy is built, candidate contribution arrays are allocated and the fastest
+    is kept (NKPR_VAL_PROBE candidates; by default only on large graphs).  The choice moves
+    data, not results: the mixed solve with three candidates is bit-identical to the one
+    without a probe (an unweighted graph, so that two handles hold the same coefficients)."""
+    monkeypatch.setenv("NKPR_BINNED", "1")
+    monkeypatch.setenv("NKPR_BINNED_ITEM", "3000")
+    monkeypatch.setenv("NKPR_DEBUG", "1")
+    z, u, v, _ = skewed_graph(120000, 900000, 73, weighted=False)
+    g = oracle_graph(z, u, v, None, True)
+    ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=0, max_iterations=5000)
+    xs = {}
+    for probe in ("3", "0"):
+        monkeypatch.setenv("NKPR_VAL_PROBE", probe)
+        capfd.readouterr()
+        dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), None)
+        x = np.full(z, 1.0 / z)
+        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
+                                           stop=capi.NKPR_STOP_L1_ERROR))
+        err = capfd.readouterr().err
+        assert ("contribution array candidate 2" in err) == (probe == "3")
+        assert st["l1_error_bound"] <= 1e-9
+        assert np.abs(x - ref).sum() <= 1e-9
+        xs[probe] = x
+        del dg
+    assert np.array_equal(xs["3"], xs["0"])
+
+
 def test_mixed_default_operator_on_large_graph(monkeypatch):
     """R-MAT scale 23 (about 1.3e8 edges, above the 2^26 edges where the GMRES corrections take
     the binned operator by default): the mixed solve certifies at 1e-9 and agrees with a
