@@ -1200,7 +1200,7 @@ void persistent_grids(nkpr_graph *g, BinLayout &L, K1 k1, K2 k2) {
 }
 
 // The binned copies serve graphs with at least kBinnedMinEdges edges (on R-MAT scale 26 the
-// FP32 operator takes 3.2-3.5 ms per application against 8.4 ms for the pull sweep; on graphs
+// FP32 operator takes 3.2-3.6 ms per application against 8.4 ms for the pull sweep; on graphs
 // whose gathers stay in the caches the pull sweep is as fast and needs no second copy).
 // NKPR_BINNED=0 / 1 (FP32 operator) and NKPR_BINNED64=0 / 1 (FP64 sweep) force either way.
 constexpr uint64_t kBinnedMinEdges = 1ull << 26;
@@ -1277,43 +1277,59 @@ void probe_val_placement(nkpr_graph *g, BinLayout &L) {
     const uint32_t n = g->n;
     if (k <= 1 || !n || !L.val.p) return;
     const hipStream_t s = g->stream;
-    std::vector<void *> cand{L.val.p};
-    for (int i = 1; i < k; ++i) {
+    // every allocation here may fail quietly: the probe is then skipped or cut short
+    auto grab = [](size_t bytes) -> void * {
         void *p = nullptr;
-        if (hipMalloc(&p, L.val.bytes) != hipSuccess) {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
-            break;
+            return nullptr;
         }
+        return p;
+    };
+    void *vb = grab(4 * (size_t)n), *wb = grab(4 * (size_t)n);
+    std::vector<void *> cand{L.val.p};
+    for (int i = 1; vb && wb && i < k; ++i) {
+        void *p = grab(L.val.bytes);
+        if (!p) break;
         cand.push_back(p);
     }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     size_t best = 0;
-    if (cand.size() > 1) {
-        DevBuf vb, wb;
-        vb.alloc(4 * (size_t)n);
-        wb.alloc(4 * (size_t)n);
-        NKPR_HIP(hipMemsetD32Async((hipDeviceptr_t)vb.p, 0x3f800000, n, s));  // v = 1
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        NKPR_HIP(hipEventCreate(&e0));
-        NKPR_HIP(hipEventCreate(&e1));
-        float best_ms = 0.f;
-        for (size_t i = 0; i < cand.size(); ++i) {
-            L.val.p = cand[i];
-            binned_apply(g, vb.as<float>(), wb.as<float>(), 0.85, false, false, nullptr, nullptr);
-            binned_apply(g, vb.as<float>(), wb.as<float>(), 0.85, false, false, e0, e1);
-            NKPR_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            NKPR_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (std::getenv("NKPR_DEBUG"))
-                std::fprintf(stderr, "[nkpr] contribution array candidate %zu: %.3f ms per application\n", i, ms);
-            if (i == 0 || ms < best_ms) best_ms = ms, best = i;
+    auto finish = [&] {
+        L.val.p = cand[best];
+        for (size_t i = 0; i < cand.size(); ++i)
+            if (i != best) (void)hipFree(cand[i]);
+        if (vb) (void)hipFree(vb);
+        if (wb) (void)hipFree(wb);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        if (cand.size() > 1) {
+            NKPR_HIP(hipMemsetD32Async((hipDeviceptr_t)vb, 0x3f800000, n, s));  // v = 1
+            NKPR_HIP(hipEventCreate(&e0));
+            NKPR_HIP(hipEventCreate(&e1));
+            float best_ms = 0.f;
+            for (size_t i = 0; i < cand.size(); ++i) {
+                L.val.p = cand[i];
+                binned_apply(g, (const float *)vb, (float *)wb, 0.85, false, false, nullptr, nullptr);
+                binned_apply(g, (const float *)vb, (float *)wb, 0.85, false, false, e0, e1);
+                NKPR_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                NKPR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (std::getenv("NKPR_DEBUG"))
+                    std::fprintf(stderr, "[nkpr] contribution array candidate %zu: %.3f ms per application\n", i, ms);
+                if (i == 0 || ms < best_ms) best_ms = ms, best = i;
+            }
+            NKPR_HIP(hipStreamSynchronize(s));
         }
-        NKPR_HIP(hipStreamSynchronize(s));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        best = 0;
+        finish();
+        throw;
     }
-    L.val.p = cand[best];
-    for (size_t i = 0; i < cand.size(); ++i)
-        if (i != best) (void)hipFree(cand[i]);
+    finish();
 }
 
 bool binned64_active(nkpr_graph *g) {
