@@ -60,7 +60,10 @@ extern "C" {
  * MIXED FP32 sweeps until the step stalls, then FP64 residual sweeps with FP32 correction
  *       solves (iterative refinement) until the certified L1 error of the sum-normalised
  *       scores is <= tol.  Always uses NKPR_STOP_L1_ERROR.
- * FP32  every sweep in FP32 (stops at the FP32 noise floor if tol is below it). */
+ * FP32  every sweep in FP32 (stops at the FP32 noise floor if tol is below it).  With
+ *       NKPR_STOP_L1_ERROR it stops when the formula of that rule, evaluated on the FP32
+ *       iterates, is <= tol; that is not a certificate (FP32 rounding is not part of it), and
+ *       the reported l1_error_bound is +inf. */
 #define NKPR_FP64 0
 #define NKPR_MIXED 1
 #define NKPR_FP32 2
@@ -132,7 +135,8 @@ typedef struct {
     uint64_t sweeps_fp32_correction; /* FP32 sweeps of correction solves (part of sweeps_fp32) */
     double last_step;         /* ||x' - x|| of the last sweep in params->norm              */
     double last_step_l1;
-    double l1_error_bound;    /* certified bound on the L1 error of the normalised scores  */
+    double l1_error_bound;    /* certified bound on the L1 error of the normalised scores
+                                 (FP64 sweeps); +inf for NKPR_FP32, which certifies nothing */
     double maximum;           /* PageRank::maximum()                                       */
     double score_sum;         /* sum of the unnormalised scores before normalisation       */
     double ms_sweeps_fp32;    /* summed sweep-kernel time (params->timing = 1)             */
@@ -201,8 +205,8 @@ typedef struct {
     uint64_t sweeps_fp64;
     uint64_t refinements;     /* FP64 residual sweeps followed by an FP32 correction solve */
     uint64_t max_iterations_col; /* largest per-column iteration count                     */
-    double max_error_bound;   /* largest per-column certified L1 bound (MIXED / L1_ERROR) or
-                                 last step in `norm` (STEP)                                */
+    double max_error_bound;   /* largest per-column certified L1 bound (MIXED / L1_ERROR;
+                                 +inf for FP32) or last step in `norm` (STEP)              */
     double ms_sweeps_fp32;    /* summed SpMM kernel time (params->timing = 1)              */
     double ms_sweeps_fp64;
     double ms_total;

@@ -39,6 +39,10 @@ class PageRankOperator:
         self.max_iterations = 100000
         self.last_forward = None
         self.last_backward = None
+        # the device handle holds one set of weights: the version of the forward call that
+        # loaded them, so that a backward of an earlier call can load its own weights again
+        self._calls = 0
+        self._loaded = -1
 
     def _set_weights(self, weight):
         if weight.numel() != self.m:
@@ -52,7 +56,18 @@ class PageRankOperator:
             self._w64 = None
         torch.cuda.current_stream(self.device).synchronize()
         self.graph.set_weights(self._w32, self._w64)
-        return self._w32, self._w64
+        self._calls += 1
+        self._loaded = self._calls
+        return self._w32, self._w64, self._loaded
+
+    def _ensure_weights(self, w32, w64, version):
+        """Load the weights of forward call `version` again when a later call replaced them:
+        the adjoint's transposed operator is refolded from the handle's current weights."""
+        if self._loaded == version:
+            return
+        torch.cuda.current_stream(self.device).synchronize()
+        self.graph.set_weights(w32, w64)
+        self._loaded = version
 
     def params(self, tol, **kw):
         stop = capi.NKPR_STOP_STEP if self.precision == capi.NKPR_FP64 else capi.NKPR_STOP_L1_ERROR
@@ -67,7 +82,7 @@ class PageRankOperator:
 class PageRankFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, weight, teleport, op):
-        w32, w64 = op._set_weights(weight)
+        w32, w64, version = op._set_weights(weight)
         x = torch.full((op.z,), 1.0 / op.z, dtype=torch.float64, device=op.device)
         t = None
         if teleport is not None:
@@ -82,6 +97,7 @@ class PageRankFunction(torch.autograd.Function):
         ctx.score_sum = st["score_sum"]
         ctx.has_teleport = teleport is not None
         ctx.w = (w32, w64)
+        ctx.version = version
         ctx.save_for_backward(x)
         return x.to(weight.dtype if weight.dtype.is_floating_point else torch.float64)
 
@@ -90,6 +106,7 @@ class PageRankFunction(torch.autograd.Function):
         (x,) = ctx.saved_tensors
         op = ctx.op
         w32, w64 = ctx.w
+        op._ensure_weights(w32, w64, ctx.version)
         c = grad_x.detach().to(torch.float64).contiguous()
         gw = torch.empty(max(op.m, 1), dtype=torch.float64, device=op.device)
         gt = torch.empty(op.z, dtype=torch.float64, device=op.device)

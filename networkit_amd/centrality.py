@@ -1,8 +1,9 @@
 """networkit.centrality equivalent for the PageRank path.
 
 PageRank, Centrality, Norm (networkit/centrality.pyx:24-100, 1970-2037) come from the Cython
-extension; rankPerNode and relativeRankErrors restate the pure-Python helpers of
-networkit/centrality.pyx:2197-2249.
+extension.  rankPerNode and relativeRankErrors provide the two pure-Python ranking helpers of
+networkit/centrality.pyx:2197-2249 with the same tie rule; unlike those, they accept rankings
+whose node ids have gaps (graphs with deleted nodes).
 """
 from ._core import Centrality, Norm, PageRank, Precision, SinkHandling  # noqa: F401
 
@@ -11,30 +12,38 @@ __all__ = ["Centrality", "Norm", "PageRank", "Precision", "SinkHandling", "rankP
 
 
 def rankPerNode(ranking):
-    """For each node id, its position in ``ranking``; nodes with equal scores share the
-    position of the first of them (centrality.pyx:2197-2221)."""
-    n_nodes = len(ranking)
-    ranking_id = [0] * n_nodes
-    for index, pair in enumerate(ranking):
-        ranking_id[pair[0]] = index
-    for index, pair in enumerate(ranking):
-        if index == 0:
-            continue
-        if pair[1] == ranking[index - 1][1]:
-            prev_node = ranking[index - 1][0]
-            ranking_id[pair[0]] = ranking_id[prev_node]
-    return ranking_id
+    """Position of every node in ``ranking`` (a list of (node, score) pairs, best first).
+
+    A maximal run of consecutive entries with equal scores shares the position of its first
+    entry, the rule of centrality.pyx:2197-2221.  The result is indexed by node id and has
+    max(len(ranking), largest id + 1) entries; an id that the ranking does not list (a deleted
+    node) holds None."""
+    size = len(ranking)
+    for node, _ in ranking:
+        size = max(size, int(node) + 1)
+    position = [None] * size
+    run_first = 0
+    previous = None
+    for i, (node, score) in enumerate(ranking):
+        # NaN != NaN, so a NaN score never joins a run (as with the reference's == test)
+        if i == 0 or not (score == previous):
+            run_first = i
+        position[int(node)] = run_first
+        previous = score
+    return position
 
 
 def relativeRankErrors(rx, ry):
-    """(r_x(u)+1) / (r_y(u)+1) per node id; empty if the rankings differ in length
-    (centrality.pyx:2223-2249)."""
-    diff = []
-    n = len(rx)
-    if not (n == len(ry)):
-        return diff
-    rnode_x = rankPerNode(rx)
-    rnode_y = rankPerNode(ry)
-    for i in range(n):
-        diff.append((rnode_x[i] + 1) / (rnode_y[i] + 1))
-    return diff
+    """(position in rx + 1) / (position in ry + 1) per node id, from rankPerNode of both
+    rankings (centrality.pyx:2223-2249); an empty list when the rankings differ in length.
+    Ids that neither ranking lists hold None."""
+    if len(rx) != len(ry):
+        return []
+    px, py = rankPerNode(rx), rankPerNode(ry)
+    size = max(len(px), len(py))
+    px += [None] * (size - len(px))
+    py += [None] * (size - len(py))
+    out = []
+    for a, b in zip(px, py):
+        out.append(None if a is None or b is None else (a + 1) / (b + 1))
+    return out

@@ -115,6 +115,7 @@ edgeweight Graph::weight(node u, node v) const {
 }
 
 edgeweight Graph::weightedDegree(node u, bool countSelfLoopsTwice) const {
+    checked(u);
     if (weighted) {
         edgeweight sum = 0.0;
         const auto &lst = outEdges[u];

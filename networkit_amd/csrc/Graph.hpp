@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <limits>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace NetworKit {
@@ -44,9 +45,13 @@ public:
     bool isWeighted() const noexcept { return weighted; }
     bool isDirected() const noexcept { return directed; }
 
-    count degree(node v) const { return outEdges[v].size(); }
-    count degreeOut(node v) const { return outEdges[v].size(); }
-    count degreeIn(node v) const { return directed ? inEdges[v].size() : outEdges[v].size(); }
+    // The accessors taking a node id throw std::out_of_range for an id >= upperNodeIdBound()
+    // (NetworKit leaves that undefined; the Python layer turns it into IndexError).
+    count degree(node v) const { return outEdges[checked(v)].size(); }
+    count degreeOut(node v) const { return outEdges[checked(v)].size(); }
+    count degreeIn(node v) const {
+        return directed ? inEdges[checked(v)].size() : outEdges[checked(v)].size();
+    }
     // Graph::computeWeightedDegree(u, inDegree = false, countSelfLoopsTwice), Graph.cpp:427-455
     edgeweight weightedDegree(node u, bool countSelfLoopsTwice = false) const;
 
@@ -65,6 +70,11 @@ public:
     uint64_t version() const noexcept { return ver; }
 
 private:
+    node checked(node v) const {
+        if (v >= z) throw std::out_of_range("node id " + std::to_string(v) + " >= " +
+                                            std::to_string(z));
+        return v;
+    }
     count n, m, z, selfLoops;
     bool weighted, directed;
     uint64_t ver = 0;

@@ -169,8 +169,7 @@ double device_dot(const double *a, const double *b, uint64_t n, double *part, do
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(64), 0, s, (const double *)part, nb, out);
     NKPR_HIP(hipGetLastError());
     double h = 0.0;
-    NKPR_HIP(hipMemcpyAsync(&h, out, 8, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{&h, out, 8}});
     return h;
 }
 
@@ -388,8 +387,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
         auto norm2 = [&](const float *v) {
             double h = 0.0;
             launch_multidot(v, v, kr.ld, 1, n, kr.dpart, kr.dout, s);
-            NKPR_HIP(hipMemcpyAsync(&h, kr.dout, 8, hipMemcpyDeviceToHost, s));
-            NKPR_HIP(hipStreamSynchronize(s));
+            read_back(s, {{&h, kr.dout, 8}});
             return std::sqrt(h);
         };
         const double g2 = norm2(r);
@@ -474,7 +472,8 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
         } else {
             hipLaunchKernelGGL(scale_kernel, dim3(nblocks(z)), dim3(256), 0, s, (const double *)lam_z, z,
                                1.0 - d, part);
-            NKPR_HIP(hipMemcpyAsync(grad_teleport, part, 8 * z, hipMemcpyDeviceToHost, s));
+            NKPR_HIP(hipGetLastError());
+            read_back(s, {{grad_teleport, part, 8 * z}});
         }
         NKPR_HIP(hipGetLastError());
     }
@@ -499,7 +498,8 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
                            dim3(256), 0, s, colptr, src, z, x_z, score_sum, (const double *)deg,
                            (const double *)lam_z, (const double *)mu_z, d, gout);
         NKPR_HIP(hipGetLastError());
-        if (!edev) NKPR_HIP(hipMemcpyAsync(grad_weight, gout, 8 * m, hipMemcpyDeviceToHost, s));
+        // the staging buffer gw is freed at the end of this block: wait for the copy first
+        if (!edev) read_back(s, {{grad_weight, gout, 8 * m}});
     }
     NKPR_HIP(hipStreamSynchronize(s));
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

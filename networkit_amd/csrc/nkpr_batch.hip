@@ -36,6 +36,12 @@ void ensure_buf(DevBuf &b, size_t bytes) { b.alloc(bytes); }
 
 namespace {
 
+uint32_t env_u32_batch(const char *name, uint32_t dflt) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return (uint32_t)std::strtoul(v, nullptr, 10);
+}
+
 constexpr int kBThreads = 512;
 constexpr int kUnroll = 8;
 constexpr int kFinThreads = 256;
@@ -722,12 +728,9 @@ void batch_layout(nkpr_graph *g) {
         NKPR_HIP(hipGetLastError());
     }
     g->b_rowptr_host.resize((size_t)n + 1);
-    NKPR_HIP(hipMemcpyAsync(g->b_rowptr_host.data(), g->b_rowptr.p, 4 * ((size_t)n + 1),
-                            hipMemcpyDeviceToHost, s));
     g->b_perm_host.resize(n);
-    NKPR_HIP(hipMemcpyAsync(g->b_perm_host.data(), g->perm.p, 4 * (size_t)n,
-                            hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{g->b_rowptr_host.data(), g->b_rowptr.p, 4 * ((size_t)n + 1)},
+                  {g->b_perm_host.data(), g->perm.p, 4 * (size_t)n}});
     g->b_ready = true;
     g->b_coef_ready = true;
 }
@@ -892,8 +895,7 @@ uint64_t batch_phase(BatchRun &br, T *x0, T *x1, int kind, PhaseRule rule, uint6
                                br.k, br.ctl, (uint64_t)max_launch);
             NKPR_HIP(hipGetLastError());
         }
-        NKPR_HIP(hipMemcpyAsync(br.ctl_host, br.ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{br.ctl_host, br.ctl, sizeof(BatchCtl)}});
         const uint64_t it = br.ctl_host->sweeps;
         if (timing) {
             for (uint64_t j = 0; j < it - done_sweeps && j < nb; ++j) {
@@ -919,18 +921,32 @@ void autotune(BatchRun &br, T *x0, T *x1) {
     const hipStream_t s = g->stream;
     constexpr int V = V16<T>::n;
     const uint32_t key = br.ld * 2 + (dbl ? 1 : 0);
+    // NKPR_PPR_LPR (4, 16 or 64 lanes per row) and NKPR_PPR_HEAVY (a split threshold >= 64)
+    // restrict the candidates to the given value: a forced configuration is neither looked up
+    // nor remembered
+    const uint32_t force_lpr = env_u32_batch("NKPR_PPR_LPR", 0);
+    const uint32_t force_heavy = env_u32_batch("NKPR_PPR_HEAVY", 0);
+    NKPR_CHECK(force_lpr == 0 || force_lpr == 4 || force_lpr == 16 || force_lpr == 64,
+               "NKPR_PPR_LPR must be 4, 16 or 64");
+    NKPR_CHECK(force_heavy == 0 || force_heavy >= 64, "NKPR_PPR_HEAVY must be >= 64");
+    const bool forced = force_lpr != 0 || force_heavy != 0;
     for (const auto &t : g->b_tuned) {
-        if (t.key == key) {
+        if (t.key == key && !forced) {
             br.lpr[dbl] = t.lpr;
             br.heavy[dbl] = t.heavy;
             return;
         }
     }
     std::vector<int> lprs;
-    for (int l : {64, 16, 4})  // keep tiles at least half full
-        if (l == 4 || (uint32_t)(l * V) <= 2 * br.ld) lprs.push_back(l);
+    if (force_lpr) {
+        lprs.push_back((int)force_lpr);
+    } else {
+        for (int l : {64, 16, 4})  // keep tiles at least half full
+            if (l == 4 || (uint32_t)(l * V) <= 2 * br.ld) lprs.push_back(l);
+    }
     // heavier rows than this leave one row group running alone for too long
-    const std::vector<uint32_t> heavies = {256, 1024, 4096};
+    std::vector<uint32_t> heavies = {256, 1024, 4096};
+    if (force_heavy) heavies = {force_heavy};
     hipEvent_t e0, e1;
     NKPR_HIP(hipEventCreate(&e0));
     NKPR_HIP(hipEventCreate(&e1));
@@ -969,7 +985,7 @@ void autotune(BatchRun &br, T *x0, T *x1) {
     (void)hipEventDestroy(e1);
     br.lpr[dbl] = best_l;
     br.heavy[dbl] = best_h;
-    g->b_tuned.push_back({key, best_l, best_h});
+    if (!forced) g->b_tuned.push_back({key, best_l, best_h});
 }
 
 template <typename T> void configure(BatchRun &br, T *x0, T *x1) {
@@ -1111,8 +1127,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
     auto debug_phase = [&](const char *name, uint64_t sweeps) {
         if (!debug) return;
         std::vector<BatchCol> c(k);
-        NKPR_HIP(hipMemcpyAsync(c.data(), br.col, sizeof(BatchCol) * k, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{c.data(), br.col, sizeof(BatchCol) * k}});
         double mb = 0.0, ml1 = 0.0;
         uint32_t fin = 0;
         for (const auto &x : c) {
@@ -1189,10 +1204,9 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
                 debug_phase("fp64 residual", 1);
                 st->sweeps_fp64 += 1;
                 result = x64[1];
-                NKPR_HIP(hipMemcpyAsync(br.ctl_host, br.ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
                 std::vector<BatchCol> cols(k);
-                NKPR_HIP(hipMemcpyAsync(cols.data(), br.col, sizeof(BatchCol) * k, hipMemcpyDeviceToHost, s));
-                NKPR_HIP(hipStreamSynchronize(s));
+                read_back(s, {{br.ctl_host, br.ctl, sizeof(BatchCtl)},
+                              {cols.data(), br.col, sizeof(BatchCol) * k}});
                 uint32_t open = 0;
                 for (uint32_t j = 0; j < k; ++j) open += cols[j].final_ ? 0u : 1u;
                 if (open == 0 || total >= max_it || round >= 30) break;
@@ -1239,15 +1253,15 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
 
     // ---- per-column normalisation by the column sum, scatter to [k][z] ------------------------
     std::vector<BatchCol> cols(k);
-    NKPR_HIP(hipMemcpyAsync(cols.data(), br.col, sizeof(BatchCol) * k, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{cols.data(), br.col, sizeof(BatchCol) * k}});
     std::vector<double> factor(k);
     for (uint32_t j = 0; j < k; ++j) {
         NKPR_CHECK(cols[j].sum > 0.0, "a column's scores sum to zero: cannot normalise");
         factor[j] = cols[j].sum;
         st->max_iterations_col = std::max<uint64_t>(st->max_iterations_col, cols[j].iterations);
+        // an FP32 run certifies nothing (its iterates carry rounding the step does not see)
         const double b = (p->precision == NKPR_MIXED || p->stop == NKPR_STOP_L1_ERROR)
-                             ? cols[j].bound
+                             ? (p->precision == NKPR_FP32 ? INFINITY : cols[j].bound)
                              : (p->norm == NKPR_L2 ? std::sqrt(cols[j].l2) : cols[j].l1);
         st->max_error_bound = std::max(st->max_error_bound, b);
         if (col_iterations) col_iterations[j] = cols[j].iterations;
@@ -1260,9 +1274,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
                        dim3(256), 0, s, (const double *)result, (const int32_t *)g->b_slot_u.as<int32_t>(), k, ld, z,
                        g->b_factor.as<double>(), sz);
     NKPR_HIP(hipGetLastError());
-    if (!p->scores_on_device)
-        NKPR_HIP(hipMemcpyAsync(scores, sz, 8 * (size_t)z * k, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{scores, sz, p->scores_on_device ? 0 : 8 * (size_t)z * k}});
     st->tile_columns = (uint32_t)(br.lpr[p->precision == NKPR_FP64 ? 1 : 0] * (p->precision == NKPR_FP64 ? 2 : 4));
     st->heavy_degree = br.heavy[p->precision == NKPR_FP64 ? 1 : 0];
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

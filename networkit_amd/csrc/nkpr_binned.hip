@@ -882,8 +882,7 @@ void binned_insum(nkpr_graph *g, BinLayout &L) {
                            g->weighted ? nullptr : g->inv32.as<float>(), best.as<unsigned long long>());
     NKPR_HIP(hipGetLastError());
     unsigned long long hb = 0;
-    NKPR_HIP(hipMemcpyAsync(&hb, best.p, 8, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{&hb, best.p, 8}});
     double v;
     std::memcpy(&v, &hb, 8);
     L.insum = v * (L.fp64 ? 1.0 + 1e-9 : 1.01);
@@ -977,8 +976,7 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
         hipLaunchKernelGGL(bin_seg_kernel, dim3(nb(nseg + 1)), dim3(256), 0, s,
                            (const uint64_t *)k1.as<uint64_t>(), m, nseg, ss, rseg.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
-        NKPR_HIP(hipMemcpyAsync(seg.data(), rseg.p, 4 * (nseg + 1), hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{seg.data(), rseg.p, 4 * (nseg + 1)}});
     }
     // padded segment starts: every segment rounded up to whole quads, laid out chunk-major
     // (pass 1) and bin-major (pass 2); both indexed by the chunk-major segment number
@@ -1061,8 +1059,7 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
                            L.nsdelta, chk.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
         uint32_t bad = 1;
-        NKPR_HIP(hipMemcpyAsync(&bad, chk.p, 4, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{&bad, chk.p, 4}});
         L.derived = bad == 0;
         if (!L.derived) {
             L.gmask.reset();
@@ -1165,7 +1162,7 @@ void release_layout(BinLayout &L) {
     for (DevBuf *b : {&L.order, &L.pos, &L.src, &L.dst, &L.coef, &L.val, &L.p1, &L.p2, &L.mb,
                       &L.part, &L.ctl, &L.partials, &L.gmask, &L.gfirst, &L.sdelta})
         b->reset();
-    L.ready = L.coef_ready = L.derived = L.probed = false;
+    L.ready = L.coef_ready = L.derived = false;
 }
 
 // Builds L if needed.  False when it cannot be built (too many edges for 32-bit positions, or
@@ -1214,17 +1211,10 @@ bool policy(const nkpr_graph *g, const char *env) {
 
 bool binned_enabled(const nkpr_graph *g) { return !g->bn32.off && policy(g, "NKPR_BINNED"); }
 
-void probe_val_placement(nkpr_graph *g, BinLayout &L);
-
 bool binned_layout(nkpr_graph *g) {
     g->bn32.span_shift = kSpan32Shift;
     g->bn32.fp64 = false;
-    if (!layout_ready(g, g->bn32)) return false;
-    if (!g->bn32.probed) {
-        g->bn32.probed = true;
-        probe_val_placement(g, g->bn32);
-    }
-    return true;
+    return layout_ready(g, g->bn32);
 }
 
 // w = B v = d P v (+ d S(v)/n) through the binned passes, or w = A v = v - B v (minus).
@@ -1263,73 +1253,6 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
                            (const BinCtl *)ctl, L.insum, v, minus, (float)damp, sink_sum, w);
     if (ev1) NKPR_HIP(hipEventRecord(ev1, s));
     NKPR_HIP(hipGetLastError());
-}
-
-// The operator's time depends on where its contribution array lies in device memory: on R-MAT
-// scale 26, sixteen allocations of the same array took 3.19-3.40 ms per application, each
-// repeatable within 0.01 ms.  So when the FP32 copy of a large graph is built, a few candidate
-// arrays are allocated, one application on a constant vector is timed with each, and the
-// fastest is kept.  NKPR_VAL_PROBE=<k> sets the number of candidates (0 or 1: no probe; by
-// default 3 from 2^26 padded edges up).  A candidate that cannot be allocated ends the probe.
-void probe_val_placement(nkpr_graph *g, BinLayout &L) {
-    int k = L.mp >= kBinnedMinEdges ? 3 : 1;
-    if (const char *e = std::getenv("NKPR_VAL_PROBE"); e && *e) k = std::atoi(e);
-    const uint32_t n = g->n;
-    if (k <= 1 || !n || !L.val.p) return;
-    const hipStream_t s = g->stream;
-    // every allocation here may fail quietly: the probe is then skipped or cut short
-    auto grab = [](size_t bytes) -> void * {
-        void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        return p;
-    };
-    void *vb = grab(4 * (size_t)n), *wb = grab(4 * (size_t)n);
-    std::vector<void *> cand{L.val.p};
-    for (int i = 1; vb && wb && i < k; ++i) {
-        void *p = grab(L.val.bytes);
-        if (!p) break;
-        cand.push_back(p);
-    }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    size_t best = 0;
-    auto finish = [&] {
-        L.val.p = cand[best];
-        for (size_t i = 0; i < cand.size(); ++i)
-            if (i != best) (void)hipFree(cand[i]);
-        if (vb) (void)hipFree(vb);
-        if (wb) (void)hipFree(wb);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-    };
-    try {
-        if (cand.size() > 1) {
-            NKPR_HIP(hipMemsetD32Async((hipDeviceptr_t)vb, 0x3f800000, n, s));  // v = 1
-            NKPR_HIP(hipEventCreate(&e0));
-            NKPR_HIP(hipEventCreate(&e1));
-            float best_ms = 0.f;
-            for (size_t i = 0; i < cand.size(); ++i) {
-                L.val.p = cand[i];
-                binned_apply(g, (const float *)vb, (float *)wb, 0.85, false, false, nullptr, nullptr);
-                binned_apply(g, (const float *)vb, (float *)wb, 0.85, false, false, e0, e1);
-                NKPR_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                NKPR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (std::getenv("NKPR_DEBUG"))
-                    std::fprintf(stderr, "[nkpr] contribution array candidate %zu: %.3f ms per application\n", i, ms);
-                if (i == 0 || ms < best_ms) best_ms = ms, best = i;
-            }
-            NKPR_HIP(hipStreamSynchronize(s));
-        }
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        best = 0;
-        finish();
-        throw;
-    }
-    finish();
 }
 
 bool binned64_active(nkpr_graph *g) {

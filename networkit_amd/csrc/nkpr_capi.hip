@@ -184,7 +184,9 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         res.sink = g->ctl_host->sink_out;
         st->last_step_l1 = g->ctl_host->l1;
         st->last_step = (p->norm == NKPR_L2) ? std::sqrt(g->ctl_host->l2) : g->ctl_host->l1;
-        st->l1_error_bound = rel_error_bound(damp, g->ctl_host->l1, g->ctl_host->sum);
+        // FP32 iterates carry rounding errors that the step does not see: an FP32 run may stop
+        // on the same formula, but it certifies nothing
+        st->l1_error_bound = dbl ? rel_error_bound(damp, g->ctl_host->l1, g->ctl_host->sum) : INFINITY;
     } else {
         // ---- mixed precision: FP64 residual sweeps certify, FP32 GMRES solves corrections ----
         // With r = b - A x from an FP64 sweep (A = I - d P, P with the sink term when sinks
@@ -219,9 +221,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             uint32_t *flag = g->flag_buf.as<uint32_t>();
             launch_constant_check(xcur, n, flag, s);
             uint32_t hf = 1;
-            NKPR_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, s));
-            NKPR_HIP(hipMemcpyAsync(&cval, xcur, 8, hipMemcpyDeviceToHost, s));
-            NKPR_HIP(hipStreamSynchronize(s));
+            read_back(s, {{&hf, flag, 4}, {&cval, xcur, 8}});
             closed = hf == 0;
         }
         if (closed) ensure_rowsum64(g);
@@ -252,8 +252,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
                 ff.n_blocks = blocks;
                 ff.rule = rr;
                 launch_finalize64(fa, ff, kind_resid, s);
-                NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost, s));
-                NKPR_HIP(hipStreamSynchronize(s));
+                read_back(s, {{g->ctl_host, g->ctl.p, sizeof(PhaseCtl)}});
             } else {
                 if (!folded) launch_prescale(xcur, g->inv64.as<double>(), xh, nullptr, nullptr, nullptr, n, g->sink_begin, s);
                 init_phase(xcur, true);
@@ -458,14 +457,11 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
     if (p->normalized) {
         launch_range_max(res.x, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);
         double mx = 0.0, s0 = 0.0;
-        NKPR_HIP(hipMemcpyAsync(&mx, g->red_out.p, 8, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipMemcpyAsync(&s0, sz, 8, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{&mx, g->red_out.p, 8}, {&s0, sz, 8}});
         maximum = std::max(mx / factor, s0);
     }
     st->maximum = maximum;
-    if (!p->scores_on_device) NKPR_HIP(hipMemcpyAsync(scores, sz, 8 * z, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{scores, sz, p->scores_on_device ? 0 : 8 * z}});
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -632,9 +628,15 @@ int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_
         ids.reserve(z);
         for (uint64_t u = 0; u < z; ++u)
             if (!exists || exists[u]) ids.push_back((int64_t)u);
-        std::stable_sort(ids.begin(), ids.end(), [&](int64_t a, int64_t b) {
-            if (scores[a] == scores[b]) return a < b;
-            return scores[a] > scores[b];
+        // Centrality.cpp:33-39 orders by score descending, ties by id ascending.  Its comparator
+        // is not a strict weak order when a score is NaN, which lets a sort read outside the
+        // range; here NaN scores rank after every number (ties by id), so the order is total.
+        std::sort(ids.begin(), ids.end(), [&](int64_t a, int64_t b) {
+            const double sa = scores[a], sb = scores[b];
+            const bool na = std::isnan(sa), nb = std::isnan(sb);
+            if (na || nb) return na != nb ? nb : a < b;
+            if (sa == sb) return a < b;
+            return sa > sb;
         });
         std::copy(ids.begin(), ids.end(), ids_out);
         if (scores_out)

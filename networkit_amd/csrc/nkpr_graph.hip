@@ -470,7 +470,7 @@ void build_schedule(nkpr_graph *g, const std::vector<uint64_t> &cp) {
     build_items(dptr.data(), g->n_d, 0, ditems, split_slots, nullptr, maxlen);
     std::map<uint32_t, SplitRec> splits;
     std::vector<uint32_t> hs((size_t)ns_tot + 1);
-    NKPR_HIP(hipMemcpy(hs.data(), g->sptr.p, 4 * (ns_tot + 1), hipMemcpyDeviceToHost));
+    read_back(s, {{hs.data(), g->sptr.p, 4 * ((size_t)ns_tot + 1)}});
     for (uint32_t q = 0; q < S; ++q) {
         g->sitem_off[q] = (uint32_t)sitems.size();
         build_items(hs.data() + (size_t)q * n_s, n_s, q, sitems, split_slots, &splits, maxlen);
@@ -592,8 +592,7 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
     }
     NKPR_HIP(hipGetLastError());
     uint32_t hflags = 0;
-    NKPR_HIP(hipMemcpyAsync(&hflags, flags, 4, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{&hflags, flags, 4}});
     NKPR_CHECK(!(hflags & kBadPtr), "colptr is not a non-decreasing offset array ending at m");
     NKPR_CHECK(!(hflags & kBadSrc), "an in-list names a source outside [0, z) or a deleted node");
     NKPR_CHECK(!(hflags & kBadExists), "a deleted slot has in-edges");
@@ -606,8 +605,7 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
         hipLaunchKernelGGL(count_kernel, dim3(blocks_for(z)), dim3(256), 0, s,
                            outcnt_b.as<uint32_t>(), exists, z, counts, counts + 1);
     unsigned long long hc[2] = {0, 0};
-    NKPR_HIP(hipMemcpyAsync(hc, counts, 16, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{hc, counts, 16}});
     const uint32_t n = (uint32_t)hc[0];
     g->n = n;
     g->sink_begin = (uint32_t)hc[1];
@@ -658,10 +656,9 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
         }
         std::vector<uint32_t> he(n_groups), hn(n_groups);
         uint32_t hf = 0;
-        NKPR_HIP(hipMemcpyAsync(he.data(), ce.p, 4 * (uint64_t)n_groups, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipMemcpyAsync(hn.data(), cn.p, 4 * (uint64_t)n_groups, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipMemcpyAsync(&hf, fl.p, 4, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{he.data(), ce.p, 4 * (uint64_t)n_groups},
+                      {hn.data(), cn.p, 4 * (uint64_t)n_groups},
+                      {&hf, fl.p, 4}});
         NKPR_CHECK(!(hf & kBadGroup), "an edge joins two graphs of the batch");
         g->n_groups = n_groups;
         g->goff.assign((size_t)n_groups + 1, 0);
@@ -782,16 +779,14 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
                                w_in, w64_in, m, flags_b.as<uint32_t>());
             NKPR_HIP(hipGetLastError());
             uint32_t hf = 0;
-            NKPR_HIP(hipMemcpyAsync(&hf, flags_b.p, 4, hipMemcpyDeviceToHost, s));
-            NKPR_HIP(hipStreamSynchronize(s));
+            read_back(s, {{&hf, flags_b.p, 4}});
             g->w32_exact = hf == 0;
         }
     }
 
     // --- schedule ---------------------------------------------------------------------------
     std::vector<uint64_t> cp((size_t)n + 1);
-    NKPR_HIP(hipMemcpyAsync(cp.data(), g->colptr.p, 8 * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{cp.data(), g->colptr.p, 8 * ((size_t)n + 1)}});
     build_schedule(g, cp);
     NKPR_HIP(hipStreamSynchronize(s));
     g->prep_ms =
@@ -890,8 +885,7 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
                            w, w64, m, flag.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
         uint32_t hf = 0;
-        NKPR_HIP(hipMemcpyAsync(&hf, flag.p, 4, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{&hf, flag.p, 4}});
         exact = hf == 0;
     } else if (!w) {
         exact = false;  // FP64 weights alone: not checked

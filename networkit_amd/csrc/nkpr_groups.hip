@@ -449,8 +449,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                                    (const uint32_t *)q.cfirst, (const double *)xcur, flag);
                 NKPR_HIP(hipGetLastError());
                 uint32_t hf = 1;
-                NKPR_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, s));
-                NKPR_HIP(hipStreamSynchronize(s));
+                read_back(s, {{&hf, flag, 4}});
                 closed = hf == 0;
             }
             if (closed) ensure_rowsum64(g);
@@ -477,8 +476,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                     st->sweeps_fp64 += 1;
                 }
                 result = ynew;
-                NKPR_HIP(hipMemcpyAsync(hc.data(), q.gc, sizeof(GroupCtl) * G, hipMemcpyDeviceToHost, s));
-                NKPR_HIP(hipStreamSynchronize(s));
+                read_back(s, {{hc.data(), q.gc, sizeof(GroupCtl) * G}});
                 // the correction must bring every graph's residual to its own target: the ratio
                 // of the worst graph sets the relative tolerance of the union's solve
                 double worst = 0.0;
@@ -501,8 +499,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
             }
         }
         st->iterations = total;
-        NKPR_HIP(hipMemcpyAsync(hc.data(), q.gc, sizeof(GroupCtl) * G, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{hc.data(), q.gc, sizeof(GroupCtl) * G}});
 
         // ---- per-graph normalisation (PageRank.cpp:73-92) and scatter --------------------------
         std::vector<double> factor(G, 1.0);
@@ -513,7 +510,8 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
             const double b = (p->precision == NKPR_MIXED || p->stop == NKPR_STOP_L1_ERROR)
                                  ? hc[k].bound
                                  : (p->norm == NKPR_L2 ? std::sqrt(hc[k].l2) : hc[k].l1);
-            st->l1_error_bound = std::max(st->l1_error_bound, hc[k].bound);
+            // an FP32 run certifies nothing (its iterates carry rounding the step does not see)
+            st->l1_error_bound = p->precision == NKPR_FP32 ? INFINITY : std::max(st->l1_error_bound, hc[k].bound);
             st->last_step = std::max(st->last_step, b);
         }
         double *fdev = g->q_factor.as<double>();
@@ -523,19 +521,18 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                            result, g->perm.as<int32_t>(), (const double *)fdev, sz, mdev);
         NKPR_HIP(hipGetLastError());
         std::vector<double> cmax(q.nch);
-        NKPR_HIP(hipMemcpyAsync(cmax.data(), mdev, 8 * (size_t)q.nch, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{cmax.data(), mdev, 8 * (size_t)q.nch}});
         if (maximum) {
             // maximum(): 1 when not normalised; otherwise the largest score, seeded with the
             // graph's first slot (PageRank.cpp:95-102)
             std::vector<Chunk> ch(q.nch);
-            NKPR_HIP(hipMemcpy(ch.data(), q.chunks, sizeof(Chunk) * q.nch, hipMemcpyDeviceToHost));
+            read_back(s, {{ch.data(), q.chunks, sizeof(Chunk) * q.nch}});
             std::vector<double> first(G, 0.0);
+            std::vector<HostCopy> reads;
             for (uint32_t k = 0; k < G; ++k) {
-                if (g->gslot[k] < g->gslot[k + 1])
-                    NKPR_HIP(hipMemcpyAsync(&first[k], sz + g->gslot[k], 8, hipMemcpyDeviceToHost, s));
+                if (g->gslot[k] < g->gslot[k + 1]) reads.push_back({&first[k], sz + g->gslot[k], 8});
             }
-            NKPR_HIP(hipStreamSynchronize(s));
+            read_back(s, reads.data(), reads.size());
             for (uint32_t k = 0; k < G; ++k) maximum[k] = p->normalized ? first[k] : 1.0;
             if (p->normalized)
                 for (uint32_t c = 0; c < q.nch; ++c)
@@ -547,8 +544,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
     if (iterations)
         for (uint32_t k = 0; k < G; ++k)
             iterations[k] = p->precision == NKPR_MIXED ? st->iterations : hc[k].iterations;
-    if (!p->scores_on_device) NKPR_HIP(hipMemcpyAsync(scores, sz, 8 * z, hipMemcpyDeviceToHost, s));
-    NKPR_HIP(hipStreamSynchronize(s));
+    read_back(s, {{scores, sz, p->scores_on_device ? 0 : 8 * z}});
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -65,6 +66,39 @@ struct Error : std::runtime_error {
     do {                                                                                     \
         if (!(cond)) throw ::nkpr::Error(NKPR_ERR_INVALID, (msg));                           \
     } while (0)
+
+// Device-to-host reads.  The targets are often stack variables or vectors of the calling frame,
+// so no copy may still be in flight when an exception unwinds that frame: every copy is
+// issued, then the stream is synchronised, and only then is a failure reported.
+struct HostCopy {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+inline void read_back(hipStream_t s, const HostCopy *copies, size_t count) {
+    hipError_t first = hipSuccess;
+    const char *what = "";
+    for (size_t i = 0; i < count; ++i) {
+        const HostCopy &c = copies[i];
+        if (c.bytes == 0) continue;
+        const hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) {
+            first = e;
+            what = "hipMemcpyAsync (device to host)";
+            break;
+        }
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (first == hipSuccess && e != hipSuccess) {
+        first = e;
+        what = "hipStreamSynchronize";
+    }
+    if (first != hipSuccess)
+        throw Error(NKPR_ERR_HIP, std::string(what) + " failed: " + hipGetErrorString(first));
+}
+inline void read_back(hipStream_t s, std::initializer_list<HostCopy> copies) {
+    read_back(s, copies.begin(), copies.size());
+}
 
 // Owning device allocation.
 struct DevBuf {
@@ -317,7 +351,6 @@ struct BinLayout {
     DevBuf gmask, gfirst, sdelta;
     uint32_t nsdelta = 0;
     bool derived = false;
-    bool probed = false;        // placement of the contribution array chosen (FP32 copy)
     double insum = 0.0;         // largest in-coefficient sum of a row (fixed-point bound)
     uint32_t np1 = 0, np2 = 0, multi = 0, nmb = 0, nslots = 0, grid1 = 0, grid2 = 0;
 };

@@ -118,9 +118,7 @@ uint64_t run_phase(RunCtx &rc, SweepArgs<T> a, Vecs<T> v, int kind, PhaseRule ru
             }
             if (post) post(v.x[cur], v.x[nxt]);
         }
-        NKPR_HIP(hipMemcpyAsync(g->ctl_host, g->ctl.p, sizeof(PhaseCtl), hipMemcpyDeviceToHost,
-                                g->stream));
-        NKPR_HIP(hipStreamSynchronize(g->stream));
+        read_back(g->stream, {{g->ctl_host, g->ctl.p, sizeof(PhaseCtl)}});
         const uint64_t it = g->ctl_host->iterations;
         const uint64_t executed = it - done_sweeps;
         if (timing) {
@@ -204,14 +202,12 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
     float *const V = k.V;
     auto dots = [&](const float *w, const float *Vb, uint32_t cnt, double *host) {
         launch_multidot(w, Vb, ld, cnt, n, k.dpart, k.dout, s);
-        NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * cnt, hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{host, k.dout, 8 * cnt}});
     };
     // w -= V c (cnt vectors), then host[i] = <u_i, w> for i < cnt and host[cnt] = <w, w>
     auto project = [&](float *w, uint32_t cnt, const double *c, double *host) {
         launch_axpy_dot(w, V, ld, cnt, n, c, k.dpart, k.dout, s);
-        NKPR_HIP(hipMemcpyAsync(host, k.dout, 8 * (cnt + 1), hipMemcpyDeviceToHost, s));
-        NKPR_HIP(hipStreamSynchronize(s));
+        read_back(s, {{host, k.dout, 8 * (cnt + 1)}});
     };
     const auto in_range = [](double nrm) { return nrm >= 0x1p-50 && nrm <= 0x1p50; };
     double beta2 = 0.0;

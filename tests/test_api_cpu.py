@@ -83,6 +83,53 @@ def test_rank_helpers():
     assert nk.centrality.relativeRankErrors(r, r[:2]) == []
 
 
+def _sorted_ranking(rng, ids, levels):
+    scores = rng.integers(0, levels, len(ids)) / 8.0
+    return sorted(zip(ids.tolist(), scores.tolist()), key=lambda p: (-p[1], p[0]))
+
+
+def test_rank_helpers_brute_force():
+    """On a ranking sorted by score, the shared position of a run of equal scores is the number
+    of entries with a strictly higher score; deleted ids (gaps) hold None."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    for trial in range(40):
+        z = int(rng.integers(1, 60))
+        keep = rng.random(z) < (1.0 if trial % 2 else 0.7)
+        keep[rng.integers(0, z)] = True
+        ids = np.nonzero(keep)[0]
+        r = _sorted_ranking(rng, ids, int(rng.integers(1, 6)))
+        pos = nk.centrality.rankPerNode(r)
+        assert len(pos) == max(len(r), int(ids.max()) + 1)
+        score = dict(r)
+        for u in range(len(pos)):
+            if u in score:
+                assert pos[u] == sum(1 for _, s in r if s > score[u])
+            else:
+                assert pos[u] is None
+        r2 = _sorted_ranking(rng, ids, 3)
+        rel = nk.centrality.relativeRankErrors(r, r2)
+        pos2 = nk.centrality.rankPerNode(r2)
+        for u in range(len(rel)):
+            if u in score:
+                assert rel[u] == (pos[u] + 1) / (pos2[u] + 1)
+            else:
+                assert rel[u] is None
+
+
+def test_graph_accessors_reject_unknown_ids():
+    G = eight()
+    for f in (G.degree, G.degreeIn, G.degreeOut, G.weightedDegree):
+        with pytest.raises(IndexError):
+            f(8)
+    with pytest.raises(IndexError):
+        G.degree(2 ** 40)
+    assert not G.hasEdge(0, 2 ** 40)
+    assert G.weight(0, 2 ** 40) == 0.0
+    with pytest.raises(RuntimeError):
+        G.addEdge(0, 8)
+
+
 def test_snap_reader_ids_and_dedup(tmp_path):
     p = tmp_path / "g.txt"
     p.write_text("# c\n10 20\n20 10\n10 20\n30\t10\n")

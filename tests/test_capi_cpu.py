@@ -83,3 +83,18 @@ def test_ranking_order_score_desc_id_asc():
     ref = sorted([i for i in range(1000) if ex[i]], key=lambda i: (-s[i], i))
     assert ids.tolist() == ref
     assert np.array_equal(vals, s[ids])
+
+
+def test_ranking_with_nan_scores_is_a_total_order():
+    # NaN scores rank after every number, ties among them by id; the order of the numbers is
+    # the reference's (score descending, id ascending)
+    rng = np.random.default_rng(2)
+    s = rng.integers(0, 6, 5000) / 7.0
+    s[rng.random(5000) < 0.2] = np.nan
+    s[rng.random(5000) < 0.05] = np.inf
+    s[rng.random(5000) < 0.05] = -np.inf
+    ids, vals = capi.ranking(s)
+    num = [i for i in range(5000) if not np.isnan(s[i])]
+    nan = [i for i in range(5000) if np.isnan(s[i])]
+    assert ids.tolist() == sorted(num, key=lambda i: (-s[i], i)) + nan
+    assert len(ids) == 5000

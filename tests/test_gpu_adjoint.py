@@ -190,3 +190,31 @@ def test_adjoint_host_arrays_and_invalid():
     with pytest.raises(capi.NkprError):
         dg.adjoint(g.in_ptr, g.in_src.astype(np.int32), x, 0.0, c,
                    weight=g.in_w.astype(np.float32))
+
+
+def test_backward_after_a_later_forward_uses_its_own_weights():
+    # op(w1), op(w2), then backward of the first: the handle holds w2 by then, and the gradient
+    # of the first call must still be the one of P(w1)
+    z, u, v, w = random_graph(250, 1400, 56, directed=True, weighted=True)
+    g = oracle.in_lists(z, u, v, w, True)
+    rng = np.random.default_rng(8)
+    w2 = rng.uniform(0.2, 4.0, len(g.in_w))
+    c = rng.normal(size=z)
+    uni = np.full(z, 1.0 / z)
+    gw1_ref, _, x1_ref = dense_grads(z, g.in_ptr, g.in_src, g.in_w, uni, c)
+    gw2_ref, _, x2_ref = dense_grads(z, g.in_ptr, g.in_src, w2, uni, c)
+    assert np.linalg.norm(gw1_ref - gw2_ref) > 1e-3 * np.linalg.norm(gw1_ref)
+    op = op_for(g)
+    ct = torch.tensor(c, device="cuda")
+    wt1 = torch.tensor(g.in_w, dtype=torch.float64, device="cuda", requires_grad=True)
+    wt2 = torch.tensor(w2, dtype=torch.float64, device="cuda", requires_grad=True)
+    x1 = op(wt1)
+    x2 = op(wt2)
+    assert np.abs(x1.detach().cpu().numpy() - x1_ref).sum() <= 1e-12
+    assert np.abs(x2.detach().cpu().numpy() - x2_ref).sum() <= 1e-12
+    (x1 * ct).sum().backward()
+    gw1 = wt1.grad.cpu().numpy()
+    assert np.linalg.norm(gw1 - gw1_ref) <= 1e-9 * np.linalg.norm(gw1_ref)
+    (x2 * ct).sum().backward()
+    gw2 = wt2.grad.cpu().numpy()
+    assert np.linalg.norm(gw2 - gw2_ref) <= 1e-9 * np.linalg.norm(gw2_ref)
