@@ -18,6 +18,11 @@
  *   nkpr_graph_set_weights  no counterpart: a new Graph with the same structure
  *   nkpr_params_default  the constructor defaults PageRank(G, 0.85, 1e-8, false), the field
  *                        defaults maxIterations = UINT64_MAX, norm = L2Norm (PageRank.hpp:40-67)
+ *   nkpr_power_run       EigenvectorCentrality::run (EigenvectorCentrality.cpp:21-66) and
+ *                        KatzCentrality::run (KatzCentrality.cpp:28-76)
+ *   nkpr_power_params_default  the constructor defaults EigenvectorCentrality(G, 1e-8)
+ *                        (EigenvectorCentrality.hpp:31) and KatzCentrality(G, 0, 0.1, 1e-8)
+ *                        (KatzCentrality.hpp:37)
  *
  * Device placement: a graph lives on the HIP device that was current when it was created and
  * owns one HIP stream (default priority); all of its work is ordered on that stream.  Calls
@@ -229,6 +234,40 @@ int nkpr_ppr_batch(nkpr_graph *g, const nkpr_params *p, uint32_t k,
                    const uint64_t *seed_offsets, const int32_t *seed_nodes,
                    const double *seed_weights, double *scores, nkpr_batch_stats *stats,
                    uint64_t *col_iterations, double *col_error_bound);
+
+/* Eigenvector and Katz centrality on a graph handle whose coefficients are the edge weights:
+ * one created with out_wdeg[u] = 1 for every node (a handle made otherwise is rejected).  Both
+ * are the power iterations of the reference, pulled over the handle's in-lists:
+ *   NKPR_EIGENVECTOR  EigenvectorCentrality::run (EigenvectorCentrality.cpp:21-66):
+ *                     y = A x, x = y / ||y||_2, until | ||y|| - previous ||y|| | <= tol;
+ *                     |x| when the score of slot 0 is negative.
+ *   NKPR_KATZ         KatzCentrality::run (KatzCentrality.cpp:28-76):
+ *                     x = alpha A (1 + x) + beta, until | ||x|| - previous ||x|| | <= tol;
+ *                     then x / ||x||_2.  alpha is used as given (the reference's default
+ *                     1 / (1 + max out-degree) is the caller's to compute); for the reference's
+ *                     edgeDirection OutEdges create the handle from the out-lists.
+ * Both start from 1 in every slot; norms are over existing nodes; slots that hold no node are
+ * set to 1 (as the reference leaves them).  The sweeps are FP64.  A Katz run whose norm stops
+ * being finite (alpha above 1 / spectral radius) fails with NKPR_ERR_INVALID, where the
+ * reference would loop forever.  stats: iterations (sweeps), last_step = the final change of
+ * the norm, score_sum = the final norm, ms_total. */
+#define NKPR_EIGENVECTOR 0
+#define NKPR_KATZ 1
+typedef struct {
+    int32_t kind;             /* NKPR_EIGENVECTOR or NKPR_KATZ                             */
+    int32_t scores_on_device; /* 1: `scores` is a device pointer                           */
+    double alpha;             /* Katz: damping of the product (>= 0)                       */
+    double beta;              /* Katz: constant added to every node                        */
+    double tol;               /* stop when the norm changes by at most tol (default 1e-8)  */
+    uint64_t max_iterations;  /* not in the reference, which loops until the test holds;
+                                 default UINT64_MAX                                        */
+    const volatile int32_t *interrupt; /* optional; checked between batches of sweeps      */
+} nkpr_power_params;
+
+/* Fills kind EIGENVECTOR, alpha 0, beta 0.1, tol 1e-8, max_iterations UINT64_MAX. */
+int nkpr_power_params_default(nkpr_power_params *p);
+/* scores[z]: output only (both reference loops start from 1, with no warm start). */
+int nkpr_power_run(nkpr_graph *g, const nkpr_power_params *p, double *scores, nkpr_stats *stats);
 
 /* Centrality::ranking: ids of the existing nodes ordered by score descending, ties by id
  * ascending; ids_out and scores_out (nullable) receive *count = number of existing nodes

@@ -20,7 +20,8 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["nkpr_sweep.hip", "nkpr_graph.hip", "nkpr_krylov.hip", "nkpr_batch.hip",
-               "nkpr_adjoint.hip", "nkpr_groups.hip", "nkpr_binned.hip", "nkpr_capi.hip"]
+               "nkpr_adjoint.hip", "nkpr_groups.hip", "nkpr_binned.hip", "nkpr_power.hip",
+               "nkpr_capi.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # packed FP32 instructions stay off (some of their forms are unreliable on this pool)
@@ -30,7 +31,7 @@ HIP_FLAGS = [
 ]
 LIB_NKPR = os.path.join(HERE, "libnkpr.so")
 LIB_NKGEN = os.path.join(HERE, "libnkgen.so")
-CXX_SOURCES = ["Graph.cpp", "PageRank.cpp"]
+CXX_SOURCES = ["Graph.cpp", "PageRank.cpp", "EigenvectorCentrality.cpp"]
 
 
 def _run(cmd, cwd=None):

@@ -105,6 +105,18 @@ cdef extern from "PageRank.hpp" namespace "NetworKit":
         const nkpr_stats& lastStats()
         const nkpr_graph_info& deviceGraphInfo() except +
 
+cdef extern from "EigenvectorCentrality.hpp" namespace "NetworKit":
+    cdef enum _EdgeDirection "NetworKit::EdgeDirection":
+        _InEdges "NetworKit::InEdges"
+        _OutEdges "NetworKit::OutEdges"
+    cdef cppclass _EigenvectorCentrality "NetworKit::EigenvectorCentrality"(_Centrality):
+        _EigenvectorCentrality(_Graph, double) except +
+        const nkpr_stats& lastStats()
+    cdef cppclass _KatzCentrality "NetworKit::KatzCentrality"(_Centrality):
+        _KatzCentrality(_Graph, double, double, double) except +
+        _EdgeDirection edgeDirection
+        const nkpr_stats& lastStats()
+
 
 cdef class Graph:
     """Graph(n=0, weighted=False, directed=False): NetworKit-compatible graph (subset)."""
@@ -354,3 +366,54 @@ cdef class PageRank(Centrality):
         return dict(z=i.z, n=i.n, m=i.m, sinks=i.sinks, max_in_degree=i.max_in_degree,
                     work_items=i.work_items, device_bytes=i.device_bytes,
                     weighted=bool(i.weighted), device=i.device, prep_ms=i.prep_ms)
+
+
+cdef dict _power_stats(nkpr_stats s):
+    return dict(iterations=s.iterations, last_step=s.last_step, norm=s.score_sum,
+                ms_total=s.ms_total)
+
+
+cdef class EigenvectorCentrality(Centrality):
+    """EigenvectorCentrality(G, tol=1e-9)
+
+    The leading eigenvector of the graph's adjacency matrix, normalised in the 2-norm
+    (networkit.centrality.EigenvectorCentrality, centrality.pyx:1950-1968), computed on the
+    MI355X.  `tol` bounds the change of the norm between iterations."""
+
+    def __cinit__(self, Graph G, double tol=1e-9):
+        self._G = G
+        self._this = new _EigenvectorCentrality(G._this[0], tol)
+
+    def stats(self):
+        """Statistics of the last run: iterations, final change and value of the norm."""
+        return _power_stats((<_EigenvectorCentrality*>(self._this)).lastStats())
+
+
+class EdgeDirection(object):
+    """Which neighbours Katz centrality sums over (KatzCentrality.hpp:17)."""
+    inEdges = 0
+    outEdges = 1
+
+
+cdef class KatzCentrality(Centrality):
+    """KatzCentrality(G, alpha=0, beta=0.1, tol=1e-8)
+
+    Katz centrality (networkit.centrality.KatzCentrality, centrality.pyx:1678-1710) computed
+    on the MI355X.  alpha = 0 takes 1 / (1 + the largest out-degree); `edgeDirection` selects
+    in-neighbours (default) or out-neighbours."""
+
+    def __cinit__(self, Graph G, alpha=0, beta=0.1, tol=1e-8):
+        self._G = G
+        self._this = new _KatzCentrality(G._this[0], alpha, beta, tol)
+
+    property edgeDirection:
+        def __get__(self):
+            return <int>(<_KatzCentrality*>(self._this)).edgeDirection
+        def __set__(self, edgeDirection):
+            if int(edgeDirection) not in (0, 1):
+                raise ValueError("edgeDirection must be EdgeDirection.inEdges or .outEdges")
+            (<_KatzCentrality*>(self._this)).edgeDirection = <_EdgeDirection>(<int>edgeDirection)
+
+    def stats(self):
+        """Statistics of the last run: iterations, final change and value of the norm."""
+        return _power_stats((<_KatzCentrality*>(self._this)).lastStats())

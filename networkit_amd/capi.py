@@ -28,7 +28,8 @@ NKPR_STOP_STEP, NKPR_STOP_L1_ERROR = 0, 1
 EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch",
             "nkpr_graph_destroy", "nkpr_graph_get_info", "nkpr_graph_stream",
             "nkpr_graph_set_weights", "nkpr_run", "nkpr_run_batch", "nkpr_adjoint",
-            "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version"]
+            "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version",
+            "nkpr_power_params_default", "nkpr_power_run"]
 ABI_VERSION = 2
 
 
@@ -69,6 +70,15 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+NKPR_EIGENVECTOR, NKPR_KATZ = 0, 1
+
+
+class PowerParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("scores_on_device", ctypes.c_int32),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("tol", ctypes.c_double),
+                ("max_iterations", ctypes.c_uint64), ("interrupt", ctypes.c_void_p)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint64), ("sweeps_fp32", ctypes.c_uint64),
                 ("sweeps_fp64", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
@@ -102,6 +112,8 @@ def lib():
     L.nkpr_graph_stream.argtypes = [vp]
     L.nkpr_graph_stream.restype = vp
     L.nkpr_run.argtypes = [vp, ctypes.POINTER(Params), vp, ctypes.POINTER(Stats)]
+    L.nkpr_power_params_default.argtypes = [ctypes.POINTER(PowerParams)]
+    L.nkpr_power_run.argtypes = [vp, ctypes.POINTER(PowerParams), vp, ctypes.POINTER(Stats)]
     L.nkpr_graph_create_batch.argtypes = [ctypes.POINTER(Csc), ctypes.c_uint32, vp,
                                           ctypes.POINTER(vp)]
     L.nkpr_run_batch.argtypes = [vp, ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Stats)]
@@ -129,6 +141,16 @@ class NkprError(RuntimeError):
 def check(rc):
     if rc != NKPR_OK:
         raise NkprError(rc, lib().nkpr_last_error().decode())
+
+
+def default_power_params(**kw):
+    p = PowerParams()
+    check(lib().nkpr_power_params_default(ctypes.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
 
 
 def default_params(**kw):
@@ -248,6 +270,18 @@ class DeviceGraph:
             assert len(scores) == self.z
         st = Stats()
         check(lib().nkpr_run(self.handle, ctypes.byref(p), _addr(scores), ctypes.byref(st)))
+        return st.as_dict()
+
+    def power_run(self, scores, params=None, **kw):
+        """nkpr_power_run (eigenvector or Katz centrality; the handle must have been created
+        with out_wdeg = 1).  ``scores`` is numpy float64[z] (output) or a torch CUDA tensor with
+        params.scores_on_device = 1.  Returns the stats dict."""
+        p = params if params is not None else default_power_params(**kw)
+        if isinstance(scores, np.ndarray):
+            assert scores.dtype == np.float64 and scores.flags.c_contiguous
+            assert len(scores) == self.z
+        st = Stats()
+        check(lib().nkpr_power_run(self.handle, ctypes.byref(p), _addr(scores), ctypes.byref(st)))
         return st.as_dict()
 
     def run_batch(self, scores, params=None, **kw):

@@ -1,14 +1,16 @@
 """networkit.centrality equivalent for the PageRank path.
 
-PageRank, Centrality, Norm (networkit/centrality.pyx:24-100, 1970-2037) come from the Cython
-extension.  rankPerNode and relativeRankErrors provide the two pure-Python ranking helpers of
+PageRank, Centrality, Norm (networkit/centrality.pyx:24-100, 1970-2037), EigenvectorCentrality
+(:1950-1968) and KatzCentrality with EdgeDirection (:1662-1710) come from the Cython extension.
+rankPerNode and relativeRankErrors provide the two pure-Python ranking helpers of
 networkit/centrality.pyx:2197-2249 with the same tie rule; unlike those, they accept rankings
 whose node ids have gaps (graphs with deleted nodes).
 """
-from ._core import Centrality, Norm, PageRank, Precision, SinkHandling  # noqa: F401
+from ._core import (Centrality, EdgeDirection, EigenvectorCentrality,  # noqa: F401
+                    KatzCentrality, Norm, PageRank, Precision, SinkHandling)
 
-__all__ = ["Centrality", "Norm", "PageRank", "Precision", "SinkHandling", "rankPerNode",
-           "relativeRankErrors"]
+__all__ = ["Centrality", "EdgeDirection", "EigenvectorCentrality", "KatzCentrality", "Norm",
+           "PageRank", "Precision", "SinkHandling", "rankPerNode", "relativeRankErrors"]
 
 
 def rankPerNode(ranking):

@@ -130,11 +130,12 @@ edgeweight Graph::weightedDegree(node u, bool countSelfLoopsTwice) const {
     return static_cast<edgeweight>(sum);
 }
 
-void Graph::exportInLists(std::vector<uint64_t> &colptr, std::vector<int32_t> &src,
-                          std::vector<float> *w32, std::vector<double> *w64) const {
+void Graph::exportLists(bool outgoing, std::vector<uint64_t> &colptr, std::vector<int32_t> &src,
+                        std::vector<float> *w32, std::vector<double> *w64) const {
     if (z >= 0x7FFFFFFFull) throw std::runtime_error("graph too large for int32 node ids");
-    const auto &lists = directed ? inEdges : outEdges;
-    const auto &wlists = directed ? inEdgeWeights : outEdgeWeights;
+    const bool in = directed && !outgoing;
+    const auto &lists = in ? inEdges : outEdges;
+    const auto &wlists = in ? inEdgeWeights : outEdgeWeights;
     colptr.assign(z + 1, 0);
     for (node u = 0; u < z; ++u) colptr[u + 1] = colptr[u] + lists[u].size();
     const uint64_t total = colptr[z];

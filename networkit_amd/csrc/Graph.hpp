@@ -63,7 +63,12 @@ public:
     // Flattened pull lists: row u holds the in-neighbours of u (directed) or the adjacency of
     // u (undirected), in insertion order.  Weights are filled only for weighted graphs.
     void exportInLists(std::vector<uint64_t> &colptr, std::vector<int32_t> &src,
-                       std::vector<float> *w32, std::vector<double> *w64) const;
+                       std::vector<float> *w32, std::vector<double> *w64) const {
+        exportLists(false, colptr, src, w32, w64);
+    }
+    // outgoing = true: row u holds u's out-neighbours instead (the same lists when undirected)
+    void exportLists(bool outgoing, std::vector<uint64_t> &colptr, std::vector<int32_t> &src,
+                     std::vector<float> *w32, std::vector<double> *w64) const;
     const std::vector<bool> &existsFlags() const noexcept { return exists; }
 
     // Increments on every structural or weight change; consumers cache device copies by it.

@@ -584,6 +584,27 @@ int nkpr_run(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *st
     });
 }
 
+int nkpr_power_params_default(nkpr_power_params *p) {
+    return guarded([&] {
+        NKPR_CHECK(p != nullptr, "params is NULL");
+        std::memset(p, 0, sizeof(*p));
+        p->kind = NKPR_EIGENVECTOR;
+        p->alpha = 0.0;
+        p->beta = 0.1;
+        p->tol = 1e-8;
+        p->max_iterations = std::numeric_limits<uint64_t>::max();
+    });
+}
+
+int nkpr_power_run(nkpr_graph *g, const nkpr_power_params *p, double *scores, nkpr_stats *stats) {
+    nkpr_stats local;
+    return guarded([&] {
+        NKPR_CHECK(g != nullptr, "graph is NULL");
+        DeviceGuard dg(g->device);
+        power_run_impl(g, p, scores, stats ? stats : &local);
+    });
+}
+
 int nkpr_graph_set_weights(nkpr_graph *g, const float *weight, const double *weight64,
                            const double *out_wdeg, int32_t device_ptrs) {
     return guarded([&] {

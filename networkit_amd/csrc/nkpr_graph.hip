@@ -33,6 +33,15 @@ inline unsigned blocks_for(uint64_t n, int t = 256) { return (unsigned)((n + t -
 
 // validation flags
 constexpr uint32_t kBadSrc = 1, kBadPtr = 2, kBadExists = 4, kBadGroup = 8;
+constexpr uint32_t kNotUnitDegree = 16;  // not an error: some node's given out-degree is not 1
+
+// Sets kNotUnitDegree in *flags when deg[i] != 1 for an existing i < count (exists may be NULL).
+__global__ void unit_degree_kernel(const double *deg, uint64_t count, const uint8_t *exists,
+                                   uint32_t *flags) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        if ((!exists || exists[i]) && deg[i] != 1.0) atomicOr(flags, kNotUnitDegree);
+}
 
 __global__ void validate_ptr_kernel(const uint64_t *colptr, uint64_t z, uint64_t m,
                                     const uint8_t *exists, uint32_t *flags) {
@@ -590,9 +599,13 @@ void create_graph(const nkpr_csc *c, nkpr_graph *g, const uint64_t *group_off,
         hipLaunchKernelGGL(out_degree_kernel, dim3(gb), dim3(256), 0, s, src_in, m, z, exists, w_in,
                            w64_in, outcnt_b.as<uint32_t>(), wdeg_own, flags);
     }
+    if (c->out_wdeg && z)
+        hipLaunchKernelGGL(unit_degree_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(z), 65536)),
+                           dim3(256), 0, s, wdeg, z, exists, flags);
     NKPR_HIP(hipGetLastError());
     uint32_t hflags = 0;
     read_back(s, {{&hflags, flags, 4}});
+    g->unit_degrees = c->out_wdeg != nullptr && !(hflags & kNotUnitDegree);
     NKPR_CHECK(!(hflags & kBadPtr), "colptr is not a non-decreasing offset array ending at m");
     NKPR_CHECK(!(hflags & kBadSrc), "an in-list names a source outside [0, z) or a deleted node");
     NKPR_CHECK(!(hflags & kBadExists), "a deleted slot has in-edges");
@@ -859,7 +872,19 @@ void set_weights(nkpr_graph *g, const float *w, const double *w64, const double 
         if (n)
             hipLaunchKernelGGL(gather_wdeg_kernel, dim3(blocks_for(n)), dim3(256), 0, s, dz,
                                g->perm.as<int32_t>(), n, deg.as<double>());
+        DevBuf uf;
+        uf.alloc(4);
+        NKPR_HIP(hipMemsetAsync(uf.p, 0, 4, s));
+        if (n)
+            hipLaunchKernelGGL(unit_degree_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(n), 65536)),
+                               dim3(256), 0, s, (const double *)deg.as<double>(), (uint64_t)n,
+                               (const uint8_t *)nullptr, uf.as<uint32_t>());
+        NKPR_HIP(hipGetLastError());
+        uint32_t hf = 0;
+        read_back(s, {{&hf, uf.p, 4}});
+        g->unit_degrees = !(hf & kNotUnitDegree);
     } else {
+        g->unit_degrees = false;
         NKPR_HIP(hipMemsetAsync(deg.p, 0, 8 * std::max<uint32_t>(n, 1), s));
         if (m)
             hipLaunchKernelGGL(sweep_wdeg_kernel, dim3((unsigned)std::min<uint64_t>(blocks_for(m), 65536)),

@@ -329,6 +329,8 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
                   double *grad_weight, double *grad_teleport, nkpr_stats *st);
 void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_t *iterations,
                     double *maximum, nkpr_stats *st);
+// nkpr_power_run (nkpr_power.hip): eigenvector and Katz centrality
+void power_run_impl(nkpr_graph *g, const nkpr_power_params *p, double *scores, nkpr_stats *st);
 void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint64_t *seed_off,
                     const int32_t *seeds, const double *seed_w, double *scores,
                     nkpr_batch_stats *st, uint64_t *col_iterations, double *col_bound);
@@ -377,6 +379,7 @@ struct nkpr_graph {
     bool rowsum_ready = false;
     nkpr::DevBuf flag_buf;  // uint32 [1]  device-side yes/no answers read back by the host
     bool w32_exact = false; // every weight is an FP32 number (weighted)
+    bool unit_degrees = false;  // created with out_wdeg = 1 for every node: coefficients = weights
     nkpr::DevBuf emap;      // uint32 [m]  input position of every edge in sweep order (weighted)
     std::vector<uint8_t> exists_host;  // empty: every slot exists
 

@@ -5,7 +5,8 @@ import this package, and only as the checker (or, for the baseline, as the thing
 host cores).  The product (``networkit_amd``) never imports it and has no CPU fallback.
 
 The arithmetic lives in ``pagerank_oracle.c`` (a restatement of
-networkit/cpp/centrality/PageRank.cpp:20-104); this module loads it with ctypes and adds the
+networkit/cpp/centrality/PageRank.cpp:20-104) and ``power_oracle.c`` (EigenvectorCentrality.cpp:
+21-66, KatzCentrality.cpp:28-76); this module loads it with ctypes and adds the
 NetworKit-semantics edge-list -> in-list conversion (Graph.cpp:549-597, Graph.hpp:1918-1938)
 that the C code consumes.
 
@@ -15,6 +16,10 @@ Parity status of each function is pinned in tests/test_oracle.py:
                   solve with that right-hand side, uniform teleport == the reference loop,
                   linearity in the teleport vector
   ranking         brute-force sort
+  eigenvector     8-node known answers (CentralityGTest.cpp:541-578), dense eigendecomposition
+                  of the adjacency matrix (symmetric and directed), regular-graph invariant
+  katz            wiki-Vote top-1 with alpha 5e-4 (CentralityGTest.cpp:216-225), sparse direct
+                  solve of (I - alpha A) x = alpha A 1 + beta 1, regular-graph invariant
   in_lists        hand-built NetworKit adjacency of small graphs (directed, undirected,
                   self-loops, multi-edges)
 """
@@ -62,6 +67,14 @@ def _load():
     lib.oracle_out_degrees.restype = None
     lib.oracle_time_sweeps.argtypes = [u64, p, p, p, ctypes.c_double, u64, ctypes.c_int, p]
     lib.oracle_time_sweeps.restype = ctypes.c_double
+    lib.oracle_eigenvector.argtypes = [u64, p, p, p, p, ctypes.c_double, u64, p,
+                                       ctypes.POINTER(u64)]
+    lib.oracle_eigenvector.restype = ctypes.c_int
+    lib.oracle_katz.argtypes = [u64, p, p, p, p, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, u64, p, ctypes.POINTER(u64)]
+    lib.oracle_katz.restype = ctypes.c_int
+    lib.oracle_katz_default_alpha.argtypes = [u64, p, p, ctypes.c_int]
+    lib.oracle_katz_default_alpha.restype = ctypes.c_double
     lib.oracle_num_threads.argtypes = []
     lib.oracle_num_threads.restype = ctypes.c_int
     _lib = lib
@@ -172,6 +185,42 @@ def time_sweeps(g, damp=0.85, sweeps=3, threads=0):
     x = np.full(g.z, 1.0 / max(g.z, 1), dtype=np.float64)
     return lib.oracle_time_sweeps(g.z, _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w), damp,
                                   int(sweeps), int(threads), _ptr(x))
+
+
+def eigenvector(g, tol=1e-8, max_iterations=2**64 - 1):
+    """The restated EigenvectorCentrality::run on in-lists ``g``.  Returns (scores,
+    iterations)."""
+    lib = _load()
+    scores = np.zeros(g.z, dtype=np.float64)
+    it = ctypes.c_uint64(0)
+    rc = lib.oracle_eigenvector(g.z, _ptr(g.exists), _ptr(g.in_ptr), _ptr(g.in_src),
+                                _ptr(g.in_w), tol, int(max_iterations), _ptr(scores),
+                                ctypes.byref(it))
+    if rc != 0:
+        raise RuntimeError("oracle_eigenvector failed")
+    return scores, int(it.value)
+
+
+def katz_default_alpha(g):
+    """1 / (1 + largest unweighted out-degree), the reference's alpha = 0 default."""
+    return float(_load().oracle_katz_default_alpha(g.z, _ptr(g.in_ptr), _ptr(g.in_src),
+                                                   int(bool(g.directed))))
+
+
+def katz(g, alpha=0.0, beta=0.1, tol=1e-8, max_iterations=2**64 - 1):
+    """The restated KatzCentrality::run (in-neighbours; pass the reversed graph's in-lists for
+    the OutEdges direction).  alpha = 0 takes the reference default.  Returns (scores,
+    iterations)."""
+    lib = _load()
+    if alpha == 0:
+        alpha = katz_default_alpha(g)
+    scores = np.zeros(g.z, dtype=np.float64)
+    it = ctypes.c_uint64(0)
+    rc = lib.oracle_katz(g.z, _ptr(g.exists), _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w),
+                         alpha, beta, tol, int(max_iterations), _ptr(scores), ctypes.byref(it))
+    if rc != 0:
+        raise RuntimeError("oracle_katz failed")
+    return scores, int(it.value)
 
 
 def num_threads():

@@ -151,3 +151,22 @@ def test_bench_refuses_more_than_one_gpu():
                        cwd=root, capture_output=True, text=True, timeout=300)
     assert p.returncode == 2
     assert "measures one GPU" in p.stderr and p.stdout == ""
+
+
+def test_eigenvector_and_katz_before_run():
+    G = eight()
+    ev = nk.centrality.EigenvectorCentrality(G)
+    kc = nk.centrality.KatzCentrality(G)
+    for c in (ev, kc):
+        assert not c.hasFinished()
+        with pytest.raises(RuntimeError):
+            c.scores()
+        with pytest.raises(RuntimeError):
+            c.ranking()
+    assert kc.edgeDirection == nk.centrality.EdgeDirection.inEdges == 0
+    kc.edgeDirection = nk.centrality.EdgeDirection.outEdges
+    assert kc.edgeDirection == 1
+    with pytest.raises(ValueError):
+        kc.edgeDirection = 5
+    with pytest.raises(RuntimeError, match="Alpha must be non negative"):
+        nk.centrality.KatzCentrality(G, -0.5)

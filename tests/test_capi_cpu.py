@@ -38,6 +38,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(capi.BatchStats) == 8 * 5 + 8 * 4 + 4 * 2
     assert ctypes.sizeof(capi.Stats) == 8 * 13
     assert ctypes.sizeof(capi.GraphInfo) == 8 * 7 + 8 + 8
+    assert ctypes.sizeof(capi.PowerParams) == 4 * 2 + 8 * 3 + 8 + 8
 
 
 def test_params_default_is_reference_default():
@@ -73,6 +74,15 @@ def test_null_arguments_are_errors_not_crashes():
     assert L.nkpr_graph_create_batch(None, 0, None, None) == capi.NKPR_ERR_INVALID
     assert L.nkpr_graph_destroy(None) == capi.NKPR_OK
     assert L.nkpr_params_default(None) == capi.NKPR_ERR_INVALID
+    assert L.nkpr_power_params_default(None) == capi.NKPR_ERR_INVALID
+    assert L.nkpr_power_run(None, None, None, None) != capi.NKPR_OK
+
+
+def test_power_params_default_is_reference_default():
+    p = capi.default_power_params()
+    assert p.kind == capi.NKPR_EIGENVECTOR and p.tol == 1e-8
+    assert p.alpha == 0.0 and p.beta == 0.1  # KatzCentrality(G, 0, 0.1, 1e-8)
+    assert p.max_iterations == 2 ** 64 - 1 and p.scores_on_device == 0
 
 
 def test_ranking_order_score_desc_id_asc():

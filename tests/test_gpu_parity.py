@@ -339,10 +339,10 @@ def test_capi_interrupt_during_a_run():
     assert e.value.code == capi.NKPR_ERR_INTERRUPTED
     assert time.time() - t0 < 30.0
     assert np.array_equal(x, start)
-    # the handle runs normally afterwards
+    # the handle runs normally afterwards (from the same, untouched, start vector)
     flag.value = 0
     st = dg.run(x, capi.default_params())
-    ref, it, _ = oracle.pagerank(oracle.in_lists(z, u, v, w, True))
+    ref, it, _ = oracle.pagerank(oracle.in_lists(z, u, v, w, True), start=start)
     assert st["iterations"] == it and np.abs(x - ref).sum() <= 1e-12
 
 

@@ -231,3 +231,106 @@ def test_oracle_teleport_linear():
     s, _, _ = oracle.pagerank(g, tol=1e-15, norm=oracle.L1, max_iterations=5000, teleport=t)
     mix = 0.3 * x1 + 0.7 * x2
     assert np.abs(s - mix / mix.sum()).sum() < 1e-11
+
+
+# ---------------------------------------------------------------- eigenvector and Katz
+EIGHT_EV = [0.2254, 0.1503, 0.5290, 0.4508, 0.3006, 0.5290, 0.2254, 0.1503]  # :570-577, 1e-4
+
+
+def adjacency(g):
+    """Dense A over existing nodes (compact order): A[u, v] = sum of the weights of u's
+    in-list entries with source v, as the reference loops pull them."""
+    exists = np.ones(g.z, bool) if g.exists is None else g.exists.astype(bool)
+    idx = -np.ones(g.z, np.int64)
+    idx[exists] = np.arange(int(exists.sum()))
+    rows = np.repeat(np.arange(g.z), np.diff(g.in_ptr.astype(np.int64)))
+    w = np.ones(len(rows)) if g.in_w is None else g.in_w
+    n = int(exists.sum())
+    A = np.zeros((n, n))
+    np.add.at(A, (idx[rows], idx[g.in_src.astype(np.int64)]), w)
+    return A, exists
+
+
+def test_eigenvector_eight_node_known_answers():
+    z, u, v, w = eight_node()
+    s, it = oracle.eigenvector(oracle.in_lists(z, u, v, w, False))
+    assert np.allclose(s, EIGHT_EV, atol=1e-4)
+    assert it > 1
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_eigenvector_is_the_leading_eigenvector(directed):
+    """Dense eigendecomposition: the Perron vector of A normalised in the 2-norm.  A transposed
+    pull (out-lists instead of in-lists) fails the directed case."""
+    z, u, v, w = random_graph(80, 900, 61, directed=directed, weighted=True)
+    g = oracle.in_lists(z, u, v, w, directed)
+    A, _ = adjacency(g)
+    vals, vecs = np.linalg.eig(A)
+    k = int(np.argmax(vals.real))
+    ref = np.abs(vecs[:, k].real)
+    ref /= np.linalg.norm(ref)
+    s, it = oracle.eigenvector(g, tol=1e-15, max_iterations=100000)
+    assert np.abs(s - ref).max() <= 1e-6
+    gt = oracle.in_lists(z, v, u, w, True) if directed else None
+    if directed:
+        st, _ = oracle.eigenvector(gt, tol=1e-15, max_iterations=100000)
+        assert np.abs(st - ref).max() > 1e-3
+
+
+def test_eigenvector_regular_graph_deleted_slot_and_sign():
+    # a 7-cycle plus chords of a circulant (4-regular): uniform 1/sqrt(n); a deleted slot keeps 1
+    n = 9
+    u = np.concatenate([np.arange(7), np.arange(7)])
+    v = np.concatenate([(np.arange(7) + 1) % 7, (np.arange(7) + 3) % 7])
+    ex = np.ones(n, np.uint8)
+    ex[[7, 8]] = 0
+    g = oracle.in_lists(n, u, v, None, False, exists=ex)
+    s, _ = oracle.eigenvector(g)
+    assert np.allclose(s[:7], 1 / np.sqrt(7), atol=1e-12)
+    assert s[7] == 1.0 and s[8] == 1.0
+    # negative weights: the loop corrects the sign when slot 0 ends negative
+    z, uu, vv, w = random_graph(40, 300, 62, directed=False, weighted=True)
+    gn = oracle.in_lists(z, uu, vv, -w, False)
+    sn, _ = oracle.eigenvector(gn, tol=1e-12)
+    assert sn[0] >= 0
+
+
+def test_katz_wiki_vote_known_answer():
+    # CentralityGTest.cpp:216-225 reads the file with SNAPGraphReader{}, whose default is
+    # undirected (SNAPGraphReader.hpp:42), despite the test's name
+    z, u, v = wiki_vote(False)
+    g = oracle.in_lists(z, u, v, None, False)
+    s, it = oracle.katz(g, alpha=5e-4)
+    assert oracle.ranking(s)[0] == 699
+    assert it > 2
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_katz_is_the_linear_solve(directed):
+    """x = alpha A (1 + x) + beta has the fixed point (I - alpha A) x = alpha A 1 + beta 1; the
+    result is x / ||x||_2.  A dropped (1 + .) or a missing beta fails it."""
+    z, u, v, w = random_graph(500, 4000, 63, directed=directed, weighted=True)
+    g = oracle.in_lists(z, u, v, w, directed)
+    A, _ = adjacency(g)
+    alpha = oracle.katz_default_alpha(g)
+    beta = 0.3
+    x = np.linalg.solve(np.eye(len(A)) - alpha * A, alpha * A.sum(1) + beta)
+    ref = x / np.linalg.norm(x)
+    s, it = oracle.katz(g, alpha=alpha, beta=beta, tol=1e-14)
+    assert np.abs(s - ref).max() <= 1e-10
+
+
+def test_katz_default_alpha_and_regular_graph():
+    z, u, v, w = random_graph(300, 2000, 64, directed=True)
+    g = oracle.in_lists(z, u, v, None, True)
+    outdeg = np.bincount(u, minlength=z)
+    assert oracle.katz_default_alpha(g) == 1.0 / (1 + outdeg.max())
+    gu = oracle.in_lists(z, u, v, None, False)
+    _, cnt = oracle.out_degrees(gu)
+    assert oracle.katz_default_alpha(gu) == 1.0 / (1 + cnt.max())
+    # 4-regular circulant: uniform scores
+    n = 11
+    uu = np.concatenate([np.arange(n), np.arange(n)])
+    vv = np.concatenate([(np.arange(n) + 1) % n, (np.arange(n) + 2) % n])
+    s, _ = oracle.katz(oracle.in_lists(n, uu, vv, None, False))
+    assert np.allclose(s, 1 / np.sqrt(n), atol=1e-12)
