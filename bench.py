@@ -59,13 +59,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
-    ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "graphs", "adjoint"],
+    ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "graphs", "adjoint",
+                                           "algebraic"],
                     default="pagerank",
                     help="pagerank: one solve on R-MAT (the benchmark line); rhg: the same on "
                          "an undirected random hyperbolic graph; ppr: batched personalised "
                          "PageRank on a Barabasi-Albert graph; graphs: one batched run over 512 "
                          "Erdos-Renyi graphs of uneven size; adjoint: forward and backward "
-                         "of differentiable PageRank on R-MAT scale 20")
+                         "of differentiable PageRank on R-MAT scale 20; algebraic: "
+                         "AlgebraicPageRank on NetworKit's CSR matrix of the R-MAT graph")
     ap.add_argument("--rhg-nodes", type=int, default=10_000_000)
     ap.add_argument("--ppr-nodes", type=int, default=2_000_000)
     ap.add_argument("--ppr-columns", type=int, default=256)
@@ -81,10 +83,11 @@ def main():
     if args.workload == "graphs":
         bench_graphs(args)
         return
-    if args.workload == "rhg":
-        args.no_cpu = True
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "algebraic":
+        bench_algebraic(args)
         return
 
     import torch
@@ -179,12 +182,19 @@ def main():
             algorithmic_bytes(m, n, weighted, "fp32") / (ms / 1e3) / 1e9))
         return
 
+    # the reference's own stopping rule (L2 step <= 1e-8, FP64): its sweep count sizes the
+    # CPU baseline's estimate of run() (not timed)
+    torch.cuda.current_stream().synchronize()
+    with torch.cuda.stream(stream):
+        x.fill_(1.0 / z)
+    its_rule = dg.run(x, capi.default_params(scores_on_device=1, precision=capi.NKPR_FP64,
+                                             tol=1e-8))["iterations"]
     # tightly converged FP64 reference (not timed)
     st = solve(capi.NKPR_FP64, 1e-13)
     ref = x.clone()
     torch.cuda.synchronize()
-    log("reference: %d FP64 sweeps, certified L1 bound %.2e" % (st["iterations"],
-                                                              st["l1_error_bound"]))
+    log("reference: %d FP64 sweeps, certified L1 bound %.2e (reference rule: %d sweeps)" % (
+        st["iterations"], st["l1_error_bound"], its_rule))
 
     def timed(precision, steps, warmup):
         for _ in range(warmup):
@@ -231,7 +241,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(g, z, m, args.cpu_seconds)
+        cpu = cpu_run_baseline(host_in_lists(g["colptr"], g["src"], g["weight"]), m,
+                               args.cpu_seconds, its_rule, "the same graph")
+    traffic, traffic_src = measured_traffic(m, z, args.workload)
     out = {
         "metric": "PageRank edge-updates/s & %HBM roofline, mixed vs FP64 at equal L1 error",
         "value": value,
@@ -251,7 +263,7 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(m, z),
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "FP32 operator application y = d P v (GMRES): " + operator_name(m),
                      "bytes_per_sweep": algorithmic_bytes(m, n, weighted, "fp32"),
                      "ms_per_sweep": ms_f32 / n_f32},
@@ -321,6 +333,12 @@ def bench_ppr(args):
         _, st, its, bnd = dg.ppr_batch(seeds, scores=x, params=p)
         return st, bnd
 
+    # per-column sweep counts of the reference rule (FP64, L2 step <= 1e-8; not timed)
+    torch.cuda.current_stream().synchronize()
+    with torch.cuda.stream(stream):
+        x.fill_(1.0 / nn)
+    _, _, its_rule, _ = dg.ppr_batch(seeds, scores=x, params=capi.default_params(
+        scores_on_device=1, precision=capi.NKPR_FP64, tol=1e-8))
     solve(capi.NKPR_FP64, 1e-13)
     ref = x.clone()
     torch.cuda.synchronize()
@@ -336,6 +354,10 @@ def bench_ppr(args):
         err = float((x - ref).abs().sum(dim=1).max())
         return sec, sts, err
 
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_ppr_baseline(host_in_lists(c["colptr"], c["src"]), nn, m, seeds, its_rule,
+                               args.cpu_seconds)
     sec_m, st_m, err_m = timed(capi.NKPR_MIXED)
     sec_d, st_d, err_d = timed(capi.NKPR_FP64)
     s0 = st_m[0][0]
@@ -367,6 +389,7 @@ def bench_ppr(args):
                      "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "FP32 SpMM sweep", "bytes_per_sweep": b32,
                      "ms_per_sweep": ms32},
+        "cpu_baseline": cpu,
         "mixed": {"ms_per_solve": sec_m * 1e3, "sweeps_fp32": s0["sweeps_fp32"],
                   "sweeps_fp64": s0["sweeps_fp64"], "refinements": s0["refinements"],
                   "max_l1_error": err_m, "max_error_bound": s0["max_error_bound"],
@@ -445,9 +468,18 @@ def bench_graphs(args):
         st, its, mx = dg.run_batch(x, params=p)
         return st
 
+    # per-graph sweep counts of the reference rule (FP64, L2 step <= 1e-8; not timed)
+    torch.cuda.current_stream().synchronize()
+    with torch.cuda.stream(stream):
+        x.copy_(start)
+    _, its_rule, _ = dg.run_batch(x, params=capi.default_params(
+        scores_on_device=1, precision=capi.NKPR_FP64, tol=1e-8))
     solve(capi.NKPR_FP64, 1e-13)
     torch.cuda.synchronize()
     ref = x.clone()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_graphs_baseline(sizes, gseeds, its_rule, args.cpu_seconds)
 
     def timed(precision):
         for _ in range(args.warmup):
@@ -463,6 +495,7 @@ def bench_graphs(args):
     sec_m, st_m, err_m = timed(capi.NKPR_MIXED)
     sec_d, st_d, err_d = timed(capi.NKPR_FP64)
     s0, d0 = st_m[0], st_d[0]
+    traffic, traffic_src = measured_traffic(m, z, "graphs")
     b32 = algorithmic_bytes(m, z, False, "fp32")
     b64 = algorithmic_bytes(m, z, False, "fp64")
     ms32 = sum(st["ms_sweeps_fp32"] for st in st_m) / max(1, sum(st["sweeps_fp32"] for st in st_m))
@@ -486,9 +519,11 @@ def bench_graphs(args):
                                "error <= %g per graph (damp 0.85, no sink handling)"
                                % (G, args.tol), "graphs": G, "z": z, "m": m},
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "FP32 pull sweep (GMRES operator)", "bytes_per_sweep": b32,
-                     "ms_per_sweep": ms32},
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": "FP32 GMRES operator: " + operator_name(m),
+                     "bytes_per_sweep": b32, "ms_per_sweep": ms32},
+        "cpu_baseline": cpu,
         "mixed": {"ms_per_solve": sec_m * 1e3, "sweeps_fp32": s0["sweeps_fp32"],
                   "sweeps_fp64": s0["sweeps_fp64"], "refinements": s0["refinements"],
                   "max_l1_error": err_m, "l1_error_bound": s0["l1_error_bound"]},
@@ -548,6 +583,22 @@ def bench_adjoint(args):
             name, f * 1e3, res[name]["forward_sweeps"], b * 1e3, res[name]["backward_sweeps"],
             b / f))
     r = res["mixed"]
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        og = host_in_lists(g["colptr"], g["src"], g["weight"])
+        threads = cpu_threads()
+        t = time.perf_counter()
+        _, it, _ = oracle.pagerank(og, threads=threads)
+        tf = time.perf_counter() - t
+        log("cpu baseline: forward run() %.1f ms (%d sweeps) on %d threads" % (tf * 1e3, it,
+                                                                              threads))
+        cpu = {"value": tf * 1e3, "unit": "ms per forward PageRank::run", "cores": threads,
+               "kind": "port",
+               "sample": "the oracle's PageRank::run on the same graph and weights to the "
+                         "reference rule (L2 step <= 1e-8, %d sweeps) on %d host threads; the "
+                         "reference has no backward pass" % (it, threads),
+               "host": host_info()}
     out = {"metric": "differentiable PageRank forward + backward time, backward / forward",
            "value": r["backward_over_forward"], "unit": "ratio", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
@@ -556,7 +607,108 @@ def bench_adjoint(args):
            "config": {"workload": "forward and adjoint of PageRank on directed R-MAT scale 20, "
                                   "edge factor 16, FP32 weights U(0.5,1.5), loss sum c_i x_i, "
                                   "c ~ N(0,1), seed 5; tol %g" % args.tol, "n": z, "m": m},
-           "mixed": r, "fp32": res["fp32"]}
+           "mixed": r, "fp32": res["fp32"], "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def bench_algebraic(args):
+    """AlgebraicPageRank (AlgebraicPageRank.hpp:94-119) on NetworKit's CSR matrix of the
+    benchmark's R-MAT graph: M = d A^T D^-1 as CSRGeneralMatrix<double> (uint64 row offsets and
+    column indices, FP64 values), resident in HBM.  One step = one nkpr_algebraic_pagerank run
+    from 1/n to ||y - x||_2 <= tol (the reference's rule), then y / sum(y).  The dominant kernel
+    is the product step y = M x + (1-d)/n with ||y - x||^2 (nkpr_csr_pagerank_step), timed with
+    HIP events on its stream."""
+    import torch
+
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.time()
+    g = gen.rmat(args.scale, args.edge_factor, seed=2, device="cuda")
+    z = g["z"]
+    colptr, src = g["colptr"], g["src"].to(torch.int64)
+    w = g["weight"].to(torch.float64)
+    # M(v, u) = d w(u, v) / outdeg_w(u): the rows are the in-lists, the columns their sources
+    deg = torch.zeros(z, dtype=torch.float64, device=dev).index_add_(0, src, w)
+    vals = 0.85 * w / deg[src]
+    del w, deg
+    mat = capi.DeviceCSR(z, z, colptr, src, vals)
+    nnz = mat.nnz
+    del g, vals
+    torch.cuda.synchronize()
+    log("algebraic: M with %d rows, %d entries, built in %.1f s" % (z, nnz, time.time() - t0))
+    scores = torch.empty(z, dtype=torch.float64, device=dev)
+    its = 0
+    for _ in range(args.warmup):
+        _, its = mat.algebraic_pagerank(tol=args.tol, scores=scores)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        _, its = mat.algebraic_pagerank(tol=args.tol, scores=scores)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / args.steps
+    # the product step alone, on torch's stream, with HIP events
+    x = torch.full((z,), 1.0 / z, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    sq = torch.empty(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    reps = 20
+    mat.pagerank_step(x, 0.15 / z, y, sq, stream=stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(reps):
+        a, b = (x, y) if i % 2 == 0 else (y, x)
+        mat.pagerank_step(a, 0.15 / z, b, sq, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / reps
+    bytes_step = 24 * nnz + 24 * z
+    ach = bytes_step / (ms_step / 1e3) / 1e9
+    log("algebraic: %.1f ms per run (%d iterations); product step %.3f ms = %.0f GB/s" % (
+        sec * 1e3, its, ms_step, ach))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        hm = oracle.CSR(z, z, mat.row_idx.cpu().numpy().view(np.uint64),
+                        mat.col_idx.cpu().numpy().view(np.uint64), mat.values.cpu().numpy())
+        threads = int(oracle.num_threads())
+        t = time.perf_counter()
+        oracle.algebraic_pagerank(hm, tol=0.0, max_iterations=1)
+        t1 = time.perf_counter() - t
+        k = int(max(2, min(40, args.cpu_seconds / max(t1, 1e-3))))
+        t = time.perf_counter()
+        oracle.algebraic_pagerank(hm, tol=0.0, max_iterations=k)
+        tk = time.perf_counter() - t
+        per = max((tk - t1) / (k - 1), 1e-9)
+        log("cpu baseline: run() with 1 iteration %.2f s, with %d %.2f s on %d threads" % (
+            t1, k, tk, threads))
+        cpu = {"value": nnz * k / tk, "unit": "edge-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": "the oracle's AlgebraicPageRank::run on the same matrix capped at %d "
+                         "iterations (%.2f s) and at 1 (%.2f s) on %d host threads" % (
+                             k, tk, t1, threads),
+               "ms_per_iteration": per * 1e3,
+               "run_to_reference_rule_s_extrapolated": max(t1 - per, 0.0) + its * per,
+               "host": host_info()}
+    out = {"metric": "AlgebraicPageRank edge-updates/s (NetworKit CSR, FP64, L2 step <= tol)",
+           "value": nnz * its / sec, "unit": "edge-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+           "data": "synthetic",
+           "config": {"workload": "AlgebraicPageRank run (damp 0.85, tol %g, L2 step) on the "
+                                  "CSR matrix d A^T D^-1 of a directed R-MAT graph, scale %d, "
+                                  "edge factor %d, a/b/c/d=.57/.19/.19/.05, duplicates and "
+                                  "self-loops removed, FP32 weights U(0.5,1.5) as FP64, seed 2"
+                                  % (args.tol, args.scale, args.edge_factor),
+                      "n": z, "nnz": nnz, "iterations": its},
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "product step y = M x + (1-d)/n, ||y - x||^2 "
+                                  "(csr_short_kernel + csr_long_kernel + csr_final_kernel)",
+                        "bytes_per_sweep": bytes_step, "ms_per_sweep": ms_step},
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
@@ -569,45 +721,155 @@ def operator_name(m):
             else "pull sweep (sweep_kernel<float> + combine_kernel<float>)")
 
 
-def measured_traffic(m, n):
-    """HBM bytes per application of the GMRES operator from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the counters' reading of a 1 GiB
-    copy; None when the summary was taken on another graph or another operator."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "pagerank_rmat26_traffic.json")
+def measured_traffic(m, n, workload):
+    """HBM bytes per application of the GMRES operator, read from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the counters' reading of a
+    1 GiB copy; not measured in this run).  Returns (bytes, source) with bytes None when no
+    summary exists for this graph and operator."""
+    name = {"pagerank": "pagerank_rmat26_traffic.json",
+            "rhg": "pagerank_rhg10m_traffic.json",
+            "graphs": "pagerank_er512_batch_traffic.json"}.get(workload)
+    if name is None:
+        return None, None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, None
     if t.get("m") != m or t.get("n") != n or t.get("operator") != operator_name(m):
-        return None
-    return t["operator_traffic_bytes"]
+        return None, None
+    return t["operator_traffic_bytes"], {"source": "profiles/" + name, "measured_in_run": False}
 
 
-def cpu_baseline(g, z, m, seconds):
-    """The oracle (a restatement of NetworKit's PageRank loop with NetworKit's element types:
-    uint64 ids, FP64 weights, FP64 scores, guided OpenMP schedule) timed on the host cores on a
-    bounded sample: whole reference sweeps over the same graph until ~`seconds` of CPU work."""
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the process's CPU affinity and lscpu's
+    model, sockets, cores per socket and threads per core."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                             text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        out = ""
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            info[keys[k.strip()]] = v.strip()
+    return info
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+
+def host_in_lists(colptr, src, weight=None):
+    """NetworKit element types on the host (uint64 offsets and ids, FP64 weights) from the
+    device CSC of a workload."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    t0 = time.time()
-    colptr = g["colptr"].cpu().numpy().astype(np.uint64)
-    src = g["src"].cpu().numpy().astype(np.uint64)
-    w = None if g["weight"] is None else g["weight"].cpu().numpy().astype(np.float64)
-    og = oracle.InLists(z, colptr, src, w)
-    log("cpu baseline: graph on host in %.1f s, %d threads" % (time.time() - t0, threads))
-    t1 = oracle.time_sweeps(og, 0.85, 1, threads)
-    sweeps = max(1, min(20, int(seconds / max(t1, 1e-3))))
-    if sweeps > 1:
-        tt = oracle.time_sweeps(og, 0.85, sweeps, threads)
-    else:
-        tt = t1
-    log("cpu baseline: %d sweeps in %.2f s" % (sweeps, tt))
-    return {"value": m * sweeps / tt, "unit": "edge-updates/s", "cores": threads,
+    cp = colptr.cpu().numpy() if hasattr(colptr, "cpu") else np.asarray(colptr)
+    sr = src.cpu().numpy() if hasattr(src, "cpu") else np.asarray(src)
+    w = None
+    if weight is not None:
+        w = (weight.cpu().numpy() if hasattr(weight, "cpu") else np.asarray(weight)).astype(np.float64)
+    return oracle.InLists(len(cp) - 1, cp.astype(np.uint64), sr.astype(np.uint64), w)
+
+
+def cpu_run_baseline(og, m, seconds, its_ref, what):
+    """The oracle's PageRank::run (a restatement of PageRank.cpp:20-104 in NetworKit's element
+    types, guided OpenMP schedule) timed on the host cores on a bounded sample: run() capped at
+    1 iteration, then at k iterations (k sized to ~`seconds` of work).  Their difference gives
+    the per-sweep time and the set-up (the weighted out-degrees of PageRank.cpp:29-30); the
+    time of run() to the reference rule (L2 step <= 1e-8, its_ref sweeps, the count the GPU's
+    FP64 run with that rule took, which equals the oracle's) is extrapolated from them."""
+    import oracle
+    threads = cpu_threads()
+    t = time.perf_counter()
+    oracle.pagerank(og, tol=0.0, max_iterations=1, threads=threads)
+    t1 = time.perf_counter() - t
+    k = int(max(2, min(40, seconds / max(t1, 1e-3))))
+    t = time.perf_counter()
+    oracle.pagerank(og, tol=0.0, max_iterations=k, threads=threads)
+    tk = time.perf_counter() - t
+    per = max((tk - t1) / (k - 1), 1e-9)
+    setup = max(t1 - per, 0.0)
+    est = setup + its_ref * per
+    log("cpu baseline: run() with 1 sweep %.2f s, with %d sweeps %.2f s on %d threads; "
+        "%.3f s per sweep, set-up %.2f s" % (t1, k, tk, threads, per, setup))
+    return {"value": m * k / tk, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+            "sample": "%s: the oracle's PageRank::run capped at %d iterations (%.2f s) and at 1 "
+                      "(%.2f s) on %d host threads" % (what, k, tk, t1, threads),
+            "ms_per_sweep": per * 1e3, "setup_ms": setup * 1e3,
+            "reference_rule_sweeps": int(its_ref),
+            "run_to_reference_rule_s_extrapolated": est, "host": host_info()}
+
+
+def cpu_ppr_baseline(og, nn, m, seeds, its_rule, seconds):
+    """The oracle's PageRank::run with each column's teleport vector (the personalised
+    replacement of (1-d)/n; the reference has no teleport vector, so its loop is otherwise
+    unchanged) to the reference rule, one column after another on the host cores, for the
+    first columns until ~`seconds`; the whole batch is extrapolated from the rate per sweep and
+    the per-column sweep counts of the GPU's FP64 run with that rule."""
+    import oracle
+    threads = cpu_threads()
+    done, sweeps, t_all = 0, 0, 0.0
+    for j, sd in enumerate(seeds):
+        tv = np.zeros(nn)
+        np.add.at(tv, np.asarray(sd, np.int64), 1.0)
+        tv /= tv.sum()
+        t = time.perf_counter()
+        _, it, _ = oracle.pagerank(og, teleport=tv, threads=threads)
+        t_all += time.perf_counter() - t
+        done += 1
+        sweeps += it
+        if it != int(its_rule[j]):
+            log("cpu baseline: column %d took %d sweeps, the GPU %d" % (j, it, int(its_rule[j])))
+        if t_all >= seconds:
+            break
+    per_sweep = t_all / max(sweeps, 1)
+    est = per_sweep * float(np.sum(its_rule))
+    log("cpu baseline: %d columns, %d sweeps in %.2f s on %d threads; batch of %d columns "
+        "~%.1f s" % (done, sweeps, t_all, threads, len(seeds), est))
+    return {"value": m * sweeps / t_all, "unit": "column-edge-updates/s", "cores": threads,
             "kind": "port",
-            "sample": "%d FP64 reference sweeps (NetworKit element types) over the full graph "
-                      "on %d host threads; %.2f s" % (sweeps, threads, tt)}
+            "sample": "the oracle's PageRank::run with the teleport vectors of the first %d "
+                      "columns, each to the reference rule (L2 step <= 1e-8), one after another "
+                      "on %d host threads: %d sweeps in %.2f s" % (done, threads, sweeps, t_all),
+            "batch_s_extrapolated": est, "host": host_info()}
+
+
+def cpu_graphs_baseline(sizes, gseeds, its_rule, seconds):
+    """The oracle's PageRank::run on the graphs of the batch one after another (each generated
+    again from its seed) to the reference rule, until ~`seconds`; the whole batch is
+    extrapolated from the rate and the per-graph sweep counts of the GPU's FP64 run."""
+    import oracle
+    from networkit_amd import generators as gen
+    threads = cpu_threads()
+    done, eu, t_all, m_all = 0, 0.0, 0.0, 0
+    for i in range(len(sizes)):
+        n = int(sizes[i])
+        u, v = gen.erdos_renyi(n, 12.0 / (n - 1), seed=int(gseeds[i]), directed=False)
+        og = oracle.in_lists(n, u, v, None, False)
+        t = time.perf_counter()
+        _, it, _ = oracle.pagerank(og, threads=threads)
+        t_all += time.perf_counter() - t
+        if it != int(its_rule[i]):
+            log("cpu baseline: graph %d took %d sweeps, the GPU %d" % (i, it, int(its_rule[i])))
+        eu += og.m * it
+        m_all += og.m
+        done += 1
+        if t_all >= seconds:
+            break
+    rate = eu / t_all
+    log("cpu baseline: %d graphs (%d arcs) in %.2f s on %d threads" % (done, m_all, t_all,
+                                                                       threads))
+    return {"value": rate, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+            "sample": "the oracle's PageRank::run on the first %d graphs of the batch, each to "
+                      "the reference rule (L2 step <= 1e-8), one after another on %d host "
+                      "threads: %.2f s" % (done, threads, t_all),
+            "host": host_info()}
 
 
 if __name__ == "__main__":

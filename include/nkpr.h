@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define NKPR_ABI_VERSION 2
+#define NKPR_ABI_VERSION 3
 
 /* status codes */
 #define NKPR_OK 0
@@ -268,6 +268,44 @@ typedef struct {
 int nkpr_power_params_default(nkpr_power_params *p);
 /* scores[z]: output only (both reference loops start from 1, with no warm start). */
 int nkpr_power_run(nkpr_graph *g, const nkpr_power_params *p, double *scores, nkpr_stats *stats);
+
+/* ---- CSR matrices and AlgebraicPageRank (NetworKit's algebraic module) -------------------
+ * A CSRGeneralMatrix<double> in NetworKit's own layout (include/networkit/algebraic/
+ * CSRGeneralMatrix.hpp: rowIdx, columnIdx, nonZeros), every array in device memory of the
+ * current device: row_idx[rows + 1] rises from 0 to nnz, col_idx[nnz] < cols, values[nnz].
+ * Entries of a row are summed in storage order groups; duplicates stay separate entries. */
+typedef struct {
+    uint64_t rows;
+    uint64_t cols;
+    uint64_t nnz;
+    const uint64_t *row_idx;
+    const uint64_t *col_idx;
+    const double *values;
+} nkpr_csr;
+
+/* Replaces CSRGeneralMatrix::operator*(const Vector&) (CSRGeneralMatrix.hpp:563-578):
+ * y[i] = sum over row i of values[k] * x[col_idx[k]].  x[cols] and y[rows] are distinct device
+ * vectors.  Enqueued on `stream` (a hipStream_t; NULL is the null stream); returns without
+ * waiting.  The matrix structure is taken as given (see nkpr_algebraic_pagerank for a run
+ * that checks it). */
+int nkpr_csr_spmv(const nkpr_csr *a, const double *x, double *y, void *stream);
+
+/* One iteration of AlgebraicPageRank::run (AlgebraicPageRank.hpp:97-101) on a square matrix
+ * M: y = M x + shift and *step_sq = ||y - x||_2^2 (FP64, a device pointer; summed in an order
+ * fixed by the matrix).  Stream-ordered like nkpr_csr_spmv; its partial sums take
+ * at most 4096 doubles from the stream's memory pool for the duration of the call. */
+int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, double *y,
+                           double *step_sq, void *stream);
+
+/* Replaces AlgebraicPageRank<CSRMatrix>::run (AlgebraicPageRank.hpp:94-119) on the matrix M
+ * its constructor builds (:35-54, M(j, i) = damp * A(i, j) / rowsum_i(A)): from 1/n in every
+ * row, y = M x + (1 - damp)/n until ||y - x||_2 <= tol (at least one iteration), then
+ * y / sum(y).  n = rows (every slot, as the reference; no exists[] handling).  The reference
+ * has no iteration cap: pass UINT64_MAX for none.  scores[rows] is a host pointer, or a device
+ * pointer when scores_on_device = 1.  The matrix is checked once (NKPR_ERR_INVALID when
+ * row_idx or a column index is out of place). Blocks until done. */
+int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
+                            double *scores, int32_t scores_on_device, uint64_t *iterations);
 
 /* Centrality::ranking: ids of the existing nodes ordered by score descending, ties by id
  * ascending; ids_out and scores_out (nullable) receive *count = number of existing nodes

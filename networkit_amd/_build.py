@@ -18,10 +18,12 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_INCLUDE = "/opt/rocm/include"
+ROCM_LIB = "/opt/rocm/lib"
 
 HIP_SOURCES = ["nkpr_sweep.hip", "nkpr_graph.hip", "nkpr_krylov.hip", "nkpr_batch.hip",
                "nkpr_adjoint.hip", "nkpr_groups.hip", "nkpr_binned.hip", "nkpr_power.hip",
-               "nkpr_capi.hip"]
+               "nkpr_csr.hip", "nkpr_capi.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # packed FP32 instructions stay off (some of their forms are unreliable on this pool)
@@ -31,7 +33,7 @@ HIP_FLAGS = [
 ]
 LIB_NKPR = os.path.join(HERE, "libnkpr.so")
 LIB_NKGEN = os.path.join(HERE, "libnkgen.so")
-CXX_SOURCES = ["Graph.cpp", "PageRank.cpp", "EigenvectorCentrality.cpp"]
+CXX_SOURCES = ["Graph.cpp", "PageRank.cpp", "EigenvectorCentrality.cpp", "AlgebraicPageRank.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -80,7 +82,9 @@ def build_cython():
     import numpy
     cxx = os.environ.get("CXX", "g++")
     flags = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function",
-             "-I" + py_inc, "-I" + INCLUDE, "-I" + CSRC, "-I" + numpy.get_include()]
+             "-I" + py_inc, "-I" + INCLUDE, "-I" + CSRC, "-I" + numpy.get_include(),
+             # the HIP runtime API (host code only: device copies of CSR matrices)
+             "-D__HIP_PLATFORM_AMD__", "-I" + ROCM_INCLUDE]
     objs = []
     for src in [cpp] + [os.path.join(CSRC, s) for s in CXX_SOURCES]:
         obj = os.path.join(BUILD, os.path.basename(src).replace(".cpp", ".host.o"))
@@ -88,7 +92,8 @@ def build_cython():
         objs.append(obj)
     tmp = out + ".tmp"
     _run([cxx, "-shared", "-fopenmp", "-o", tmp] + objs +
-         ["-L" + HERE, "-lnkpr", "-Wl,-rpath,$ORIGIN"])
+         ["-L" + HERE, "-lnkpr", "-Wl,-rpath,$ORIGIN", "-L" + ROCM_LIB, "-lamdhip64",
+          "-Wl,-rpath," + ROCM_LIB])
     os.replace(tmp, out)
     return out
 

@@ -117,6 +117,11 @@ cdef extern from "EigenvectorCentrality.hpp" namespace "NetworKit":
         _EdgeDirection edgeDirection
         const nkpr_stats& lastStats()
 
+cdef extern from "AlgebraicPageRank.hpp" namespace "NetworKit":
+    cdef cppclass _AlgebraicPageRank "NetworKit::AlgebraicPageRank<NetworKit::CSRMatrix>"(_Centrality):
+        _AlgebraicPageRank(_Graph, double, double) except +
+        count numberOfIterations() except +
+
 
 cdef class Graph:
     """Graph(n=0, weighted=False, directed=False): NetworKit-compatible graph (subset)."""
@@ -417,3 +422,20 @@ cdef class KatzCentrality(Centrality):
     def stats(self):
         """Statistics of the last run: iterations, final change and value of the norm."""
         return _power_stats((<_KatzCentrality*>(self._this)).lastStats())
+
+
+cdef class AlgebraicPageRank(Centrality):
+    """AlgebraicPageRank(G, damp=0.85, tol=1e-8)
+
+    PageRank on the algebraic module's CSR matrix (C++ NetworKit::AlgebraicPageRank<CSRMatrix>,
+    AlgebraicPageRank.hpp:25-150; not exported by NetworKit's Python package): the matrix
+    M = damp A^T D^-1 is built on the host at construction, run() iterates M x + (1-damp)/n on
+    the MI355X until the L2 step is at most tol, then divides by the sum.  Every slot of the
+    graph is a row (deleted nodes included, as in the reference); maximum() is 1."""
+
+    def __cinit__(self, Graph G, double damp=0.85, double tol=1e-8):
+        self._G = G
+        self._this = new _AlgebraicPageRank(G._this[0], damp, tol)
+
+    def numberOfIterations(self):
+        return (<_AlgebraicPageRank*>(self._this)).numberOfIterations()

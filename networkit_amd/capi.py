@@ -29,8 +29,9 @@ EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch
             "nkpr_graph_destroy", "nkpr_graph_get_info", "nkpr_graph_stream",
             "nkpr_graph_set_weights", "nkpr_run", "nkpr_run_batch", "nkpr_adjoint",
             "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version",
-            "nkpr_power_params_default", "nkpr_power_run"]
-ABI_VERSION = 2
+            "nkpr_power_params_default", "nkpr_power_run", "nkpr_csr_spmv",
+            "nkpr_csr_pagerank_step", "nkpr_algebraic_pagerank"]
+ABI_VERSION = 3
 
 
 class Csc(ctypes.Structure):
@@ -79,6 +80,12 @@ class PowerParams(ctypes.Structure):
                 ("max_iterations", ctypes.c_uint64), ("interrupt", ctypes.c_void_p)]
 
 
+class CsrMatrix(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_uint64), ("cols", ctypes.c_uint64), ("nnz", ctypes.c_uint64),
+                ("row_idx", ctypes.c_void_p), ("col_idx", ctypes.c_void_p),
+                ("values", ctypes.c_void_p)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint64), ("sweeps_fp32", ctypes.c_uint64),
                 ("sweeps_fp64", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
@@ -123,6 +130,12 @@ def lib():
     L.nkpr_ppr_batch.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_uint32, vp, vp, vp, vp,
                                  ctypes.POINTER(BatchStats), vp, vp]
     L.nkpr_ranking.argtypes = [ctypes.c_uint64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.nkpr_csr_spmv.argtypes = [ctypes.POINTER(CsrMatrix), vp, vp, vp]
+    L.nkpr_csr_pagerank_step.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.c_double, vp, vp, vp,
+                                         vp]
+    L.nkpr_algebraic_pagerank.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_uint64, vp, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_uint64)]
     L.nkpr_last_error.restype = ctypes.c_char_p
     L.nkpr_abi_version.restype = ctypes.c_int
     for name in EXPORTED:
@@ -351,3 +364,67 @@ def ranking(scores, exists=None):
     check(lib().nkpr_ranking(z, _addr(scores), _addr(ex), _addr(ids), _addr(vals),
                              ctypes.byref(k)))
     return ids[:k.value], vals[:k.value]
+
+
+class DeviceCSR:
+    """A CSRGeneralMatrix<double> in device memory (torch CUDA tensors: row_idx and col_idx
+    int64 viewed as NetworKit's uint64 index, values float64), for nkpr_csr_spmv,
+    nkpr_csr_pagerank_step and nkpr_algebraic_pagerank.  ``row_idx`` etc. may be numpy
+    arrays; they are copied to the current device."""
+
+    def __init__(self, rows, cols, row_idx, col_idx, values):
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+
+        def on_dev(a, dt):
+            if isinstance(a, np.ndarray):
+                a = torch.from_numpy(np.ascontiguousarray(a).view(np.int64) if a.dtype == np.uint64
+                                     else np.ascontiguousarray(a))
+            return a.to(device=dev, dtype=dt).contiguous()
+
+        self.row_idx = on_dev(row_idx, torch.int64)
+        self.col_idx = on_dev(col_idx, torch.int64)
+        self.values = on_dev(values, torch.float64)
+        self.rows, self.cols = int(rows), int(cols)
+        self.nnz = int(self.col_idx.numel())
+        assert self.row_idx.numel() == self.rows + 1 and self.values.numel() == self.nnz
+
+    def struct(self):
+        return CsrMatrix(rows=self.rows, cols=self.cols, nnz=self.nnz,
+                         row_idx=self.row_idx.data_ptr(), col_idx=self.col_idx.data_ptr(),
+                         values=self.values.data_ptr())
+
+    def spmv(self, x, y=None, stream=None):
+        """y = A x on the device (x a float64 CUDA tensor of cols entries)."""
+        import torch
+        if y is None:
+            y = torch.empty(self.rows, dtype=torch.float64, device=x.device)
+        m = self.struct()
+        check(lib().nkpr_csr_spmv(ctypes.byref(m), x.data_ptr(), y.data_ptr(), stream))
+        return y
+
+    def pagerank_step(self, x, shift, y=None, step_sq=None, stream=None):
+        """y = M x + shift and ||y - x||^2 (a one-element float64 CUDA tensor)."""
+        import torch
+        if y is None:
+            y = torch.empty(self.rows, dtype=torch.float64, device=x.device)
+        if step_sq is None:
+            step_sq = torch.empty(1, dtype=torch.float64, device=x.device)
+        m = self.struct()
+        check(lib().nkpr_csr_pagerank_step(ctypes.byref(m), float(shift), x.data_ptr(),
+                                           y.data_ptr(), step_sq.data_ptr(), stream))
+        return y, step_sq
+
+    def algebraic_pagerank(self, damp=0.85, tol=1e-8, max_iterations=2**64 - 1, scores=None):
+        """nkpr_algebraic_pagerank on this matrix as M; returns (scores, iterations) with
+        scores a numpy array (or the given CUDA tensor, filled in place)."""
+        m = self.struct()
+        it = ctypes.c_uint64(0)
+        if scores is None:
+            out = np.zeros(self.rows, dtype=np.float64)
+            check(lib().nkpr_algebraic_pagerank(ctypes.byref(m), damp, tol, int(max_iterations),
+                                                _addr(out), 0, ctypes.byref(it)))
+            return out, int(it.value)
+        check(lib().nkpr_algebraic_pagerank(ctypes.byref(m), damp, tol, int(max_iterations),
+                                            scores.data_ptr(), 1, ctypes.byref(it)))
+        return scores, int(it.value)

@@ -60,6 +60,20 @@ public:
             if (exists[v]) handle(v);
     }
 
+    // Graph::forEdges (Graph.hpp:1941-1945, 1895-1915): u ascending, each out-list in
+    // storage order; an undirected edge is visited once, at its endpoint u >= v.
+    // handle(u, v, weight).
+    template <typename L> void forEdges(L handle) const {
+        for (node u = 0; u < z; ++u) {
+            const auto &lst = outEdges[u];
+            for (size_t i = 0; i < lst.size(); ++i) {
+                const node v = lst[i];
+                if (directed || u >= v)
+                    handle(u, v, weighted ? outEdgeWeights[u][i] : defaultEdgeWeight);
+            }
+        }
+    }
+
     // Flattened pull lists: row u holds the in-neighbours of u (directed) or the adjacency of
     // u (undirected), in insertion order.  Weights are filled only for weighted graphs.
     void exportInLists(std::vector<uint64_t> &colptr, std::vector<int32_t> &src,

@@ -641,6 +641,25 @@ int nkpr_ppr_batch(nkpr_graph *g, const nkpr_params *p, uint32_t k,
     });
 }
 
+int nkpr_csr_spmv(const nkpr_csr *a, const double *x, double *y, void *stream) {
+    return guarded([&] { csr_spmv_impl(a, x, y, (hipStream_t)stream); });
+}
+
+int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, double *y,
+                           double *step_sq, void *stream) {
+    return guarded([&] {
+        csr_step_impl(m, shift, x, y, step_sq, nullptr, 0, (hipStream_t)stream);
+    });
+}
+
+int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
+                            double *scores, int32_t scores_on_device, uint64_t *iterations) {
+    return guarded([&] {
+        algebraic_pagerank_impl(m, damp, tol, max_iterations, scores, scores_on_device,
+                                iterations);
+    });
+}
+
 int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_t *ids_out,
                  double *scores_out, uint64_t *count) {
     return guarded([&] {

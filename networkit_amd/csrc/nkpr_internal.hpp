@@ -331,6 +331,15 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                     double *maximum, nkpr_stats *st);
 // nkpr_power_run (nkpr_power.hip): eigenvector and Katz centrality
 void power_run_impl(nkpr_graph *g, const nkpr_power_params *p, double *scores, nkpr_stats *st);
+
+// CSR products and AlgebraicPageRank (nkpr_csr.hip).  csr_step_impl takes its partials from
+// `scratch` (scratch_count doubles) when that is not NULL, else from the stream's pool.
+uint32_t csr_step_partials(uint64_t rows, uint64_t nnz);
+void csr_spmv_impl(const nkpr_csr *a, const double *x, double *y, hipStream_t s);
+void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, double *step_sq,
+                   double *scratch, uint64_t scratch_count, hipStream_t s);
+void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
+                             double *scores, int32_t scores_on_device, uint64_t *iterations);
 void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint64_t *seed_off,
                     const int32_t *seeds, const double *seed_w, double *scores,
                     nkpr_batch_stats *st, uint64_t *col_iterations, double *col_bound);

@@ -5,8 +5,9 @@ import this package, and only as the checker (or, for the baseline, as the thing
 host cores).  The product (``networkit_amd``) never imports it and has no CPU fallback.
 
 The arithmetic lives in ``pagerank_oracle.c`` (a restatement of
-networkit/cpp/centrality/PageRank.cpp:20-104) and ``power_oracle.c`` (EigenvectorCentrality.cpp:
-21-66, KatzCentrality.cpp:28-76); this module loads it with ctypes and adds the
+networkit/cpp/centrality/PageRank.cpp:20-104), ``power_oracle.c`` (EigenvectorCentrality.cpp:
+21-66, KatzCentrality.cpp:28-76) and ``algebraic_oracle.c`` (AlgebraicPageRank.hpp:35-119,
+CSRGeneralMatrix.hpp:151-182,563-578); this module loads it with ctypes and adds the
 NetworKit-semantics edge-list -> in-list conversion (Graph.cpp:549-597, Graph.hpp:1918-1938)
 that the C code consumes.
 
@@ -22,6 +23,12 @@ Parity status of each function is pinned in tests/test_oracle.py:
                   solve of (I - alpha A) x = alpha A 1 + beta 1, regular-graph invariant
   in_lists        hand-built NetworKit adjacency of small graphs (directed, undirected,
                   self-loops, multi-edges)
+  for_edges       hand-built forEdges order of small graphs (directed, undirected, self-loops)
+  csr_spmv        scipy.sparse products
+  algebraic_matrix  dense construction of d A^T D^-1 from the edge list
+  algebraic_pagerank  reference known answers (AlgebraicPageRankGTest.cpp:21-94), sparse direct
+                  solve of (I - M) x = (1-d)/n 1, equality with pagerank() on graphs without
+                  deleted nodes
 """
 import ctypes
 import os
@@ -65,8 +72,6 @@ def _load():
     lib.oracle_ranking.restype = u64
     lib.oracle_out_degrees.argtypes = [u64, p, p, p, p, p]
     lib.oracle_out_degrees.restype = None
-    lib.oracle_time_sweeps.argtypes = [u64, p, p, p, ctypes.c_double, u64, ctypes.c_int, p]
-    lib.oracle_time_sweeps.restype = ctypes.c_double
     lib.oracle_eigenvector.argtypes = [u64, p, p, p, p, ctypes.c_double, u64, p,
                                        ctypes.POINTER(u64)]
     lib.oracle_eigenvector.restype = ctypes.c_int
@@ -75,6 +80,16 @@ def _load():
     lib.oracle_katz.restype = ctypes.c_int
     lib.oracle_katz_default_alpha.argtypes = [u64, p, p, ctypes.c_int]
     lib.oracle_katz_default_alpha.restype = ctypes.c_double
+    lib.oracle_csr_spmv.argtypes = [u64, p, p, p, p, p]
+    lib.oracle_csr_spmv.restype = None
+    lib.oracle_algebraic_nnz.argtypes = [u64, p, p, ctypes.c_int]
+    lib.oracle_algebraic_nnz.restype = u64
+    lib.oracle_algebraic_matrix.argtypes = [u64, u64, p, p, p, ctypes.c_int, ctypes.c_double,
+                                            p, p, p]
+    lib.oracle_algebraic_matrix.restype = ctypes.c_int
+    lib.oracle_algebraic_pagerank.argtypes = [u64, p, p, p, ctypes.c_double, ctypes.c_double,
+                                              u64, p, ctypes.POINTER(u64)]
+    lib.oracle_algebraic_pagerank.restype = ctypes.c_int
     lib.oracle_num_threads.argtypes = []
     lib.oracle_num_threads.restype = ctypes.c_int
     _lib = lib
@@ -179,14 +194,6 @@ def out_degrees(g):
     return deg, cnt
 
 
-def time_sweeps(g, damp=0.85, sweeps=3, threads=0):
-    """Wall seconds of ``sweeps`` reference sweeps on the host cores (CPU baseline)."""
-    lib = _load()
-    x = np.full(g.z, 1.0 / max(g.z, 1), dtype=np.float64)
-    return lib.oracle_time_sweeps(g.z, _ptr(g.in_ptr), _ptr(g.in_src), _ptr(g.in_w), damp,
-                                  int(sweeps), int(threads), _ptr(x))
-
-
 def eigenvector(g, tol=1e-8, max_iterations=2**64 - 1):
     """The restated EigenvectorCentrality::run on in-lists ``g``.  Returns (scores,
     iterations)."""
@@ -220,6 +227,96 @@ def katz(g, alpha=0.0, beta=0.1, tol=1e-8, max_iterations=2**64 - 1):
                          alpha, beta, tol, int(max_iterations), _ptr(scores), ctypes.byref(it))
     if rc != 0:
         raise RuntimeError("oracle_katz failed")
+    return scores, int(it.value)
+
+
+def for_edges(z, u, v, w=None, directed=True):
+    """The edges of the NetworKit Graph built by ``addEdge(u[i], v[i], w[i])`` in order, in
+    Graph::forEdges order (Graph.hpp:1941-1945, 1895-1915): u ascending, each out-list in
+    insertion order; an undirected edge is visited once, at its endpoint u >= v (the out-list of
+    an undirected graph holds an edge at both endpoints, a self-loop once).  Returns (eu, ev,
+    ew) with ew None when unweighted."""
+    u = np.asarray(u, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64)
+    wt = None if w is None else np.asarray(w, dtype=np.float64)
+    if directed:
+        rows, cols, seq, ws = u, v, np.arange(len(u), dtype=np.int64), wt
+    else:
+        loop = u == v
+        k = len(u)
+        rows = np.concatenate([u, v[~loop]])
+        cols = np.concatenate([v, u[~loop]])
+        seq = np.concatenate([2 * np.arange(k), 2 * np.arange(k)[~loop] + 1])
+        ws = None if wt is None else np.concatenate([wt, wt[~loop]])
+        keep = rows >= cols
+        rows, cols, seq = rows[keep], cols[keep], seq[keep]
+        ws = None if ws is None else ws[keep]
+    order = np.lexsort((seq, rows))
+    return (rows[order].astype(np.uint64), cols[order].astype(np.uint64),
+            None if ws is None else ws[order])
+
+
+class CSR:
+    """A CSRGeneralMatrix<double> in NetworKit's element types: row_idx[rows+1], col_idx[nnz]
+    (uint64), values[nnz] (float64)."""
+
+    def __init__(self, rows, cols, row_idx, col_idx, values):
+        self.rows = int(rows)
+        self.cols = int(cols)
+        self.row_idx = np.ascontiguousarray(row_idx, dtype=np.uint64)
+        self.col_idx = np.ascontiguousarray(col_idx, dtype=np.uint64)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+
+    @property
+    def nnz(self):
+        return int(self.row_idx[-1])
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.values, self.col_idx.astype(np.int64),
+                              self.row_idx.astype(np.int64)), shape=(self.rows, self.cols))
+
+
+def csr_spmv(a, x):
+    """CSRGeneralMatrix::operator*(const Vector&) (CSRGeneralMatrix.hpp:563-578)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.shape != (a.cols,):
+        raise ValueError("x must have one entry per column")
+    y = np.zeros(a.rows, dtype=np.float64)
+    _load().oracle_csr_spmv(a.rows, _ptr(a.row_idx), _ptr(a.col_idx), _ptr(a.values), _ptr(x),
+                            _ptr(y))
+    return y
+
+
+def algebraic_matrix(z, eu, ev, ew=None, directed=True, damp=0.85):
+    """The matrix M that AlgebraicPageRank(G, damp) builds (AlgebraicPageRank.hpp:35-54) from
+    G's edges in forEdges order (see for_edges)."""
+    lib = _load()
+    eu = np.ascontiguousarray(eu, dtype=np.uint64)
+    ev = np.ascontiguousarray(ev, dtype=np.uint64)
+    ew = None if ew is None else np.ascontiguousarray(ew, dtype=np.float64)
+    nnz = int(lib.oracle_algebraic_nnz(len(eu), _ptr(eu), _ptr(ev), int(bool(directed))))
+    row = np.zeros(z + 1, dtype=np.uint64)
+    col = np.zeros(max(nnz, 1), dtype=np.uint64)
+    val = np.zeros(max(nnz, 1), dtype=np.float64)
+    rc = lib.oracle_algebraic_matrix(z, len(eu), _ptr(eu), _ptr(ev), _ptr(ew),
+                                     int(bool(directed)), damp, _ptr(row), _ptr(col), _ptr(val))
+    if rc != 0:
+        raise RuntimeError("oracle_algebraic_matrix failed")
+    return CSR(z, z, row, col[:nnz], val[:nnz])
+
+
+def algebraic_pagerank(mat, damp=0.85, tol=1e-8, max_iterations=2**64 - 1):
+    """The restated AlgebraicPageRank::run (AlgebraicPageRank.hpp:94-119) on M.  Returns
+    (scores, iterations)."""
+    lib = _load()
+    scores = np.zeros(mat.rows, dtype=np.float64)
+    it = ctypes.c_uint64(0)
+    rc = lib.oracle_algebraic_pagerank(mat.rows, _ptr(mat.row_idx), _ptr(mat.col_idx),
+                                       _ptr(mat.values), damp, tol, int(max_iterations),
+                                       _ptr(scores), ctypes.byref(it))
+    if rc != 0:
+        raise RuntimeError("oracle_algebraic_pagerank failed")
     return scores, int(it.value)
 
 

@@ -224,64 +224,6 @@ uint64_t oracle_ranking(uint64_t z, const uint8_t *exists, const double *scores,
     return k;
 }
 
-/* Timing helper for the CPU baseline: runs exactly `sweeps` reference sweeps (the same loop
- * body as oracle_pagerank) on a caller-owned start vector and returns the wall time in
- * seconds.  Used to time a bounded sample of a large workload. */
-double oracle_time_sweeps(uint64_t z, const uint64_t *in_ptr, const uint64_t *in_src,
-                          const double *in_w, double damp, uint64_t sweeps, int num_threads,
-                          double *scores) {
-    uint64_t it = 0;
-    double t0, t1;
-    double *deg = (double *)malloc(sizeof(double) * (z ? z : 1));
-    double *pr = (double *)malloc(sizeof(double) * (z ? z : 1));
-    if (!deg || !pr) {
-        free(deg); free(pr);
-        return -1.0;
-    }
-#ifdef _OPENMP
-    if (num_threads > 0) omp_set_num_threads(num_threads);
-#else
-    (void)num_threads;
-#endif
-    oracle_out_degrees(z, in_ptr, in_src, in_w, deg, NULL);
-    memcpy(pr, scores, sizeof(double) * z);
-    const double teleport = (1.0 - damp) / (double)z;
-    double *x = scores, *y = pr;
-#ifdef _OPENMP
-    t0 = omp_get_wtime();
-#else
-    t0 = 0.0;
-#endif
-    for (it = 0; it < sweeps; ++it) {
-#pragma omp parallel for schedule(guided)
-        for (int64_t u = 0; u < (int64_t)z; ++u) {
-            double acc = 0.0;
-            for (uint64_t e = in_ptr[u]; e < in_ptr[u + 1]; ++e) {
-                uint64_t v = in_src[e];
-                double w = in_w ? in_w[e] : 1.0;
-                acc += x[v] * w / deg[v];
-            }
-            y[u] = acc * damp + teleport;
-        }
-        double s = 0.0;
-#pragma omp parallel for reduction(+ : s)
-        for (int64_t u = 0; u < (int64_t)z; ++u) {
-            double d = x[u] - y[u];
-            s += d * d;
-        }
-        (void)s;
-        double *t = x; x = y; y = t;
-    }
-#ifdef _OPENMP
-    t1 = omp_get_wtime();
-#else
-    t1 = 0.0;
-#endif
-    if (x != scores) memcpy(scores, x, sizeof(double) * z);
-    free(deg); free(pr);
-    return t1 - t0;
-}
-
 int oracle_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -49,7 +49,7 @@ def test_params_default_is_reference_default():
     assert p.sink_handling == capi.NKPR_SINK_NONE
     assert p.normalized == 0 and p.precision == capi.NKPR_FP64
     assert p.stop == capi.NKPR_STOP_STEP
-    assert capi.lib().nkpr_abi_version() == capi.ABI_VERSION == 2
+    assert capi.lib().nkpr_abi_version() == capi.ABI_VERSION == 3
 
 
 def test_graph_create_without_device_reports_nodevice():

@@ -51,8 +51,10 @@ def converged_start(g, y, sink_mode):
 def oracle_pair(g, sink_mode=0):
     """(reference-rule scores, iterations, converged scores) of the oracle."""
     step, it, _ = oracle.pagerank(g, sink_mode=sink_mode)
-    tight, _, _ = oracle.pagerank(g, tol=1e-12, norm=oracle.L1, max_iterations=20000,
-                                  sink_mode=sink_mode, start=converged_start(g, step, sink_mode))
+    print("oracle: reference rule %d sweeps" % it, flush=True)
+    tight, it2, _ = oracle.pagerank(g, tol=1e-12, norm=oracle.L1, max_iterations=20000,
+                                    sink_mode=sink_mode, start=converged_start(g, step, sink_mode))
+    print("oracle: converged run %d more sweeps" % it2, flush=True)
     return step, it, tight
 
 
@@ -165,6 +167,8 @@ def test_graph_batch_union_default_paths(capfd):
             assert st["l1_error_bound"] <= 1e-9
         worst = 0.0
         for q, og in enumerate(graphs):
+            if q % 10 == 0:
+                print("graph %d of %d against the oracle" % (q, G), flush=True)
             xq = x[offs[q]:offs[q + 1]]
             if precision == capi.NKPR_FP64:
                 ref, it, _ = oracle.pagerank(og)

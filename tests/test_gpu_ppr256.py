@@ -41,7 +41,9 @@ def make_case(kind):
         g = oracle.in_lists(z, u, v, w, True)
     seeds = [rng.choice(z, int(rng.integers(1, 1001)), replace=True) for _ in range(K)]
     step, tight = [], []
-    for s in seeds:
+    for j, s in enumerate(seeds):
+        if j % 64 == 0:
+            print("%s: oracle column %d of %d" % (kind, j, K), flush=True)
         t = teleport(z, s)
         r, it, _ = oracle.pagerank(g, teleport=t)
         step.append((r, it))

@@ -334,3 +334,84 @@ def test_katz_default_alpha_and_regular_graph():
     vv = np.concatenate([(np.arange(n) + 1) % n, (np.arange(n) + 2) % n])
     s, _ = oracle.katz(oracle.in_lists(n, uu, vv, None, False))
     assert np.allclose(s, 1 / np.sqrt(n), atol=1e-12)
+
+
+# ---- AlgebraicPageRank and the CSR product (AlgebraicPageRank.hpp:35-119,
+#      CSRGeneralMatrix.hpp:151-182, 563-578) ----
+
+def test_for_edges_order():
+    # directed: u ascending, each out-list in insertion order
+    eu, ev, ew = oracle.for_edges(4, [2, 0, 2, 1, 0], [1, 3, 0, 1, 2], [1., 2., 3., 4., 5.])
+    assert eu.tolist() == [0, 0, 1, 2, 2]
+    assert ev.tolist() == [3, 2, 1, 1, 0]
+    assert ew.tolist() == [2., 5., 4., 1., 3.]
+    # undirected: each edge once, at its endpoint u >= v; self-loop once
+    eu, ev, _ = oracle.for_edges(4, [0, 3, 2, 1], [2, 1, 2, 3], None, directed=False)
+    assert list(zip(eu.tolist(), ev.tolist())) == [(2, 0), (2, 2), (3, 1), (3, 1)]
+
+
+def test_csr_spmv_matches_scipy():
+    rng = np.random.default_rng(3)
+    rows, cols, nnz = 300, 200, 4000
+    r = np.sort(rng.integers(0, rows, nnz))
+    row_idx = np.zeros(rows + 1, np.uint64)
+    row_idx[1:] = np.cumsum(np.bincount(r, minlength=rows))
+    a = oracle.CSR(rows, cols, row_idx, rng.integers(0, cols, nnz), rng.normal(size=nnz))
+    x = rng.normal(size=cols)
+    # duplicates inside a row stay separate entries and still add up
+    np.testing.assert_allclose(oracle.csr_spmv(a, x), a.to_scipy() @ x, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_algebraic_matrix_is_d_At_over_outdeg(directed):
+    z, u, v, w = random_graph(40, 300, 8, directed=directed, weighted=True)
+    eu, ev, ew = oracle.for_edges(z, u, v, w, directed)
+    mat = oracle.algebraic_matrix(z, eu, ev, ew, directed, damp=0.85)
+    A = np.zeros((z, z))
+    for a, b, c in zip(u, v, w):
+        A[a, b] += c
+        if not directed and a != b:
+            A[b, a] += c
+    deg = A.sum(axis=1)
+    inv = np.where(deg > 0, 1.0 / np.where(deg > 0, deg, 1.0), 0.0)
+    M = 0.85 * (A * inv[:, None]).T
+    np.testing.assert_allclose(mat.to_scipy().toarray(), M, rtol=1e-14, atol=1e-15)
+    assert mat.nnz == (len(u) if directed else len(u) + int((u != v).sum()))
+
+
+def test_algebraic_pagerank_known_answers():
+    # AlgebraicPageRankGTest.cpp:55-94 (Matlab values, +-1e-4)
+    z, u, v, w = eight_node()
+    eu, ev, ew = oracle.for_edges(z, u, v, w, directed=False)
+    s, _ = oracle.algebraic_pagerank(oracle.algebraic_matrix(z, eu, ev, ew, False))
+    np.testing.assert_allclose(np.abs(s), EIGHT_PR, atol=1e-4)
+    # AlgebraicPageRankGTest.cpp:21-52: wiki-Vote as SNAPGraphReader reads it (undirected),
+    # top-1 node 699 with 0.00432 +- 1e-3
+    z, u, v = wiki_vote(False)
+    eu, ev, _ = oracle.for_edges(z, u, v, None, directed=False)
+    s, _ = oracle.algebraic_pagerank(oracle.algebraic_matrix(z, eu, ev, None, False))
+    top = int(np.lexsort((np.arange(z), -s))[0])
+    assert top == WIKI_TOP[False][0]
+    assert abs(s[top] - WIKI_TOP[False][1]) <= 1e-3
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_algebraic_pagerank_fixed_point_and_pagerank(directed):
+    """The fixed point is the solution of (I - M) x = (1-d)/n 1 normalised; with every slot
+    present it is also PageRank's result (the same loop, M folded), same iteration count."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    z, u, v, w = random_graph(500, 4000, 21, directed=directed, weighted=True)
+    eu, ev, ew = oracle.for_edges(z, u, v, w, directed)
+    mat = oracle.algebraic_matrix(z, eu, ev, ew, directed)
+    s, it = oracle.algebraic_pagerank(mat, tol=1e-13)
+    x = spl.spsolve(sp.identity(z, format="csc") - mat.to_scipy().tocsc(),
+                    np.full(z, 0.15 / z))
+    np.testing.assert_allclose(s, x / x.sum(), rtol=0, atol=1e-12)
+    s8, it8 = oracle.algebraic_pagerank(mat)
+    g = oracle.in_lists(z, u, v, w, directed)
+    ref, it_ref, _ = oracle.pagerank(g)
+    assert it8 == it_ref
+    assert np.abs(s8 - ref).sum() <= 1e-12
+    # the cap stops the loop
+    assert oracle.algebraic_pagerank(mat, max_iterations=3)[1] == 3
