@@ -798,9 +798,12 @@ def cpu_run_baseline(og, m, seconds, its_ref, what):
     est = setup + its_ref * per
     log("cpu baseline: run() with 1 sweep %.2f s, with %d sweeps %.2f s on %d threads; "
         "%.3f s per sweep, set-up %.2f s" % (t1, k, tk, threads, per, setup))
-    return {"value": m * k / tk, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+    return {"value": m * its_ref / est, "unit": "edge-updates/s", "cores": threads,
+            "kind": "port",
             "sample": "%s: the oracle's PageRank::run capped at %d iterations (%.2f s) and at 1 "
-                      "(%.2f s) on %d host threads" % (what, k, tk, t1, threads),
+                      "(%.2f s) on %d host threads; value = m x %d sweeps / (set-up + %d x the "
+                      "time per sweep), a whole run() to the reference rule" % (
+                          what, k, tk, t1, threads, its_ref, its_ref),
             "ms_per_sweep": per * 1e3, "setup_ms": setup * 1e3,
             "reference_rule_sweeps": int(its_ref),
             "run_to_reference_rule_s_extrapolated": est, "host": host_info()}

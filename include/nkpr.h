@@ -307,6 +307,44 @@ int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, dou
 int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                             double *scores, int32_t scores_on_device, uint64_t *iterations);
 
+/* ---- NetworKit binary graph files (nkbg002) ------------------------------------------------
+ * Host-side CSR lists of a graph in NetworKit's binary format (include/networkit/io/
+ * NetworkitBinaryGraph.hpp:20-141).  Arrays are host memory owned by the struct (released by
+ * nkpr_nkbg_free).  The pull lists are what PageRank reads: the in-lists of a directed graph,
+ * the adjacency of an undirected one (u's entries with v <= u, then those with v > u, as the
+ * reference's reader appends them); colptr/src/weight/exists plug into nkpr_csc as they are
+ * (with device_ptrs = 0, weights as weight64). */
+typedef struct {
+    uint64_t nodes;          /* slots, exists[] marks the nodes                              */
+    uint64_t edges;          /* Graph::numberOfEdges of the stored graph                      */
+    uint64_t self_loops;
+    int32_t directed;
+    int32_t weighted;
+    int32_t weight_format;   /* 0 none, 1 unsigned varint, 2 signed varint, 3 double, 4 float */
+    int32_t reserved;
+    uint64_t *colptr;        /* [nodes + 1] pull lists                                        */
+    int32_t *src;            /* [colptr[nodes]]                                               */
+    double *weight;          /* [colptr[nodes]], NULL when unweighted                         */
+    uint64_t *out_ptr;       /* out-lists of a directed graph when requested, else NULL       */
+    int32_t *out_dst;
+    double *out_weight;
+    uint8_t *exists;         /* [nodes]                                                       */
+} nkpr_nkbg;
+
+/* Replaces NetworkitBinaryReader::read (networkit/cpp/io/NetworkitBinaryReader.cpp:19-237)
+ * without NetworKit's per-node vectors: decodes the file on host threads straight into the
+ * lists above.  with_out_lists = 1 also fills out_ptr/out_dst/out_weight of a directed graph.
+ * A malformed file is NKPR_ERR_INVALID.  Node ids must fit int32. */
+int nkpr_nkbg_read(const char *path, int32_t with_out_lists, nkpr_nkbg *out);
+void nkpr_nkbg_free(nkpr_nkbg *g);
+/* Replaces NetworkitBinaryWriter::write (networkit/cpp/io/NetworkitBinaryWriter.cpp:19-350):
+ * g's out-lists (out_ptr/out_dst/out_weight; for an undirected graph the adjacency, an edge at
+ * both endpoints and a self-loop once) and, when directed, its in-lists (colptr/src/weight)
+ * in storage order.  chunks as the reference's writer (default 32, at most nodes);
+ * weight_format -1 detects it from the weights as the reference's autoDetect does.  Unlike
+ * the reference's writer, deleted slots keep their ids (flag bit 0 clear). */
+int nkpr_nkbg_write(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32_t weight_format);
+
 /* Centrality::ranking: ids of the existing nodes ordered by score descending, ties by id
  * ascending; ids_out and scores_out (nullable) receive *count = number of existing nodes
  * entries.  All arrays are host pointers. */

@@ -53,6 +53,7 @@ cdef extern from "nkpr.h":
 
 cdef extern from "Graph.hpp" namespace "NetworKit":
     cdef cppclass _Graph "NetworKit::Graph":
+        _Graph() except +
         _Graph(count, bool_t, bool_t) except +
         node addNode() except +
         node addNodes(count) except +
@@ -121,6 +122,16 @@ cdef extern from "AlgebraicPageRank.hpp" namespace "NetworKit":
     cdef cppclass _AlgebraicPageRank "NetworKit::AlgebraicPageRank<NetworKit::CSRMatrix>"(_Centrality):
         _AlgebraicPageRank(_Graph, double, double) except +
         count numberOfIterations() except +
+
+cdef extern from "NetworkitBinaryIO.hpp" namespace "NetworKit":
+    cdef cppclass _NetworkitBinaryReader "NetworKit::NetworkitBinaryReader":
+        _NetworkitBinaryReader()
+        _Graph read(string) except + nogil
+    cdef enum _NetworkitBinaryWeights "NetworKit::NetworkitBinaryWeights":
+        pass
+    cdef cppclass _NetworkitBinaryWriter "NetworKit::NetworkitBinaryWriter":
+        _NetworkitBinaryWriter(uint64_t, _NetworkitBinaryWeights)
+        void write(_Graph, string) except + nogil
 
 
 cdef class Graph:
@@ -439,3 +450,24 @@ cdef class AlgebraicPageRank(Centrality):
 
     def numberOfIterations(self):
         return (<_AlgebraicPageRank*>(self._this)).numberOfIterations()
+
+
+def _read_nkbg(path):
+    """NetworkitBinaryReader::read through the C++ drop-in: a Graph from an nkbg002 file."""
+    cdef _NetworkitBinaryReader reader
+    cdef string p = path.encode()
+    cdef Graph G = Graph(0)
+    with nogil:
+        G._this[0] = reader.read(p)
+    return G
+
+
+def _write_nkbg(Graph G, path, uint64_t chunks=32, int weightsType=5):
+    """NetworkitBinaryWriter(chunks, weightsType).write(G, path) through the C++ drop-in."""
+    cdef _NetworkitBinaryWriter* w = new _NetworkitBinaryWriter(chunks, <_NetworkitBinaryWeights>weightsType)
+    cdef string p = path.encode()
+    try:
+        with nogil:
+            w.write(G._this[0], p)
+    finally:
+        del w

@@ -30,7 +30,8 @@ EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch
             "nkpr_graph_set_weights", "nkpr_run", "nkpr_run_batch", "nkpr_adjoint",
             "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version",
             "nkpr_power_params_default", "nkpr_power_run", "nkpr_csr_spmv",
-            "nkpr_csr_pagerank_step", "nkpr_algebraic_pagerank"]
+            "nkpr_csr_pagerank_step", "nkpr_algebraic_pagerank", "nkpr_nkbg_read",
+            "nkpr_nkbg_free", "nkpr_nkbg_write"]
 ABI_VERSION = 3
 
 
@@ -86,6 +87,16 @@ class CsrMatrix(ctypes.Structure):
                 ("values", ctypes.c_void_p)]
 
 
+class Nkbg(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_uint64), ("edges", ctypes.c_uint64),
+                ("self_loops", ctypes.c_uint64), ("directed", ctypes.c_int32),
+                ("weighted", ctypes.c_int32), ("weight_format", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("colptr", ctypes.c_void_p),
+                ("src", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("out_ptr", ctypes.c_void_p), ("out_dst", ctypes.c_void_p),
+                ("out_weight", ctypes.c_void_p), ("exists", ctypes.c_void_p)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint64), ("sweeps_fp32", ctypes.c_uint64),
                 ("sweeps_fp64", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
@@ -136,10 +147,15 @@ def lib():
     L.nkpr_algebraic_pagerank.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.c_double,
                                           ctypes.c_double, ctypes.c_uint64, vp, ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_uint64)]
+    L.nkpr_nkbg_read.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(Nkbg)]
+    L.nkpr_nkbg_free.argtypes = [ctypes.POINTER(Nkbg)]
+    L.nkpr_nkbg_free.restype = None
+    L.nkpr_nkbg_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(Nkbg), ctypes.c_uint64,
+                                  ctypes.c_int32]
     L.nkpr_last_error.restype = ctypes.c_char_p
     L.nkpr_abi_version.restype = ctypes.c_int
     for name in EXPORTED:
-        if name not in ("nkpr_graph_stream", "nkpr_last_error"):
+        if name not in ("nkpr_graph_stream", "nkpr_last_error", "nkpr_nkbg_free"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -428,3 +444,60 @@ class DeviceCSR:
         check(lib().nkpr_algebraic_pagerank(ctypes.byref(m), damp, tol, int(max_iterations),
                                             scores.data_ptr(), 1, ctypes.byref(it)))
         return scores, int(it.value)
+
+
+def _copy(ptr, dtype, count):
+    if not ptr or count == 0:
+        return np.zeros(count, dtype=dtype)
+    ct = {np.uint64: ctypes.c_uint64, np.int32: ctypes.c_int32, np.float64: ctypes.c_double,
+          np.uint8: ctypes.c_uint8}[dtype]
+    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), shape=(count,)).copy()
+
+
+def read_nkbg(path, with_out_lists=False):
+    """nkpr_nkbg_read: a NetworKit binary graph file as flat host lists (numpy copies).
+    Returns a dict: nodes, edges, self_loops, directed, weighted, weight_format, colptr, src,
+    weight (None when unweighted), exists, and out_ptr / out_dst / out_weight for a directed
+    graph when with_out_lists."""
+    g = Nkbg()
+    check(lib().nkpr_nkbg_read(os.fsencode(path), 1 if with_out_lists else 0, ctypes.byref(g)))
+    try:
+        n = g.nodes
+        m = _copy(g.colptr, np.uint64, n + 1)[-1] if n else 0
+        out = {"nodes": n, "edges": g.edges, "self_loops": g.self_loops,
+               "directed": bool(g.directed), "weighted": bool(g.weighted),
+               "weight_format": g.weight_format,
+               "colptr": _copy(g.colptr, np.uint64, n + 1),
+               "src": _copy(g.src, np.int32, int(m)),
+               "weight": _copy(g.weight, np.float64, int(m)) if g.weighted else None,
+               "exists": _copy(g.exists, np.uint8, n)}
+        if g.out_ptr:
+            op = _copy(g.out_ptr, np.uint64, n + 1)
+            out["out_ptr"] = op
+            out["out_dst"] = _copy(g.out_dst, np.int32, int(op[-1]))
+            out["out_weight"] = _copy(g.out_weight, np.float64, int(op[-1])) if g.weighted else None
+        return out
+    finally:
+        lib().nkpr_nkbg_free(ctypes.byref(g))
+
+
+def write_nkbg(path, nodes, directed, out_ptr, out_dst, out_weight=None, in_ptr=None,
+               in_src=None, in_weight=None, exists=None, chunks=32, weight_format=-1):
+    """nkpr_nkbg_write from host lists in storage order (see nkpr.h)."""
+    keep = []
+
+    def arr(a, dt):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    g = Nkbg(nodes=int(nodes), directed=1 if directed else 0,
+             weighted=0 if out_weight is None else 1,
+             out_ptr=arr(out_ptr, np.uint64), out_dst=arr(out_dst, np.int32),
+             out_weight=arr(out_weight, np.float64), colptr=arr(in_ptr, np.uint64),
+             src=arr(in_src, np.int32), weight=arr(in_weight, np.float64),
+             exists=arr(exists, np.uint8))
+    check(lib().nkpr_nkbg_write(os.fsencode(path), ctypes.byref(g), int(chunks),
+                                int(weight_format)))

@@ -130,6 +130,41 @@ edgeweight Graph::weightedDegree(node u, bool countSelfLoopsTwice) const {
     return static_cast<edgeweight>(sum);
 }
 
+Graph Graph::fromLists(count z_, bool weighted_, bool directed_, const uint8_t *exists_,
+                       const uint64_t *outPtr, const int32_t *outIds, const double *outW,
+                       const uint64_t *pullPtr, const int32_t *pullIds, const double *pullW,
+                       count edges, count loops) {
+    Graph G(z_, weighted_, directed_);
+    auto fill = [&](std::vector<std::vector<node>> &lists,
+                    std::vector<std::vector<edgeweight>> &wl, const uint64_t *ptr,
+                    const int32_t *ids, const double *w) {
+        for (node u = 0; u < z_; ++u) {
+            const uint64_t b = ptr[u], e = ptr[u + 1];
+            lists[u].assign(ids + b, ids + e);
+            if (weighted_) {
+                if (w) wl[u].assign(w + b, w + e);
+                else wl[u].assign(e - b, defaultEdgeWeight);
+            }
+        }
+    };
+    if (directed_) {
+        fill(G.outEdges, G.outEdgeWeights, outPtr, outIds, outW);
+        fill(G.inEdges, G.inEdgeWeights, pullPtr, pullIds, pullW);
+    } else {
+        fill(G.outEdges, G.outEdgeWeights, pullPtr, pullIds, pullW);
+    }
+    if (exists_)
+        for (node u = 0; u < z_; ++u)
+            if (!exists_[u]) {
+                G.exists[u] = false;
+                --G.n;
+            }
+    G.m = edges;
+    G.selfLoops = loops;
+    G.ver = 1;
+    return G;
+}
+
 void Graph::exportLists(bool outgoing, std::vector<uint64_t> &colptr, std::vector<int32_t> &src,
                         std::vector<float> *w32, std::vector<double> *w64) const {
     if (z >= 0x7FFFFFFFull) throw std::runtime_error("graph too large for int32 node ids");

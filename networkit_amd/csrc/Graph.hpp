@@ -85,6 +85,20 @@ public:
                      std::vector<float> *w32, std::vector<double> *w64) const;
     const std::vector<bool> &existsFlags() const noexcept { return exists; }
 
+    // A graph built from flat lists in storage order (what a file reader decodes): for an
+    // undirected graph `pull` holds each node's adjacency; for a directed one `out` holds the
+    // out-lists and `pull` the in-lists.  Weights may be NULL (unweighted); exists may be NULL
+    // (every slot is a node).  edges and selfLoops are the counts the lists describe.
+    static Graph fromLists(count z, bool weighted, bool directed, const uint8_t *exists,
+                           const uint64_t *outPtr, const int32_t *outIds, const double *outW,
+                           const uint64_t *pullPtr, const int32_t *pullIds, const double *pullW,
+                           count edges, count selfLoops);
+    // The out-lists in storage order (for an undirected graph, the adjacency).
+    void exportOutLists(std::vector<uint64_t> &ptr, std::vector<int32_t> &ids,
+                        std::vector<double> *w) const {
+        exportLists(true, ptr, ids, nullptr, w);
+    }
+
     // Increments on every structural or weight change; consumers cache device copies by it.
     uint64_t version() const noexcept { return ver; }
 

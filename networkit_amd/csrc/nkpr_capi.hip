@@ -660,6 +660,23 @@ int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t
     });
 }
 
+int nkpr_nkbg_read(const char *path, int32_t with_out_lists, nkpr_nkbg *out) {
+    return guarded([&] {
+        try {
+            nkbg_read_impl(path, with_out_lists, out);
+        } catch (...) {
+            nkbg_free_impl(out);
+            throw;
+        }
+    });
+}
+
+void nkpr_nkbg_free(nkpr_nkbg *g) { nkbg_free_impl(g); }
+
+int nkpr_nkbg_write(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32_t weight_format) {
+    return guarded([&] { nkbg_write_impl(path, g, chunks, weight_format); });
+}
+
 int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_t *ids_out,
                  double *scores_out, uint64_t *count) {
     return guarded([&] {

@@ -340,6 +340,11 @@ void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, 
                    double *scratch, uint64_t scratch_count, hipStream_t s);
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                              double *scores, int32_t scores_on_device, uint64_t *iterations);
+
+// NetworKit binary graph files (nkpr_nkbg.hip)
+void nkbg_read_impl(const char *path, int32_t with_out_lists, nkpr_nkbg *out);
+void nkbg_free_impl(nkpr_nkbg *g);
+void nkbg_write_impl(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32_t weight_format);
 void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint64_t *seed_off,
                     const int32_t *seeds, const double *seed_w, double *scores,
                     nkpr_batch_stats *st, uint64_t *col_iterations, double *col_bound);

@@ -1,7 +1,11 @@
-"""Edge-list readers that produce the node numbering NetworKit's readers produce.
+"""Graph readers that produce the node numbering NetworKit's readers produce.
 
-Only the SNAP reader is provided: the reference's PageRank known answers are stated in the
-node ids of ``SNAPGraphReader`` (networkit/cpp/io/SNAPGraphReader.cpp:18-111).
+- ``SNAPGraphReader``: the reference's PageRank known answers are stated in the node ids of
+  NetworKit's SNAP reader (networkit/cpp/io/SNAPGraphReader.cpp:18-111).
+- ``NetworkitBinaryReader`` / ``NetworkitBinaryWriter``: NetworKit's binary format (nkbg002,
+  networkit/cpp/io/NetworkitBinaryReader.cpp, NetworkitBinaryWriter.cpp) through the C++
+  drop-ins; ``readNetworkitBinaryLists`` decodes a file straight into the flat pull lists the
+  device library takes (no Graph in between).
 """
 import gzip
 
@@ -68,3 +72,36 @@ class SNAPGraphReader:
         G = Graph(z, False, self.directed)
         G.addEdges(u, v)
         return G
+
+
+class NetworkitBinaryWeights:
+    """NetworkitBinaryWriter.hpp: the weight format of a written file."""
+    none, unsignedFormat, signedFormat, doubleFormat, floatFormat, autoDetect = range(6)
+
+
+class NetworkitBinaryReader:
+    """``NetworkitBinaryReader().read(path)`` -> Graph (NetworkitBinaryReader.hpp:21-35)."""
+
+    def read(self, path):
+        from ._core import _read_nkbg
+        return _read_nkbg(str(path))
+
+
+class NetworkitBinaryWriter:
+    """``NetworkitBinaryWriter(chunks=32, weightsType=autoDetect).write(G, path)``
+    (NetworkitBinaryWriter.hpp:20-45)."""
+
+    def __init__(self, chunks=32, weightsType=NetworkitBinaryWeights.autoDetect):
+        self.chunks = chunks
+        self.weightsType = weightsType
+
+    def write(self, G, path):
+        from ._core import _write_nkbg
+        _write_nkbg(G, str(path), self.chunks, self.weightsType)
+
+
+def readNetworkitBinaryLists(path, with_out_lists=False):
+    """The pull lists of an nkbg002 file as numpy arrays (colptr, src, weight, exists, ...),
+    ready for ``capi.DeviceGraph`` (see ``capi.read_nkbg``)."""
+    from .capi import read_nkbg
+    return read_nkbg(path, with_out_lists)

@@ -57,17 +57,55 @@
  * Graph.cpp:427-455.  The unweighted out-count (Graph::degree, Graph.hpp:1207) goes to cnt. */
 void oracle_out_degrees(uint64_t z, const uint64_t *in_ptr, const uint64_t *in_src,
                         const double *in_w, double *deg, uint64_t *cnt) {
+    int threads = 1;
+#ifdef _OPENMP
+    threads = omp_get_max_threads();
+#endif
+    /* The reference sums each node's out-list on its own thread.  The oracle holds in-lists,
+     * so in parallel each thread adds the entries of a fixed range of rows (static schedule)
+     * into a private array, and the private arrays are added in thread order: the result
+     * depends on the thread count only.  Counts are exact either way.  Large graphs whose
+     * private arrays would not fit a 16 GB budget are summed on one thread. */
+    double *part = NULL;
+    if (threads > 1 && (double)z * (double)threads * 8.0 <= 16e9)
+        part = (double *)calloc((size_t)z * (size_t)threads, sizeof(double));
     for (uint64_t v = 0; v < z; ++v) {
         deg[v] = 0.0;
         if (cnt) cnt[v] = 0;
     }
-    for (uint64_t u = 0; u < z; ++u) {
-        for (uint64_t e = in_ptr[u]; e < in_ptr[u + 1]; ++e) {
-            uint64_t v = in_src[e];
-            deg[v] += in_w ? in_w[e] : 1.0;
-            if (cnt) cnt[v] += 1;
+    if (!part) {
+        for (uint64_t u = 0; u < z; ++u) {
+            for (uint64_t e = in_ptr[u]; e < in_ptr[u + 1]; ++e) {
+                uint64_t v = in_src[e];
+                deg[v] += in_w ? in_w[e] : 1.0;
+                if (cnt) cnt[v] += 1;
+            }
+        }
+        return;
+    }
+#pragma omp parallel num_threads(threads)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double *mine = part + (size_t)t * z;
+#pragma omp for schedule(static)
+        for (int64_t u = 0; u < (int64_t)z; ++u) {
+            for (uint64_t e = in_ptr[u]; e < in_ptr[u + 1]; ++e) {
+                const uint64_t v = in_src[e];
+                mine[v] += in_w ? in_w[e] : 1.0;
+                if (cnt) __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
+            }
+        }
+#pragma omp for schedule(static)
+        for (int64_t v = 0; v < (int64_t)z; ++v) {
+            double s = 0.0;
+            for (int k = 0; k < threads; ++k) s += part[(size_t)k * z + v];
+            deg[v] = s;
         }
     }
+    free(part);
 }
 
 /* PageRank::run.  scores[z] holds the start vector on entry (the caller applies the
