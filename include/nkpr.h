@@ -297,6 +297,17 @@ int nkpr_csr_spmv(const nkpr_csr *a, const double *x, double *y, void *stream);
 int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, double *y,
                            double *step_sq, void *stream);
 
+/* No counterpart in the reference: nkpr_csr_pagerank_step for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; its partial sums live in a 32 KiB device array of
+ * the library (4096 doubles, held by every GPU the library runs on until the process ends)
+ * instead of the stream's pool, and it calls no allocating routine.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other threads) race and
+ * must be ordered by the caller.  A matrix whose step would need more than 4096 partials is
+ * rejected (NKPR_ERR_INVALID; the step's grids are capped so that none does). */
+int nkpr_csr_pagerank_step_noalloc(const nkpr_csr *m, double shift, const double *x, double *y,
+                                   double *step_sq, void *stream);
+
 /* Replaces AlgebraicPageRank<CSRMatrix>::run (AlgebraicPageRank.hpp:94-119) on the matrix M
  * its constructor builds (:35-54, M(j, i) = damp * A(i, j) / rowsum_i(A)): from 1/n in every
  * row, y = M x + (1 - damp)/n until ||y - x||_2 <= tol (at least one iteration), then

@@ -31,7 +31,7 @@ EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch
             "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version",
             "nkpr_power_params_default", "nkpr_power_run", "nkpr_csr_spmv",
             "nkpr_csr_pagerank_step", "nkpr_algebraic_pagerank", "nkpr_nkbg_read",
-            "nkpr_nkbg_free", "nkpr_nkbg_write"]
+            "nkpr_nkbg_free", "nkpr_nkbg_write", "nkpr_csr_pagerank_step_noalloc"]
 ABI_VERSION = 3
 
 
@@ -144,6 +144,8 @@ def lib():
     L.nkpr_csr_spmv.argtypes = [ctypes.POINTER(CsrMatrix), vp, vp, vp]
     L.nkpr_csr_pagerank_step.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.c_double, vp, vp, vp,
                                          vp]
+    L.nkpr_csr_pagerank_step_noalloc.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.c_double, vp,
+                                                 vp, vp, vp]
     L.nkpr_algebraic_pagerank.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.c_double,
                                           ctypes.c_double, ctypes.c_uint64, vp, ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_uint64)]
@@ -429,6 +431,14 @@ class DeviceCSR:
         m = self.struct()
         check(lib().nkpr_csr_pagerank_step(ctypes.byref(m), float(shift), x.data_ptr(),
                                            y.data_ptr(), step_sq.data_ptr(), stream))
+        return y, step_sq
+
+    def pagerank_step_noalloc(self, x, shift, y, step_sq, stream=None):
+        """nkpr_csr_pagerank_step_noalloc: the step into caller-provided y and step_sq (no
+        allocation by the library)."""
+        m = self.struct()
+        check(lib().nkpr_csr_pagerank_step_noalloc(ctypes.byref(m), float(shift), x.data_ptr(),
+                                                   y.data_ptr(), step_sq.data_ptr(), stream))
         return y, step_sq
 
     def algebraic_pagerank(self, damp=0.85, tol=1e-8, max_iterations=2**64 - 1, scores=None):

@@ -652,6 +652,11 @@ int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, dou
     });
 }
 
+int nkpr_csr_pagerank_step_noalloc(const nkpr_csr *m, double shift, const double *x, double *y,
+                                   double *step_sq, void *stream) {
+    return guarded([&] { csr_step_noalloc_impl(m, shift, x, y, step_sq, (hipStream_t)stream); });
+}
+
 int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                             double *scores, int32_t scores_on_device, uint64_t *iterations) {
     return guarded([&] {

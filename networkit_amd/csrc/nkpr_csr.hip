@@ -34,6 +34,14 @@
 
 #include "nkpr_internal.hpp"
 
+#include <mutex>
+
+// Scratch of nkpr_csr_pagerank_step_noalloc: the step's per-block partials (at most
+// kShortGrid + kLongGrid = 4096 doubles, 32 KiB) in a device array that every GPU the library
+// runs on holds until the process ends.  Host code never uses this name as an address: it
+// takes the array's address per device with hipGetSymbolAddress (csr_noalloc_scratch).
+static __device__ __align__(256) double scratch[4096];
+
 namespace nkpr {
 namespace {
 
@@ -337,6 +345,35 @@ void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, 
         PoolBuf buf(8 * (size_t)std::max<uint32_t>(need, 1), s);
         launch_product<true>(A, m->nnz, x, shift, y, static_cast<double *>(buf.p), step_sq, s);
     }
+}
+
+// The address of `scratch` on the current device, taken once per device (each GPU holds its
+// own copy of the array at its own address).
+double *csr_noalloc_scratch() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    int dev = -1;
+    NKPR_HIP(hipGetDevice(&dev));
+    NKPR_CHECK(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        NKPR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NKPR_CHECK(p != nullptr, "hipGetSymbolAddress returned NULL");
+        addr[dev] = static_cast<double *>(p);
+    }
+    return addr[dev];
+}
+
+void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, double *y,
+                           double *step_sq, hipStream_t s) {
+    constexpr uint64_t kScratch = sizeof(scratch) / sizeof(double);
+    static_assert(kScratch >= kShortGrid + kLongGrid, "the step's grids fit the scratch");
+    NKPR_CHECK(m != nullptr, "matrix is NULL");
+    NKPR_CHECK(csr_step_partials(m->rows, m->nnz) <= kScratch,
+               "the step needs more partials than the 4096 the scratch holds");
+    csr_step_impl(m, shift, x, y, step_sq, csr_noalloc_scratch(), kScratch, s);
 }
 
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,

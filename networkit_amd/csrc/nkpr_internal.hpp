@@ -338,6 +338,8 @@ uint32_t csr_step_partials(uint64_t rows, uint64_t nnz);
 void csr_spmv_impl(const nkpr_csr *a, const double *x, double *y, hipStream_t s);
 void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, double *step_sq,
                    double *scratch, uint64_t scratch_count, hipStream_t s);
+void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, double *y,
+                           double *step_sq, hipStream_t s);
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                              double *scores, int32_t scores_on_device, uint64_t *iterations);
 
