@@ -75,6 +75,20 @@ constexpr uint64_t kMaxPadded = (1ull << 32) - (1ull << 16);
 #define NKPR_P2_QUADS 2
 #endif
 constexpr int kP1Quads = NKPR_P1_QUADS, kP2Quads = NKPR_P2_QUADS;
+// Hot sources of the FP32 operator: the first `hot` relabelled ids (the sources with the most
+// out-edges), at most kHot.  Their edges skip pass 1 and the contribution array: pass 2 keeps
+// their values in an LDS table next to the accumulators and gathers from it, so such an edge
+// moves 8 bytes (packed source and destination offsets, coefficient) instead of 16.  A hot
+// entry packs (source << kBinShift) | destination offset; padding points at the zero entry
+// tab[kHot].  With the table (64 KB) beside the accumulators (64 KB) one pass-2 block fits a
+// CU, so that kernel keeps more quads in flight per lane.
+constexpr int kHotShift = 14;
+constexpr uint32_t kHot = 1u << kHotShift;
+static_assert(kHotShift + 1 + kBinShift <= 32, "hot entries pack into 32 bits");
+#ifndef NKPR_P2H_QUADS
+#define NKPR_P2H_QUADS 4
+#endif
+constexpr int kP2HotQuads = NKPR_P2H_QUADS;
 
 // Control words of one application: the largest |x| of the sources (the bits of a
 // non-negative double order like unsigned integers) and the queue counters of the passes.
@@ -135,12 +149,14 @@ __global__ void bin_row_kernel(const uint32_t *rowptr, uint32_t n, uint32_t *row
 // key = ((chunk * nbins + bin) << ss) | source offset: inside a segment the edges are sorted
 // by source, which spreads the destinations of a wave's atomics over the accumulator
 // (chunks of 2^ss sources, bins of kBin destinations)
+// Hot edges (source below `hot`) go to the segments of a chunk past the last one, one per bin.
 __global__ void bin_key_kernel(const int32_t *src, const uint32_t *row, uint64_t m, uint32_t nbins,
-                               int ss, uint64_t *key, uint32_t *val) {
+                               int ss, uint32_t nchunks, uint32_t hot, uint64_t *key, uint32_t *val) {
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t v = (uint32_t)src[e];
-        const uint64_t seg = (uint64_t)(v >> ss) * nbins + (row[e] >> kBinShift);
+        const uint32_t chunk = v < hot ? nchunks : (v >> ss);
+        const uint64_t seg = (uint64_t)chunk * nbins + (row[e] >> kBinShift);
         key[e] = (seg << ss) | (v & ((1u << ss) - 1));
         val[e] = (uint32_t)e;
     }
@@ -164,12 +180,55 @@ __global__ void bin_seg_kernel(const uint64_t *key, uint64_t m, uint64_t nseg, i
 // order[p] = edge at chunk-major padded position p: the sorted edge j of segment s goes to
 // cseg[s] + (j - rseg[s]) (rseg: unpadded segment starts, cseg: padded ones)
 __global__ void bin_place_kernel(const uint64_t *key, const uint32_t *ids, uint64_t m, int ss,
-                                 const uint32_t *rseg, const uint32_t *cseg, uint32_t *order) {
+                                 uint64_t nseg, const uint32_t *rseg, const uint32_t *cseg,
+                                 uint32_t *order) {
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
          j += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t sidx = key[j] >> ss;
+        if (sidx >= nseg) continue;  // a hot edge
         order[cseg[sidx] + ((uint32_t)j - rseg[sidx])] = ids[j];
     }
+}
+
+// horder[p] = edge at padded hot position p: the sorted hot edge j of bin b (segment nseg + b)
+// goes to hseg[b] + (j - rseg[nseg + b])
+__global__ void bin_hot_place_kernel(const uint64_t *key, const uint32_t *ids, uint64_t j0,
+                                     uint64_t m, int ss, uint64_t nseg, const uint32_t *rseg,
+                                     const uint32_t *hseg, uint32_t *horder) {
+    for (uint64_t j = j0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t sidx = key[j] >> ss;
+        horder[hseg[sidx - nseg] + ((uint32_t)j - rseg[sidx])] = ids[j];
+    }
+}
+
+// Hot entries: packed (source << kBinShift) | destination offset, and the coefficient; a
+// padding slot points at the zero table entry kHot with coefficient 0.
+__global__ void bin_hot_emit_kernel(const uint32_t *horder, uint64_t mh, const int32_t *src,
+                                    const uint32_t *row, const float *coef, uint32_t *hidx,
+                                    float *hcoef) {
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < mh;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = horder[p];
+        if (e == kPadEdge) {
+            hidx[p] = kHot << kBinShift;
+            if (hcoef) hcoef[p] = 0.f;
+            continue;
+        }
+        hidx[p] = ((uint32_t)src[e] << kBinShift) | (row[e] & (kBin - 1));
+        if (hcoef) hcoef[p] = coef[e];
+    }
+}
+
+// count of the edges whose source is below `bound` (one atomic per block)
+__global__ void bin_count_below_kernel(const int32_t *src, uint64_t m, uint32_t bound,
+                                       unsigned long long *count) {
+    unsigned long long c = 0;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (uint64_t)gridDim.x * blockDim.x)
+        c += (uint32_t)src[e] < bound;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
 
 // pos[k] = bin-major position of chunk-major quad k; one wave per segment s, whose padded
@@ -443,40 +502,111 @@ __global__ __launch_bounds__(kPassThreads) void bin_scatter_kernel(
 
 // Adds the quads f0, f0 + 64, f0 + 128, ... below `end` (quads from the start of bval) into
 // acc, kP2Quads per step with the next step's loads issued before this step's adds.
+template <int kQ>
 __device__ __forceinline__ void stream_quads(unsigned long long *acc, const uint16_t *bdst,
                                              const float *bval, float scale, uint32_t f0,
                                              uint32_t end) {
     if (f0 >= end) return;
     const Quad none = {{0u, 0u}, {0.f, 0.f, 0.f, 0.f}};
-    Quad cur[kP2Quads];
-    bool has[kP2Quads];
+    Quad cur[kQ];
+    bool has[kQ];
 #pragma unroll
-    for (int k = 0; k < kP2Quads; ++k) {
+    for (int k = 0; k < kQ; ++k) {
         const uint32_t f = f0 + 64u * k;
         has[k] = f < end;
         cur[k] = has[k] ? load_quad(bdst, bval, f * kQuad) : none;
     }
-    for (uint32_t f = f0 + 64u * kP2Quads; f < end; f += 64u * kP2Quads) {
-        Quad nxt[kP2Quads];
-        bool hn[kP2Quads];
+    for (uint32_t f = f0 + 64u * kQ; f < end; f += 64u * kQ) {
+        Quad nxt[kQ];
+        bool hn[kQ];
 #pragma unroll
-        for (int k = 0; k < kP2Quads; ++k) {
+        for (int k = 0; k < kQ; ++k) {
             const uint32_t fk = f + 64u * k;
             hn[k] = fk < end;
             nxt[k] = hn[k] ? load_quad(bdst, bval, fk * kQuad) : none;
         }
 #pragma unroll
-        for (int k = 0; k < kP2Quads; ++k)
+        for (int k = 0; k < kQ; ++k)
             if (has[k]) add_quad(acc, cur[k], scale);
 #pragma unroll
-        for (int k = 0; k < kP2Quads; ++k) {
+        for (int k = 0; k < kQ; ++k) {
             cur[k] = nxt[k];
             has[k] = hn[k];
         }
     }
 #pragma unroll
-    for (int k = 0; k < kP2Quads; ++k)
+    for (int k = 0; k < kQ; ++k)
         if (has[k]) add_quad(acc, cur[k], scale);
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// four hot entries of pass 2: packed source and destination offsets, coefficients
+struct HotQuad {
+    u32x4 id;
+    f32x4 c;
+};
+
+__device__ __forceinline__ HotQuad load_hot(const uint32_t *hidx, const float *hcoef, uint32_t e) {
+    HotQuad o;
+    o.id = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(hidx + e));
+    o.c = hcoef ? __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(hcoef + e))
+                : f32x4{1.f, 1.f, 1.f, 1.f};
+    return o;
+}
+
+// The contribution of a hot entry is computed as pass 1 computes it (table value times the
+// coefficient in FP32), so the sums are bit-identical to those of the two-pass route.
+__device__ __forceinline__ void add_hot(unsigned long long *acc, const float *tab, uint32_t id,
+                                        float c, float scale) {
+    fx_add(acc, id & (kBin - 1), tab[id >> kBinShift] * c, scale);
+}
+
+// Adds the hot quads f0, f0 + 64, ... below `end`, kQ per step, loads one step ahead.
+template <int kQ>
+__device__ __forceinline__ void stream_hot(unsigned long long *acc, const float *tab,
+                                           const uint32_t *hidx, const float *hcoef, float scale,
+                                           uint32_t f0, uint32_t end) {
+    if (f0 >= end) return;
+    HotQuad cur[kQ];
+    bool has[kQ];
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+        const uint32_t f = f0 + 64u * k;
+        has[k] = f < end;
+        if (has[k]) cur[k] = load_hot(hidx, hcoef, f * kQuad);
+    }
+    for (uint32_t f = f0 + 64u * kQ; f < end; f += 64u * kQ) {
+        HotQuad nxt[kQ];
+        bool hn[kQ];
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) {
+            const uint32_t fk = f + 64u * k;
+            hn[k] = fk < end;
+            if (hn[k]) nxt[k] = load_hot(hidx, hcoef, fk * kQuad);
+        }
+#pragma unroll
+        for (int k = 0; k < kQ; ++k)
+            if (has[k]) {
+                add_hot(acc, tab, cur[k].id.x, cur[k].c.x, scale);
+                add_hot(acc, tab, cur[k].id.y, cur[k].c.y, scale);
+                add_hot(acc, tab, cur[k].id.z, cur[k].c.z, scale);
+                add_hot(acc, tab, cur[k].id.w, cur[k].c.w, scale);
+            }
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) {
+            cur[k] = nxt[k];
+            has[k] = hn[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kQ; ++k)
+        if (has[k]) {
+            add_hot(acc, tab, cur[k].id.x, cur[k].c.x, scale);
+            add_hot(acc, tab, cur[k].id.y, cur[k].c.y, scale);
+            add_hot(acc, tab, cur[k].id.z, cur[k].c.z, scale);
+            add_hot(acc, tab, cur[k].id.w, cur[k].c.w, scale);
+        }
 }
 
 // Pass 2: item = {bin, first edge, end edge, flags | slot << 2} (bin-major, quad-aligned).  Each
@@ -484,11 +614,17 @@ __device__ __forceinline__ void stream_quads(unsigned long long *acc, const uint
 // one per lane) into the bin's LDS accumulator.  A bin held by one item is written out as
 // w = v - (d y + st) (minus) or w = B v = d y + st at once; a split bin's items write their
 // accumulators to slots of their own.
+// HOT: hitems[it] = {first, end} hot entry of the item (quad-aligned); an item with hot entries
+// first stages the table tab[i] = v[i] (* inv[i]) of the `hot` hot sources (0 beyond), the
+// value pass 1 would have staged, and adds them after its contributions.
+template <bool HOT>
 __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
     const uint4 *items, uint32_t nitems, const uint16_t *bdst, const float *bval, long long *part,
     uint32_t n, BinCtl *ctl, double insum, const float *v, bool minus, float d,
-    const double *sink_sum, float *w) {
+    const double *sink_sum, float *w, const uint2 *hitems, const uint32_t *hidx,
+    const float *hcoef, const float *inv, uint32_t hot) {
     __shared__ unsigned long long acc[kBin];
+    __shared__ __align__(16) float htab[HOT ? kHot + 1 : 1];
     __shared__ uint32_t slot;
     const int ex = fixed_exponent(ctl, insum);
     const float scale = ldexpf(1.f, ex);
@@ -498,12 +634,27 @@ __global__ __launch_bounds__(kPassThreads) void bin_gather_kernel(
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t it = next_item(&ctl->q2, &slot); it < nitems; it = next_item(&ctl->q2, &slot)) {
         const uint4 item = items[it];
+        const uint2 hi = HOT ? hitems[it] : uint2{0u, 0u};
         for (uint32_t i = threadIdx.x; i < kBin; i += kPassThreads) acc[i] = 0ull;
+        if (HOT && hi.y > hi.x) {
+            const uint32_t lim = min(hot, n);
+            for (uint32_t i = threadIdx.x; i <= kHot; i += kPassThreads) {
+                float t = 0.f;
+                if (i < lim) t = inv ? v[i] * inv[i] : v[i];
+                htab[i] = t;
+            }
+        }
         __syncthreads();
         const uint32_t q0 = item.y / kQuad, total = item.z / kQuad - q0;
         const uint32_t share = ((total + kWaves - 1) / kWaves + 63) & ~63u;
         const uint32_t wb = min(total, wave * share), we = min(total, wb + share);
-        stream_quads(acc, bdst, bval, scale, q0 + wb + lane, q0 + we);
+        stream_quads<HOT ? kP2HotQuads : kP2Quads>(acc, bdst, bval, scale, q0 + wb + lane, q0 + we);
+        if (HOT && hi.y > hi.x) {
+            const uint32_t h0 = hi.x / kQuad, htot = hi.y / kQuad - h0;
+            const uint32_t hs = ((htot + kWaves - 1) / kWaves + 63) & ~63u;
+            const uint32_t hb = min(htot, wave * hs), he = min(htot, hb + hs);
+            stream_hot<kP2HotQuads>(acc, htab, hidx, hcoef, scale, h0 + hb + lane, h0 + he);
+        }
         __syncthreads();
         const uint32_t base = item.x << kBinShift;
         if ((item.w & kItemSole) && minus) {
@@ -919,6 +1070,35 @@ void emit(nkpr_graph *g, BinLayout &L, const uint32_t *row) {
     NKPR_HIP(hipGetLastError());
 }
 
+void emit_hot(nkpr_graph *g, BinLayout &L, const uint32_t *row) {
+    if (!L.mh) return;
+    hipLaunchKernelGGL(bin_hot_emit_kernel, dim3(nb(L.mh)), dim3(256), 0, g->stream,
+                       L.horder.as<uint32_t>(), L.mh, g->b_src.as<int32_t>(), row,
+                       g->weighted ? g->b_coef32.as<float>() : nullptr, L.hidx.as<uint32_t>(),
+                       g->weighted ? L.hcoef.as<float>() : nullptr);
+    NKPR_HIP(hipGetLastError());
+}
+
+// The number of hot sources of the FP32 copy: NKPR_BINNED_HOT=0 turns the bypass off, a value
+// k >= 1 takes min(k, kHot) ids (tests use small ones); by default kHot ids when their edges
+// are at least 5% of all (power-law sources; graphs without hubs keep the two-pass route).
+uint32_t choose_hot(nkpr_graph *g, const BinLayout &L) {
+    if (L.fp64 || g->n == 0 || g->m == 0) return 0;
+    if (const char *e = std::getenv("NKPR_BINNED_HOT"); e && *e) {
+        const long long k = std::atoll(e);
+        return k <= 0 ? 0u : (uint32_t)std::min<long long>(k, kHot);
+    }
+    DevBuf cnt;
+    cnt.alloc(8);
+    NKPR_HIP(hipMemsetAsync(cnt.p, 0, 8, g->stream));
+    hipLaunchKernelGGL(bin_count_below_kernel, dim3(nb(g->m)), dim3(256), 0, g->stream,
+                       g->b_src.as<int32_t>(), g->m, kHot, cnt.as<unsigned long long>());
+    NKPR_HIP(hipGetLastError());
+    unsigned long long c = 0;
+    read_back(g->stream, {{&c, cnt.p, 8}});
+    return c * 20 >= g->m ? kHot : 0u;
+}
+
 // Builds the binned copy L (once per graph; coefficients again after new weights).  Returns
 // false when the padded copy would not fit 32-bit positions.
 bool build_layout(nkpr_graph *g, BinLayout &L) {
@@ -941,27 +1121,35 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
                            g->b_rowptr.as<uint32_t>(), n, row.as<uint32_t>());
     if (L.ready) {
         // new coefficients, same order: re-emit from the kept order
-        if (g->weighted) emit(g, L, row.as<uint32_t>());
+        if (g->weighted) {
+            emit(g, L, row.as<uint32_t>());
+            emit_hot(g, L, row.as<uint32_t>());
+        }
         NKPR_HIP(hipStreamSynchronize(s));
         binned_insum(g, L);
         L.coef_ready = true;
         return true;
     }
+    // hot sources skip pass 1: their edges sort into one extra segment per bin (a chunk past
+    // the last one)
+    L.hot = choose_hot(g, L);
+    const uint64_t nseg_all = nseg + (L.hot ? nbins : 0);
     // sort the edges by key and find the unpadded segment starts (chunk-major)
-    std::vector<uint32_t> seg(nseg + 1, 0u);
-    rseg.alloc(4 * (nseg + 1));
+    std::vector<uint32_t> seg(nseg_all + 1, 0u);
+    rseg.alloc(4 * (nseg_all + 1));
     if (m) {
         k1.alloc(8 * m);
         k2.alloc(8 * m);
         v1.alloc(4 * m);
         v2.alloc(4 * m);
         hipLaunchKernelGGL(bin_key_kernel, dim3(nb(m)), dim3(256), 0, s, g->b_src.as<int32_t>(),
-                           row.as<uint32_t>(), m, nbins, ss, k1.as<uint64_t>(), v1.as<uint32_t>());
+                           row.as<uint32_t>(), m, nbins, ss, nchunks, L.hot, k1.as<uint64_t>(),
+                           v1.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
         hipcub::DoubleBuffer<uint64_t> kb(k1.as<uint64_t>(), k2.as<uint64_t>());
         hipcub::DoubleBuffer<uint32_t> vb(v1.as<uint32_t>(), v2.as<uint32_t>());
         int bits = ss + 1;
-        while (bits < 64 && (1ull << (bits - ss)) < nseg) ++bits;
+        while (bits < 64 && (1ull << (bits - ss)) < nseg_all) ++bits;
         size_t tb = 0;
         NKPR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (size_t)m, 0, bits, s));
         tmp.alloc(tb);
@@ -973,10 +1161,10 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
         k2.reset();
         v2.reset();
         tmp.reset();
-        hipLaunchKernelGGL(bin_seg_kernel, dim3(nb(nseg + 1)), dim3(256), 0, s,
-                           (const uint64_t *)k1.as<uint64_t>(), m, nseg, ss, rseg.as<uint32_t>());
+        hipLaunchKernelGGL(bin_seg_kernel, dim3(nb(nseg_all + 1)), dim3(256), 0, s,
+                           (const uint64_t *)k1.as<uint64_t>(), m, nseg_all, ss, rseg.as<uint32_t>());
         NKPR_HIP(hipGetLastError());
-        read_back(s, {{seg.data(), rseg.p, 4 * (nseg + 1)}});
+        read_back(s, {{seg.data(), rseg.p, 4 * (nseg_all + 1)}});
     }
     // padded segment starts: every segment rounded up to whole quads, laid out chunk-major
     // (pass 1) and bin-major (pass 2); both indexed by the chunk-major segment number
@@ -1015,9 +1203,36 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
     NKPR_HIP(hipMemsetAsync(L.order.p, 0xFF, 4 * std::max<uint64_t>(mp, 1), s));
     if (m)
         hipLaunchKernelGGL(bin_place_kernel, dim3(nb(m)), dim3(256), 0, s, (const uint64_t *)k1.as<uint64_t>(),
-                           (const uint32_t *)v1.as<uint32_t>(), m, ss, (const uint32_t *)rseg.as<uint32_t>(),
+                           (const uint32_t *)v1.as<uint32_t>(), m, ss, nseg, (const uint32_t *)rseg.as<uint32_t>(),
                            (const uint32_t *)cseg.as<uint32_t>(), L.order.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
+    // hot entries: per bin, sorted by source, padded to whole quads
+    std::vector<uint32_t> hseg(nbins + 1, 0u);
+    if (L.hot) {
+        uint64_t h = 0;
+        for (uint32_t b = 0; b < nbins; ++b) {
+            hseg[b] = (uint32_t)h;
+            h += ((uint64_t)(seg[nseg + b + 1] - seg[nseg + b]) + kQuad - 1) / kQuad * kQuad;
+        }
+        NKPR_CHECK(h <= kMaxPadded, "binned layout: hot entries exceed 32-bit positions");
+        hseg[nbins] = (uint32_t)h;
+        L.mh = h;
+        DevBuf hs;
+        hs.alloc(4 * (nbins + 1));
+        NKPR_HIP(hipMemcpyAsync(hs.p, hseg.data(), 4 * (nbins + 1), hipMemcpyHostToDevice, s));
+        L.horder.alloc(4 * std::max<uint64_t>(h, 1));
+        NKPR_HIP(hipMemsetAsync(L.horder.p, 0xFF, 4 * std::max<uint64_t>(h, 1), s));
+        if (m > seg[nseg])
+            hipLaunchKernelGGL(bin_hot_place_kernel, dim3(nb(m - seg[nseg])), dim3(256), 0, s,
+                               (const uint64_t *)k1.as<uint64_t>(), (const uint32_t *)v1.as<uint32_t>(),
+                               (uint64_t)seg[nseg], m, ss, nseg, (const uint32_t *)rseg.as<uint32_t>(),
+                               (const uint32_t *)hs.as<uint32_t>(), L.horder.as<uint32_t>());
+        NKPR_HIP(hipGetLastError());
+        L.hidx.alloc(4 * std::max<uint64_t>(h, 1) + 64);
+        if (g->weighted) L.hcoef.alloc(4 * std::max<uint64_t>(h, 1) + 64);
+        emit_hot(g, L, row.as<uint32_t>());
+        NKPR_HIP(hipStreamSynchronize(s));  // hs goes out of scope
+    }
     L.pos.alloc(4 * std::max<uint64_t>(mp / kQuad, 1));
     if (mp)
         hipLaunchKernelGGL(bin_pos_kernel, dim3((unsigned)std::min<uint64_t>((nseg * 64 + 255) / 256, 1u << 18)),
@@ -1102,6 +1317,7 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
         if (v > 0) e1max = e2max = (v + kQuad - 1) / kQuad * kQuad;
     }
     std::vector<uint4> p1, p2, mb;  // mb: {bin, first slot, slot count, partials slot}
+    std::vector<uint2> p2h;          // hot entries of each pass-2 item
     std::vector<uint64_t> cost1, cost2;
     for (uint32_t c = 0; c < nchunks; ++c) {
         const uint32_t b0 = chost[(uint64_t)c * nbins], b1 = chost[(uint64_t)(c + 1) * nbins];
@@ -1110,17 +1326,31 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
             p1.push_back(make_uint4(c, (uint32_t)e, (uint32_t)e1, 0));
             cost1.push_back(e1 - e + span);  // edges + the LDS table fill
         }
+        // the fixed-point scale needs max |v| of the hot sources too: their chunk keeps an
+        // item (an empty one when all its edges are hot), whose table fill finds it
+        if (c == 0 && L.hot && b0 == b1) {
+            p1.push_back(make_uint4(0, b0, b0, 0));
+            cost1.push_back(span);
+        }
     }
     // every bin finished in one place (a sole item or the reduce kernel) has a partials slot
     uint32_t slots = 0, pslots = 0;
     for (uint32_t b = 0; b < nbins; ++b) {
+        // the bin's work: its contributions, then its hot entries; pieces of at most e2max
         const uint64_t s0 = bin_start[b], s1 = bin_start[b + 1];
-        const uint32_t pieces = (uint32_t)std::max<uint64_t>(1, (s1 - s0 + e2max - 1) / e2max);
+        const uint64_t h0 = L.hot ? hseg[b] : 0, h1 = L.hot ? hseg[b + 1] : 0;
+        const uint64_t work = (s1 - s0) + (h1 - h0);
+        const uint32_t pieces = (uint32_t)std::max<uint64_t>(1, (work + e2max - 1) / e2max);
         for (uint32_t i = 0; i < pieces; ++i) {
-            const uint64_t e0 = s0 + i * e2max, e1 = std::min<uint64_t>(e0 + e2max, s1);
+            const uint64_t a = i * e2max, z = std::min<uint64_t>(a + e2max, work);
+            const uint64_t e0 = s0 + std::min(a, s1 - s0), e1 = s0 + std::min(z, s1 - s0);
+            const uint64_t f0 = h0 + (a > s1 - s0 ? a - (s1 - s0) : 0);
+            const uint64_t f1 = h0 + (z > s1 - s0 ? z - (s1 - s0) : 0);
             const uint32_t flags = pieces == 1 ? kItemSole | (pslots << 2) : (slots + i) << 2;
             p2.push_back(make_uint4(b, (uint32_t)e0, (uint32_t)e1, flags));
-            cost2.push_back(e1 - e0 + kBin);  // edges + the accumulator clear and flush
+            p2h.push_back(make_uint2((uint32_t)f0, (uint32_t)f1));
+            // entries + the accumulator clear and flush (+ the hot table when staged)
+            cost2.push_back((e1 - e0) + (f1 - f0) + kBin + (f1 > f0 ? kHot : 0));
         }
         if (pieces > 1) {
             // every item of a split bin writes its accumulator to a slot of its own; the FP64
@@ -1133,7 +1363,22 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
         }
     }
     sort_by_cost(p1, cost1);
-    sort_by_cost(p2, cost2);
+    {
+        // p2 and p2h in the same order
+        std::vector<uint4> idx(p2.size());
+        for (size_t i = 0; i < p2.size(); ++i) idx[i] = make_uint4((uint32_t)i, 0, 0, 0);
+        sort_by_cost(idx, cost2);
+        std::vector<uint4> q2(p2.size());
+        std::vector<uint2> qh(p2.size());
+        for (size_t i = 0; i < idx.size(); ++i) {
+            q2[i] = p2[idx[i].x];
+            qh[i] = p2h[idx[i].x];
+        }
+        p2.swap(q2);
+        p2h.swap(qh);
+    }
+    L.p2h.alloc(sizeof(uint2) * std::max<size_t>(p2h.size(), 1));
+    if (!p2h.empty()) NKPR_HIP(hipMemcpyAsync(L.p2h.p, p2h.data(), sizeof(uint2) * p2h.size(), hipMemcpyHostToDevice, s));
     L.p1.alloc(sizeof(uint4) * std::max<size_t>(p1.size(), 1));
     L.p2.alloc(sizeof(uint4) * std::max<size_t>(p2.size(), 1));
     if (!p1.empty()) NKPR_HIP(hipMemcpyAsync(L.p1.p, p1.data(), sizeof(uint4) * p1.size(), hipMemcpyHostToDevice, s));
@@ -1151,8 +1396,9 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
     L.ctl.alloc(sizeof(BinCtl));
     NKPR_HIP(hipStreamSynchronize(s));
     if (std::getenv("NKPR_DEBUG"))
-        std::fprintf(stderr, "[nkpr] binned %s layout: %u chunks x %u bins, %llu padded edges, %u + %u items (%u split)\n",
-                     L.fp64 ? "FP64" : "FP32", nchunks, nbins, (unsigned long long)mp, L.np1, L.np2, L.multi);
+        std::fprintf(stderr, "[nkpr] binned %s layout: %u chunks x %u bins, %llu padded edges, %llu hot entries (%u hot sources), %u + %u items (%u split)\n",
+                     L.fp64 ? "FP64" : "FP32", nchunks, nbins, (unsigned long long)mp,
+                     (unsigned long long)L.mh, L.hot, L.np1, L.np2, L.multi);
     L.ready = true;
     L.coef_ready = true;
     return true;
@@ -1160,8 +1406,11 @@ bool build_layout(nkpr_graph *g, BinLayout &L) {
 
 void release_layout(BinLayout &L) {
     for (DevBuf *b : {&L.order, &L.pos, &L.src, &L.dst, &L.coef, &L.val, &L.p1, &L.p2, &L.mb,
-                      &L.part, &L.ctl, &L.partials, &L.gmask, &L.gfirst, &L.sdelta})
+                      &L.part, &L.ctl, &L.partials, &L.gmask, &L.gfirst, &L.sdelta, &L.hidx,
+                      &L.hcoef, &L.horder, &L.p2h})
         b->reset();
+    L.hot = 0;
+    L.mh = 0;
     L.ready = L.coef_ready = L.derived = false;
 }
 
@@ -1232,7 +1481,8 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
     // NKPR_BINNED_POS=1 reads the stored places instead of deriving them
     const char *pe = std::getenv("NKPR_BINNED_POS");
     const bool derived = L.derived && !(pe && *pe && std::atoi(pe) != 0);
-    persistent_grids(g, L, bin_scatter_kernel<true>, bin_gather_kernel);
+    const auto gather = L.hot ? bin_gather_kernel<true> : bin_gather_kernel<false>;
+    persistent_grids(g, L, bin_scatter_kernel<true>, gather);
     if (ev0) NKPR_HIP(hipEventRecord(ev0, s));
     BinCtl *ctl = L.ctl.as<BinCtl>();
     hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
@@ -1243,10 +1493,12 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
                        g->weighted ? L.coef.as<float>() : nullptr, L.pos.as<uint32_t>(),
                        L.gmask.as<unsigned long long>(), L.gfirst.as<uint32_t>(), L.sdelta.as<uint32_t>(),
                        L.nsdelta, (uint32_t)L.mp, L.val.as<float>(), ctl);
-    hipLaunchKernelGGL(bin_gather_kernel, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
+    hipLaunchKernelGGL(gather, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np2, 1), L.grid2)),
                        dim3(kPassThreads), 0, s, L.p2.as<uint4>(), L.np2, L.dst.as<uint16_t>(),
                        L.val.as<float>(), L.part.as<long long>(), n, ctl, L.insum, v, minus, (float)damp,
-                       sink_sum, w);
+                       sink_sum, w, L.p2h.as<uint2>(), L.hidx.as<uint32_t>(),
+                       g->weighted ? L.hcoef.as<float>() : nullptr,
+                       g->weighted ? nullptr : g->inv32.as<float>(), L.hot);
     if (L.nmb)
         hipLaunchKernelGGL(bin_reduce_kernel, dim3(nb((uint64_t)L.nmb * kBin)), dim3(256), 0, s,
                            L.mb.as<uint4>(), L.nmb, (const long long *)L.part.as<long long>(), n,

@@ -371,6 +371,12 @@ struct BinLayout {
     bool derived = false;
     double insum = 0.0;         // largest in-coefficient sum of a row (fixed-point bound)
     uint32_t np1 = 0, np2 = 0, multi = 0, nmb = 0, nslots = 0, grid1 = 0, grid2 = 0;
+    // hot sources (FP32 copy): ids [0, hot) bypass pass 1; their entries per bin, padded to
+    // quads (hidx packed offsets, hcoef, horder the edge of each position), the hot range of
+    // every pass-2 item (p2h)
+    uint32_t hot = 0;
+    uint64_t mh = 0;
+    DevBuf hidx, hcoef, horder, p2h;
 };
 }  // namespace nkpr
 
