@@ -285,26 +285,27 @@ typedef struct {
 
 /* Replaces CSRGeneralMatrix::operator*(const Vector&) (CSRGeneralMatrix.hpp:563-578):
  * y[i] = sum over row i of values[k] * x[col_idx[k]].  x[cols] and y[rows] are distinct device
- * vectors.  Enqueued on `stream` (a hipStream_t; NULL is the null stream); returns without
- * waiting.  The matrix structure is taken as given (see nkpr_algebraic_pagerank for a run
- * that checks it). */
+ * vectors; rows < 2^32 - 1.  Enqueued on `stream` (a hipStream_t; NULL is the null stream);
+ * returns without waiting.  Its scratch comes from the stream's memory pool.  The matrix
+ * structure is taken as given (see nkpr_algebraic_pagerank for a run that checks it). */
 int nkpr_csr_spmv(const nkpr_csr *a, const double *x, double *y, void *stream);
 
 /* One iteration of AlgebraicPageRank::run (AlgebraicPageRank.hpp:97-101) on a square matrix
  * M: y = M x + shift and *step_sq = ||y - x||_2^2 (FP64, a device pointer; summed in an order
- * fixed by the matrix).  Stream-ordered like nkpr_csr_spmv; its partial sums take
- * at most 4096 doubles from the stream's memory pool for the duration of the call. */
+ * fixed by the matrix).  Stream-ordered like nkpr_csr_spmv.  Like nkpr_csr_spmv it takes its
+ * scratch (the list of rows longer than 32 x the lanes per row, their pieces of 4096 entries,
+ * block sums: about 8 bytes per 4096 entries) from the stream's memory pool for the call. */
 int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, double *y,
                            double *step_sq, void *stream);
 
 /* No counterpart in the reference: nkpr_csr_pagerank_step for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
- * hipMalloc).  Same arguments and results; its partial sums live in a 32 KiB device array of
- * the library (4096 doubles, held by every GPU the library runs on until the process ends)
- * instead of the stream's pool, and it calls no allocating routine.  Every call on one GPU
- * shares that array: calls that overlap on one GPU (other streams, other threads) race and
- * must be ordered by the caller.  A matrix whose step would need more than 4096 partials is
- * rejected (NKPR_ERR_INVALID; the step's grids are capped so that none does). */
+ * hipMalloc).  Same arguments and results; its scratch lives in a 12.1 MiB device array of the
+ * library (held by every GPU the library runs on until the process ends) instead of the
+ * stream's pool, and it calls no allocating routine.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race and must be
+ * ordered by the caller.  A matrix with more than 2^19 long rows or 2^20 pieces of 4096
+ * entries (beyond about 1.07e9 entries) is rejected with NKPR_ERR_INVALID before any launch. */
 int nkpr_csr_pagerank_step_noalloc(const nkpr_csr *m, double shift, const double *x, double *y,
                                    double *step_sq, void *stream);
 

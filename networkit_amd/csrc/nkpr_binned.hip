@@ -75,13 +75,13 @@ constexpr uint64_t kMaxPadded = (1ull << 32) - (1ull << 16);
 #define NKPR_P2_QUADS 2
 #endif
 constexpr int kP1Quads = NKPR_P1_QUADS, kP2Quads = NKPR_P2_QUADS;
-// Hot sources of the FP32 operator: the first `hot` relabelled ids (the sources with the most
-// out-edges), at most kHot.  Their edges skip pass 1 and the contribution array: pass 2 keeps
-// their values in an LDS table next to the accumulators and gathers from it, so such an edge
-// moves 8 bytes (packed source and destination offsets, coefficient) instead of 16.  A hot
-// entry packs (source << kBinShift) | destination offset; padding points at the zero entry
-// tab[kHot].  With the table (64 KB) beside the accumulators (64 KB) one pass-2 block fits a
-// CU, so that kernel keeps more quads in flight per lane.
+// Hot sources of the FP32 operator (optional, NKPR_BINNED_HOT): the first `hot` relabelled ids
+// (the sources with the most out-edges), at most kHot.  Their edges skip pass 1 and the
+// contribution array: pass 2 keeps their values in an LDS table next to the accumulators and
+// gathers from it, so such an edge moves 8 bytes (packed source and destination offsets,
+// coefficient) instead of 16.  A hot entry packs (source << kBinShift) | destination offset;
+// padding points at the zero entry tab[kHot].  With the table (64 KB) beside the accumulators
+// (64 KB) one pass-2 block fits a CU, so that kernel keeps more quads in flight per lane.
 constexpr int kHotShift = 14;
 constexpr uint32_t kHot = 1u << kHotShift;
 static_assert(kHotShift + 1 + kBinShift <= 32, "hot entries pack into 32 bits");
@@ -1079,12 +1079,17 @@ void emit_hot(nkpr_graph *g, BinLayout &L, const uint32_t *row) {
     NKPR_HIP(hipGetLastError());
 }
 
-// The number of hot sources of the FP32 copy: NKPR_BINNED_HOT=0 turns the bypass off, a value
-// k >= 1 takes min(k, kHot) ids (tests use small ones); by default kHot ids when their edges
-// are at least 5% of all (power-law sources; graphs without hubs keep the two-pass route).
+// The number of hot sources of the FP32 copy.  The bypass is off unless NKPR_BINNED_HOT asks
+// for it: a value k >= 1 takes min(k, kHot) ids, "auto" takes kHot ids when their edges are
+// at least 5% of all.  Measured on R-MAT scale 26 (20% of the edges hot) in one process with
+// both copies built: 3.33 ms per application with the bypass against 3.03 ms without, same
+// results; the table halves pass 2's occupancy (one block per CU) and that costs more than
+// the bytes it saves (16.4 instead of 18.2 GB of traffic per application).
 uint32_t choose_hot(nkpr_graph *g, const BinLayout &L) {
     if (L.fp64 || g->n == 0 || g->m == 0) return 0;
-    if (const char *e = std::getenv("NKPR_BINNED_HOT"); e && *e) {
+    const char *e = std::getenv("NKPR_BINNED_HOT");
+    if (!e || !*e) return 0;
+    if (std::strcmp(e, "auto") != 0) {
         const long long k = std::atoll(e);
         return k <= 0 ? 0u : (uint32_t)std::min<long long>(k, kHot);
     }

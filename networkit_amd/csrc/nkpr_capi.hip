@@ -648,7 +648,7 @@ int nkpr_csr_spmv(const nkpr_csr *a, const double *x, double *y, void *stream) {
 int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, double *y,
                            double *step_sq, void *stream) {
     return guarded([&] {
-        csr_step_impl(m, shift, x, y, step_sq, nullptr, 0, (hipStream_t)stream);
+        csr_step_impl(m, shift, x, y, step_sq, (hipStream_t)stream);
     });
 }
 

@@ -12,16 +12,27 @@
 //   nkpr_algebraic_pagerank AlgebraicPageRank::run (AlgebraicPageRank.hpp:94-119): the loop to
 //                           ||y - x||_2 <= tol from 1/n, then y / sum(y)
 //
-// Kernels.  Rows are processed in two kernels, both of them deterministic (every sum is added
-// in an order fixed by the matrix alone):
-//   csr_short_kernel<G>  rows with at most kLongRow entries; a group of G lanes (4..64, chosen
-//       from the average row length) owns a row, strides over its entries and reduces with
-//       shuffles.  The grid is capped at kShortGrid blocks that step over row tiles.
-//   csr_long_kernel      rows with more entries; each of at most kLongGrid blocks of 1024
-//       threads scans a contiguous row range for them and reduces each with the whole block.
-// The step variant fuses the shift and (y - x)^2: each block writes one FP64 partial, and a
-// one-block kernel adds the partials in block order.  Its scratch is therefore at most
-// kShortGrid + kLongGrid doubles whatever the size of the matrix.
+// Kernels.  Every sum is added in an order fixed by the matrix alone (deterministic):
+//   csr_short_kernel<G>   rows of at most 32 G entries: a group of G lanes (4..64, chosen from
+//       the average row length) owns a row, strides over it and reduces with shuffles; at most
+//       kShortGrid blocks step over row tiles.
+//   long rows             the rest, cut into pieces of kPiece entries so that a few huge rows
+//       (power-law matrices) spread over the whole device:
+//       csr_long_scan_kernel    kLongGrid blocks count the long rows and pieces of their row
+//                               ranges;
+//       csr_long_offsets_kernel one block: exclusive sums of those counts;
+//       csr_long_list_kernel    the same blocks write {row, first piece} of their long rows,
+//                               in row order;
+//       csr_piece_kernel        a block per piece (grid-stride): the piece's partial sum;
+//       csr_long_fix_kernel     per long row: its pieces added in order, the shift, y, and
+//                               for a step (y - x)^2.
+// The step's (y - x)^2 is added per block (short kernel and fix kernel), and the block sums in
+// block order by a one-block kernel into a device scalar.
+//
+// Scratch of a product: the long-row list (at most nnz / (32 G + 1) rows) and piece sums (at
+// most nnz / kPiece + list rows), the block counts, and the step's block sums; a plain product
+// and a step take it from the stream's memory pool for the duration of the call, the
+// allocation-free step from a static device array (which caps the matrices it accepts).
 //
 // Algorithmic bytes per product (HBM-bound, random gathers of x): per entry col 8 + value 8 +
 // gathered x 8; per row row_idx 8 + y 8 (+ x 8 for the step's norm).
@@ -30,27 +41,32 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "nkpr_internal.hpp"
 
-#include <mutex>
-
-// Scratch of nkpr_csr_pagerank_step_noalloc: the step's per-block partials (at most
-// kShortGrid + kLongGrid = 4096 doubles, 32 KiB) in a device array that every GPU the library
-// runs on holds until the process ends.  Host code never uses this name as an address: it
-// takes the array's address per device with hipGetSymbolAddress (csr_noalloc_scratch).
-static __device__ __align__(256) double scratch[4096];
+// Scratch of nkpr_csr_pagerank_step_noalloc (12.1 MiB): the long-row list (2^19 rows), the
+// piece sums (2^20), the block counts and offsets, and the step's block sums.  Every GPU the
+// library runs on holds it until the process ends.  Host code never uses this name as an
+// address: it takes the array's address per device with hipGetSymbolAddress
+// (csr_noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[(1u << 19) * 8 + (1u << 20) * 8 + 65536];
 
 namespace nkpr {
 namespace {
 
-constexpr uint32_t kLongRow = 2048;
 constexpr int kShortThreads = 256;
 constexpr uint32_t kShortGrid = 2048;
-constexpr int kLongThreads = 1024;
+constexpr int kLongThreads = 1024;   // scan and list blocks
 constexpr uint32_t kLongGrid = 2048;
+constexpr int kPieceThreads = 256;
+constexpr uint32_t kPiece = 4096;    // entries per piece of a long row
+constexpr uint32_t kPieceGrid = 4096;
+constexpr int kFixThreads = 256;
+constexpr uint32_t kFixGrid = 256;
 constexpr int kFinalThreads = 256;
+constexpr uint32_t kNoallocList = 1u << 19, kNoallocPieces = 1u << 20;
 
 struct CsrView {
     uint64_t rows;
@@ -58,6 +74,45 @@ struct CsrView {
     const uint64_t *col_idx;
     const double *val;
 };
+
+// The scratch of one product, carved from one buffer.
+struct LongWork {
+    uint32_t *bcount;   // [kLongGrid] long rows per scan block
+    uint32_t *bpieces;  // [kLongGrid] pieces per scan block
+    uint32_t *boff;     // [kLongGrid + 1] first list slot per scan block
+    uint32_t *bpoff;    // [kLongGrid + 1] first piece per scan block
+    uint32_t *total;    // [2] long rows, pieces
+    uint2 *list;        // [list cap] {row, first piece}
+    double *piece;      // [piece cap]
+    double *partials;   // [kShortGrid + kFixGrid] the step's block sums
+};
+
+size_t work_bytes(uint64_t list_cap, uint64_t piece_cap) {
+    return 8 * list_cap + 8 * piece_cap + 4 * (4 * (size_t)kLongGrid + 4) +
+           8 * (size_t)(kShortGrid + kFixGrid) + 256;
+}
+
+LongWork carve(void *base, uint64_t list_cap, uint64_t piece_cap) {
+    char *p = static_cast<char *>(base);
+    LongWork w;
+    w.list = reinterpret_cast<uint2 *>(p);
+    p += 8 * list_cap;
+    w.piece = reinterpret_cast<double *>(p);
+    p += 8 * piece_cap;
+    w.partials = reinterpret_cast<double *>(p);
+    p += 8 * (size_t)(kShortGrid + kFixGrid);
+    uint32_t *u = reinterpret_cast<uint32_t *>(p);
+    w.bcount = u;
+    w.bpieces = u + kLongGrid;
+    w.boff = u + 2 * kLongGrid;
+    w.bpoff = u + 3 * kLongGrid + 1;
+    w.total = u + 4 * kLongGrid + 2;
+    return w;
+}
+
+__device__ __forceinline__ uint32_t pieces_of(uint64_t len) {
+    return (uint32_t)((len + kPiece - 1) / kPiece);
+}
 
 // Sum of the products of row entries [b, e) taken by one lane in steps of `stride`; two
 // accumulators keep two gathers in flight.
@@ -75,6 +130,22 @@ __device__ __forceinline__ double row_part(const CsrView &A, const double *__res
     return s0 + s1;
 }
 
+// Block sum (fixed order: shuffles inside each wave, then thread 0 adds the waves); the
+// result is valid in thread 0.
+template <int kThreads>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double wsum[kThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
+    __syncthreads();
+    return tot;
+}
+
 template <int G, bool STEP>
 __global__ __launch_bounds__(kShortThreads) void csr_short_kernel(CsrView A,
                                                                   const double *__restrict__ x,
@@ -82,6 +153,7 @@ __global__ __launch_bounds__(kShortThreads) void csr_short_kernel(CsrView A,
                                                                   double *partials,
                                                                   uint64_t tiles) {
     constexpr uint32_t groups = kShortThreads / G;
+    constexpr uint64_t kLong = 32u * G;
     const uint32_t lane = threadIdx.x % G;
     const uint32_t group = threadIdx.x / G;
     double sq = 0.0;
@@ -91,7 +163,7 @@ __global__ __launch_bounds__(kShortThreads) void csr_short_kernel(CsrView A,
         bool own = false;
         if (r < A.rows) {
             const uint64_t b = A.row_idx[r], e = A.row_idx[r + 1];
-            own = e - b <= kLongRow;
+            own = e - b <= kLong;
             if (own) s = row_part(A, x, b + lane, e, G);
         }
 #pragma unroll
@@ -106,78 +178,162 @@ __global__ __launch_bounds__(kShortThreads) void csr_short_kernel(CsrView A,
         }
     }
     if (STEP) {
-        // block sum in a fixed order: within each wave by shuffles, then wave 0 adds the waves
-        __shared__ double wsum[kShortThreads / 64];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
-        if (threadIdx.x % 64 == 0) wsum[threadIdx.x / 64] = sq;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double tot = 0.0;
-            for (int w = 0; w < kShortThreads / 64; ++w) tot += wsum[w];
-            partials[blockIdx.x] = tot;
-        }
+        const double tot = block_sum<kShortThreads>(sq);
+        if (threadIdx.x == 0) partials[blockIdx.x] = tot;
     }
 }
 
-template <bool STEP>
-__global__ __launch_bounds__(kLongThreads) void csr_long_kernel(CsrView A,
-                                                                const double *__restrict__ x,
-                                                                double shift, double *y,
-                                                                double *partials,
-                                                                uint64_t rows_per_block) {
-    __shared__ uint64_t list[kLongThreads];
-    __shared__ uint32_t wcount[kLongThreads / 64];
-    __shared__ double wsum[kLongThreads / 64];
-    __shared__ uint32_t nlist;
-    const uint32_t tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_block;
-    const uint64_t r1 = std::min<uint64_t>(A.rows, r0 + rows_per_block);
-    double sq = 0.0;  // thread 0 only
-    for (uint64_t base = r0; base < r1; base += kLongThreads) {
-        // the long rows of this chunk, in row order
-        const uint64_t r = base + tid;
-        const bool is_long = r < r1 && A.row_idx[r + 1] - A.row_idx[r] > kLongRow;
-        const uint64_t mask = __ballot(is_long);
-        if (lane == 0) wcount[wave] = (uint32_t)__popcll(mask);
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t acc = 0;
-            for (int w = 0; w < kLongThreads / 64; ++w) {
-                const uint32_t c = wcount[w];
-                wcount[w] = acc;
-                acc += c;
-            }
-            nlist = acc;
-        }
-        __syncthreads();
-        if (is_long) {
-            const uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
-            list[wcount[wave] + (uint32_t)__popcll(below)] = r;
-        }
-        __syncthreads();
-        const uint32_t cnt = nlist;
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const uint64_t row = list[i];
-            double s = row_part(A, x, A.row_idx[row] + tid, A.row_idx[row + 1], kLongThreads);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-            if (lane == 0) wsum[wave] = s;
-            __syncthreads();
-            if (tid == 0) {
-                double tot = 0.0;
-                for (int w = 0; w < kLongThreads / 64; ++w) tot += wsum[w];
-                const double v = tot + shift;
-                y[row] = v;
-                if (STEP) {
-                    const double d = v - x[row];
-                    sq += d * d;
-                }
-            }
-            __syncthreads();
+// Long rows of block b's row range [b rpb, (b + 1) rpb): count and pieces.
+__global__ __launch_bounds__(kLongThreads) void csr_long_scan_kernel(CsrView A, uint64_t rpb,
+                                                                     uint64_t lim, LongWork w) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+    const uint64_t r1 = std::min<uint64_t>(A.rows, r0 + rpb);
+    uint32_t cnt = 0, pcs = 0;
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += kLongThreads) {
+        const uint64_t len = A.row_idx[r + 1] - A.row_idx[r];
+        if (len > lim) {
+            ++cnt;
+            pcs += pieces_of(len);
         }
     }
-    if (STEP && tid == 0) partials[blockIdx.x] = sq;
+    __shared__ uint32_t sc[kLongThreads / 64], sp[kLongThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        pcs += __shfl_xor(pcs, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sc[threadIdx.x >> 6] = cnt;
+        sp[threadIdx.x >> 6] = pcs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, q = 0;
+        for (int i = 0; i < kLongThreads / 64; ++i) {
+            c += sc[i];
+            q += sp[i];
+        }
+        w.bcount[blockIdx.x] = c;
+        w.bpieces[blockIdx.x] = q;
+    }
+}
+
+// One block: exclusive sums of the scan blocks' counts; total = {long rows, pieces}.
+__global__ void csr_long_offsets_kernel(LongWork w, uint32_t nblocks) {
+    if (threadIdx.x != 0) return;
+    uint32_t c = 0, q = 0;
+    for (uint32_t b = 0; b < nblocks; ++b) {
+        w.boff[b] = c;
+        w.bpoff[b] = q;
+        c += w.bcount[b];
+        q += w.bpieces[b];
+    }
+    w.boff[nblocks] = c;
+    w.bpoff[nblocks] = q;
+    w.total[0] = c;
+    w.total[1] = q;
+}
+
+// The long rows of block b's range in row order: list[boff[b] + i] = {row, first piece}.
+__global__ __launch_bounds__(kLongThreads) void csr_long_list_kernel(CsrView A, uint64_t rpb,
+                                                                     uint64_t lim, LongWork w) {
+    __shared__ uint32_t wc[kLongThreads / 64], wp[kLongThreads / 64];
+    __shared__ uint32_t base_c, base_p;
+    const uint32_t tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+    const uint64_t r1 = std::min<uint64_t>(A.rows, r0 + rpb);
+    if (tid == 0) {
+        base_c = w.boff[blockIdx.x];
+        base_p = w.bpoff[blockIdx.x];
+    }
+    __syncthreads();
+    for (uint64_t base = r0; base < r1; base += kLongThreads) {
+        const uint64_t r = base + tid;
+        uint32_t np = 0;
+        if (r < r1) {
+            const uint64_t len = A.row_idx[r + 1] - A.row_idx[r];
+            if (len > lim) np = pieces_of(len);
+        }
+        // in-wave inclusive prefix sums of the long-row count and of the pieces
+        uint32_t ic = np ? 1u : 0u, ip = np;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t tc = __shfl_up(ic, o, 64), tp = __shfl_up(ip, o, 64);
+            if (lane >= (uint32_t)o) {
+                ic += tc;
+                ip += tp;
+            }
+        }
+        if (lane == 63) {
+            wc[wave] = ic;
+            wp[wave] = ip;
+        }
+        __syncthreads();
+        uint32_t pc = base_c, pp = base_p;
+        for (uint32_t k = 0; k < wave; ++k) {
+            pc += wc[k];
+            pp += wp[k];
+        }
+        if (np) w.list[pc + ic - 1] = make_uint2((uint32_t)r, pp + ip - np);
+        __syncthreads();
+        if (tid == 0) {
+            for (int k = 0; k < kLongThreads / 64; ++k) {
+                base_c += wc[k];
+                base_p += wp[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Partial sum of every piece (a block per piece, grid-stride over total[1] pieces).
+__global__ __launch_bounds__(kPieceThreads) void csr_piece_kernel(CsrView A,
+                                                                  const double *__restrict__ x,
+                                                                  LongWork w) {
+    const uint32_t nrows = w.total[0], npieces = w.total[1];
+    for (uint32_t p = blockIdx.x; p < npieces; p += gridDim.x) {
+        // the long row holding piece p: the last list entry whose first piece is <= p
+        uint32_t lo = 0, hi = nrows;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (w.list[mid].y <= p) lo = mid;
+            else hi = mid;
+        }
+        const uint2 e = w.list[lo];
+        const uint64_t rb = A.row_idx[e.x], re = A.row_idx[e.x + 1];
+        const uint64_t b = rb + (uint64_t)(p - e.y) * kPiece;
+        const uint64_t end = std::min<uint64_t>(re, b + kPiece);
+        const double s = block_sum<kPieceThreads>(row_part(A, x, b + threadIdx.x, end, kPieceThreads));
+        if (threadIdx.x == 0) w.piece[p] = s;
+    }
+}
+
+// Every long row from its pieces in order; for a step, (y - x)^2 per block into
+// partials[first + block].
+template <bool STEP>
+__global__ __launch_bounds__(kFixThreads) void csr_long_fix_kernel(CsrView A,
+                                                                   const double *__restrict__ x,
+                                                                   double shift, double *y,
+                                                                   LongWork w, uint32_t first) {
+    const uint32_t nrows = w.total[0], npieces = w.total[1];
+    double sq = 0.0;
+    for (uint32_t i = blockIdx.x * kFixThreads + threadIdx.x; i < nrows;
+         i += gridDim.x * kFixThreads) {
+        const uint2 e = w.list[i];
+        const uint32_t pend = i + 1 < nrows ? w.list[i + 1].y : npieces;
+        double s = 0.0;
+        for (uint32_t p = e.y; p < pend; ++p) s += w.piece[p];
+        const double v = s + shift;
+        y[e.x] = v;
+        if (STEP) {
+            const double d = v - x[e.x];
+            sq += d * d;
+        }
+    }
+    if (STEP) {
+        const double tot = block_sum<kFixThreads>(sq);
+        if (threadIdx.x == 0) w.partials[first + blockIdx.x] = tot;
+    }
 }
 
 // out[0] = sum of partials[0, count) in a fixed order.
@@ -253,15 +409,22 @@ uint32_t short_grid(uint64_t rows, uint32_t lanes) {
 }
 
 uint32_t long_grid(uint64_t rows) {
-    // at least one 1024-row chunk per block
     const uint64_t chunks = (rows + kLongThreads - 1) / kLongThreads;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, kLongGrid));
+}
+
+// Upper bounds of the long-row list and of the pieces of a matrix with this shape.
+void long_caps(uint64_t rows, uint64_t nnz, uint64_t &list_cap, uint64_t &piece_cap) {
+    const uint64_t lim = 32ull * lanes_for(rows, nnz);
+    list_cap = std::min<uint64_t>(rows, nnz / (lim + 1)) + 1;
+    piece_cap = nnz / kPiece + list_cap + 1;
 }
 
 CsrView view_of(const nkpr_csr *a) {
     NKPR_CHECK(a != nullptr, "matrix is NULL");
     NKPR_CHECK(a->rows == 0 || (a->row_idx && a->col_idx && a->values),
                "matrix arrays must not be NULL");
+    NKPR_CHECK(a->rows < 0xFFFFFFFFull, "more than 2^32 - 2 rows");
     CsrView v;
     v.rows = a->rows;
     v.row_idx = a->row_idx;
@@ -270,38 +433,43 @@ CsrView view_of(const nkpr_csr *a) {
     return v;
 }
 
-// The two product kernels (and, for a step, the final sum of their partials into *sq_out).
+// The product kernels (and, for a step, the final sum of their block sums into *sq_out).
 template <bool STEP>
-void launch_product(const CsrView &A, uint64_t nnz_hint, const double *x, double shift, double *y,
-                    double *partials, double *sq_out, hipStream_t s) {
+void launch_product(const CsrView &A, uint64_t nnz, const double *x, double shift, double *y,
+                    const LongWork &w, double *sq_out, hipStream_t s) {
     if (A.rows == 0) {
         if (STEP) hipLaunchKernelGGL(csr_fill_kernel, dim3(1), dim3(64), 0, s, sq_out, 1, 0.0);
         return;
     }
-    const uint32_t lanes = lanes_for(A.rows, nnz_hint);
+    const uint32_t lanes = lanes_for(A.rows, nnz);
     const uint32_t gs = short_grid(A.rows, lanes);
     const uint64_t tiles = (A.rows + kShortThreads / lanes - 1) / (kShortThreads / lanes);
     switch (lanes) {
-    case 4: hipLaunchKernelGGL((csr_short_kernel<4, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, partials, tiles); break;
-    case 8: hipLaunchKernelGGL((csr_short_kernel<8, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, partials, tiles); break;
-    case 16: hipLaunchKernelGGL((csr_short_kernel<16, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, partials, tiles); break;
-    case 32: hipLaunchKernelGGL((csr_short_kernel<32, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, partials, tiles); break;
-    default: hipLaunchKernelGGL((csr_short_kernel<64, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, partials, tiles); break;
+    case 4: hipLaunchKernelGGL((csr_short_kernel<4, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, w.partials, tiles); break;
+    case 8: hipLaunchKernelGGL((csr_short_kernel<8, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, w.partials, tiles); break;
+    case 16: hipLaunchKernelGGL((csr_short_kernel<16, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, w.partials, tiles); break;
+    case 32: hipLaunchKernelGGL((csr_short_kernel<32, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, w.partials, tiles); break;
+    default: hipLaunchKernelGGL((csr_short_kernel<64, STEP>), dim3(gs), dim3(kShortThreads), 0, s, A, x, shift, y, w.partials, tiles); break;
     }
     NKPR_HIP(hipGetLastError());
+    const uint64_t lim = 32ull * lanes;
     const uint32_t gl = long_grid(A.rows);
     const uint64_t rpb = (A.rows + gl - 1) / gl;
-    hipLaunchKernelGGL((csr_long_kernel<STEP>), dim3(gl), dim3(kLongThreads), 0, s, A, x, shift,
-                       y, STEP ? partials + gs : nullptr, rpb);
+    hipLaunchKernelGGL(csr_long_scan_kernel, dim3(gl), dim3(kLongThreads), 0, s, A, rpb, lim, w);
+    hipLaunchKernelGGL(csr_long_offsets_kernel, dim3(1), dim3(64), 0, s, w, gl);
+    hipLaunchKernelGGL(csr_long_list_kernel, dim3(gl), dim3(kLongThreads), 0, s, A, rpb, lim, w);
+    hipLaunchKernelGGL(csr_piece_kernel, dim3(kPieceGrid), dim3(kPieceThreads), 0, s, A, x, w);
+    hipLaunchKernelGGL((csr_long_fix_kernel<STEP>), dim3(kFixGrid), dim3(kFixThreads), 0, s, A, x,
+                       shift, y, w, gs);
     NKPR_HIP(hipGetLastError());
     if (STEP) {
-        hipLaunchKernelGGL(csr_final_kernel, dim3(1), dim3(kFinalThreads), 0, s, partials,
-                           gs + gl, sq_out);
+        hipLaunchKernelGGL(csr_final_kernel, dim3(1), dim3(kFinalThreads), 0, s, w.partials,
+                           gs + kFixGrid, sq_out);
         NKPR_HIP(hipGetLastError());
     }
 }
 
-// Device scratch of the step's partials, taken from the stream's memory pool for one call.
+// Device scratch of one call, taken from the stream's memory pool.
 struct PoolBuf {
     void *p = nullptr;
     hipStream_t s = nullptr;
@@ -317,42 +485,12 @@ struct PoolBuf {
     }
 };
 
-}  // namespace
-
-uint32_t csr_step_partials(uint64_t rows, uint64_t nnz) {
-    return rows ? short_grid(rows, lanes_for(rows, nnz)) + long_grid(rows) : 0;
-}
-
-void csr_spmv_impl(const nkpr_csr *a, const double *x, double *y, hipStream_t s) {
-    const CsrView A = view_of(a);
-    NKPR_CHECK(A.rows == 0 || (x && y), "x and y must not be NULL");
-    NKPR_CHECK(x != y || A.rows == 0, "x and y must be different vectors");
-    launch_product<false>(A, a->nnz, x, 0.0, y, nullptr, nullptr, s);
-}
-
-void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, double *step_sq,
-                   double *scratch, uint64_t scratch_count, hipStream_t s) {
-    const CsrView A = view_of(m);
-    NKPR_CHECK(m->rows == m->cols, "a PageRank step needs a square matrix");
-    NKPR_CHECK(step_sq != nullptr, "step_sq must not be NULL");
-    NKPR_CHECK(A.rows == 0 || (x && y), "x and y must not be NULL");
-    NKPR_CHECK(x != y || A.rows == 0, "x and y must be different vectors");
-    const uint32_t need = csr_step_partials(A.rows, m->nnz);
-    if (scratch) {
-        NKPR_CHECK(need <= scratch_count, "step scratch too small");
-        launch_product<true>(A, m->nnz, x, shift, y, scratch, step_sq, s);
-    } else {
-        PoolBuf buf(8 * (size_t)std::max<uint32_t>(need, 1), s);
-        launch_product<true>(A, m->nnz, x, shift, y, static_cast<double *>(buf.p), step_sq, s);
-    }
-}
-
 // The address of `scratch` on the current device, taken once per device (each GPU holds its
 // own copy of the array at its own address).
-double *csr_noalloc_scratch() {
+void *csr_noalloc_scratch() {
     constexpr int kMaxDevices = 64;
     static std::mutex mu;
-    static double *addr[kMaxDevices] = {};
+    static void *addr[kMaxDevices] = {};
     int dev = -1;
     NKPR_HIP(hipGetDevice(&dev));
     NKPR_CHECK(dev >= 0 && dev < kMaxDevices, "device index out of range");
@@ -361,19 +499,55 @@ double *csr_noalloc_scratch() {
         void *p = nullptr;
         NKPR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
         NKPR_CHECK(p != nullptr, "hipGetSymbolAddress returned NULL");
-        addr[dev] = static_cast<double *>(p);
+        addr[dev] = p;
     }
     return addr[dev];
 }
 
+}  // namespace
+
+void csr_spmv_impl(const nkpr_csr *a, const double *x, double *y, hipStream_t s) {
+    const CsrView A = view_of(a);
+    NKPR_CHECK(A.rows == 0 || (x && y), "x and y must not be NULL");
+    NKPR_CHECK(x != y || A.rows == 0, "x and y must be different vectors");
+    if (A.rows == 0) return;
+    uint64_t lc, pc;
+    long_caps(A.rows, a->nnz, lc, pc);
+    PoolBuf buf(work_bytes(lc, pc), s);
+    launch_product<false>(A, a->nnz, x, 0.0, y, carve(buf.p, lc, pc), nullptr, s);
+}
+
+void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, double *step_sq,
+                   hipStream_t s) {
+    const CsrView A = view_of(m);
+    NKPR_CHECK(m->rows == m->cols, "a PageRank step needs a square matrix");
+    NKPR_CHECK(step_sq != nullptr, "step_sq must not be NULL");
+    NKPR_CHECK(A.rows == 0 || (x && y), "x and y must not be NULL");
+    NKPR_CHECK(x != y || A.rows == 0, "x and y must be different vectors");
+    uint64_t lc, pc;
+    long_caps(A.rows, m->nnz, lc, pc);
+    PoolBuf buf(work_bytes(lc, pc), s);
+    launch_product<true>(A, m->nnz, x, shift, y, carve(buf.p, lc, pc), step_sq, s);
+}
+
 void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, double *y,
                            double *step_sq, hipStream_t s) {
-    constexpr uint64_t kScratch = sizeof(scratch) / sizeof(double);
-    static_assert(kScratch >= kShortGrid + kLongGrid, "the step's grids fit the scratch");
-    NKPR_CHECK(m != nullptr, "matrix is NULL");
-    NKPR_CHECK(csr_step_partials(m->rows, m->nnz) <= kScratch,
-               "the step needs more partials than the 4096 the scratch holds");
-    csr_step_impl(m, shift, x, y, step_sq, csr_noalloc_scratch(), kScratch, s);
+    static_assert(sizeof(scratch) >= 8 * (size_t)kNoallocList + 8 * (size_t)kNoallocPieces +
+                                          4 * (4 * (size_t)kLongGrid + 4) +
+                                          8 * (size_t)(kShortGrid + kFixGrid) + 256,
+                  "the scratch holds the capped work");
+    const CsrView A = view_of(m);
+    NKPR_CHECK(m->rows == m->cols, "a PageRank step needs a square matrix");
+    NKPR_CHECK(step_sq != nullptr, "step_sq must not be NULL");
+    NKPR_CHECK(A.rows == 0 || (x && y), "x and y must not be NULL");
+    NKPR_CHECK(x != y || A.rows == 0, "x and y must be different vectors");
+    uint64_t lc, pc;
+    long_caps(A.rows, m->nnz, lc, pc);
+    NKPR_CHECK(lc <= kNoallocList && pc <= kNoallocPieces,
+               "matrix too large for the allocation-free step (at most 2^19 long rows and 2^20 "
+               "pieces of 4096 entries: about 1.07e9 entries)");
+    launch_product<true>(A, m->nnz, x, shift, y, carve(csr_noalloc_scratch(), kNoallocList,
+                                                       kNoallocPieces), step_sq, s);
 }
 
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
@@ -386,12 +560,13 @@ void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_
     NKPR_CHECK(tol >= 0.0, "tol must be >= 0");
     NKPR_CHECK(max_iterations >= 1, "max_iterations must be >= 1");
     const uint64_t n = A.rows, nnz = m->nnz;
-    const uint32_t np = csr_step_partials(n, nnz);
+    uint64_t lc, pc;
+    long_caps(n, nnz, lc, pc);
     // buffers first: the stream owner below is destroyed (and synchronised) before them
-    DevBuf xa, xb, part, out, flag;
+    DevBuf xa, xb, work, out, flag;
     xa.alloc(8 * n);
     xb.alloc(8 * n);
-    part.alloc(8 * (size_t)std::max<uint32_t>(np, 1024));
+    work.alloc(std::max<size_t>(work_bytes(lc, pc), 8 * 1024));
     out.alloc(16);
     flag.alloc(4);
     hipStream_t s;
@@ -414,6 +589,7 @@ void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_
     read_back(s, {{&bad, flag.p, 4}, {&last, A.row_idx + n, 8}});
     NKPR_CHECK(bad == 0 && last == nnz,
                "malformed CSR matrix (row_idx must rise from 0 to nnz, column indices < cols)");
+    const LongWork w = carve(work.p, lc, pc);
     const double tele = (1.0 - damp) / (double)n;
     hipLaunchKernelGGL(csr_fill_kernel, dim3(1024), dim3(256), 0, s, xa.as<double>(), n,
                        1.0 / (double)n);
@@ -422,15 +598,14 @@ void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_
     uint64_t it = 0;
     double sq = 0.0;
     do {
-        launch_product<true>(A, nnz, cur, tele, nxt, part.as<double>(), out.as<double>(), s);
+        launch_product<true>(A, nnz, cur, tele, nxt, w, out.as<double>(), s);
         read_back(s, {{&sq, out.p, 8}});
         std::swap(cur, nxt);
         ++it;
     } while (std::sqrt(sq) > tol && it < max_iterations);
-    hipLaunchKernelGGL(csr_sum_kernel, dim3(1024), dim3(kFinalThreads), 0, s, cur, n,
-                       part.as<double>());
-    hipLaunchKernelGGL(csr_final_kernel, dim3(1), dim3(kFinalThreads), 0, s, part.as<double>(),
-                       1024u, out.as<double>());
+    hipLaunchKernelGGL(csr_sum_kernel, dim3(1024), dim3(kFinalThreads), 0, s, cur, n, w.partials);
+    hipLaunchKernelGGL(csr_final_kernel, dim3(1), dim3(kFinalThreads), 0, s, w.partials, 1024u,
+                       out.as<double>());
     hipLaunchKernelGGL(csr_scale_kernel, dim3(1024), dim3(256), 0, s, cur, n, out.as<double>());
     NKPR_HIP(hipGetLastError());
     if (scores_on_device) {

@@ -332,12 +332,11 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
 // nkpr_power_run (nkpr_power.hip): eigenvector and Katz centrality
 void power_run_impl(nkpr_graph *g, const nkpr_power_params *p, double *scores, nkpr_stats *st);
 
-// CSR products and AlgebraicPageRank (nkpr_csr.hip).  csr_step_impl takes its partials from
-// `scratch` (scratch_count doubles) when that is not NULL, else from the stream's pool.
-uint32_t csr_step_partials(uint64_t rows, uint64_t nnz);
+// CSR products and AlgebraicPageRank (nkpr_csr.hip).  The plain product and the step take
+// their scratch from the stream's memory pool, the allocation-free step from a static array.
 void csr_spmv_impl(const nkpr_csr *a, const double *x, double *y, hipStream_t s);
 void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, double *step_sq,
-                   double *scratch, uint64_t scratch_count, hipStream_t s);
+                   hipStream_t s);
 void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, double *y,
                            double *step_sq, hipStream_t s);
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,

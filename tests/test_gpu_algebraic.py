@@ -2,8 +2,9 @@
 (AlgebraicPageRank.hpp:35-119, CSRGeneralMatrix.hpp:151-182, 563-578).
 
   nkpr_csr_spmv: row length mixes that select every lane width (4..64 lanes per row) and the
-      long-row kernel (rows longer than 2048 entries), with empty rows and duplicate entries,
-      against oracle.csr_spmv (sums in another order: 1e-13 relative to the sum of |terms|);
+      long-row path (rows longer than 32 x the lanes per row, cut into pieces of 4096
+      entries), with empty rows and duplicate entries, against oracle.csr_spmv (sums in
+      another order: 1e-13 relative to the sum of |terms|);
   nkpr_csr_pagerank_step: y and ||y - x||^2 against numpy;
   nkpr_algebraic_pagerank (C ABI and the C++/Cython drop-in): the oracle's iteration count and
       L1 <= 1e-12 on the reference's known-answer graphs, graphs with deleted slots and
@@ -165,3 +166,22 @@ def test_algebraic_pagerank_rejects_malformed_matrix():
     d = capi.DeviceCSR(100, 100, a.row_idx, col, a.values)
     with pytest.raises(capi.NkprError, match="malformed"):
         d.algebraic_pagerank()
+
+
+def test_csr_spmv_many_long_rows_and_pieces():
+    """Power-law rows: thousands of rows above the short-row limit spread over many scan
+    blocks, rows of several pieces (4096 entries each), against the oracle product; the step on
+    the same matrix is bit-identical on a second call."""
+    import torch
+    rng = np.random.default_rng(31)
+    rows = 400_000
+    lengths = np.minimum((rng.pareto(1.2, rows) * 3).astype(np.int64), 40_000)
+    a = random_csr(rows, rows, lengths, 32)
+    d, x, got = check_spmv(a, seed=33)
+    lens = np.diff(a.row_idx.astype(np.int64))
+    assert (lens > 32 * 16).sum() > 500 and (lens > 4096).sum() > 20
+    tx = torch.from_numpy(x).cuda()
+    y1, s1 = d.pagerank_step(tx, 0.1)
+    y2, s2 = d.pagerank_step(tx, 0.1)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2) and float(s1.item()) == float(s2.item())

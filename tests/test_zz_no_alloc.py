@@ -59,3 +59,21 @@ def test_noalloc_step_rejects_bad_arguments():
     sq = torch.empty(1, dtype=torch.float64, device="cuda")
     with pytest.raises(capi.NkprError, match="different"):
         d.pagerank_step_noalloc(x, 0.0, x, sq)
+
+
+def test_noalloc_step_rejects_matrices_beyond_its_scratch():
+    """A matrix that could need more long-row pieces than the scratch holds is refused before
+    any kernel runs (the struct claims 2^33 entries; no array is read)."""
+    import ctypes
+    import torch
+    a = matrix(100, 4, 0, 6)
+    d = capi.DeviceCSR(a.rows, a.cols, a.row_idx, a.col_idx, a.values)
+    m = d.struct()
+    m.nnz = 1 << 33
+    x = torch.ones(a.rows, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    sq = torch.empty(1, dtype=torch.float64, device="cuda")
+    rc = capi.lib().nkpr_csr_pagerank_step_noalloc(ctypes.byref(m), 0.0, x.data_ptr(),
+                                                   y.data_ptr(), sq.data_ptr(), None)
+    assert rc == capi.NKPR_ERR_INVALID
+    assert b"too large" in capi.lib().nkpr_last_error()
