@@ -300,12 +300,15 @@ int nkpr_csr_pagerank_step(const nkpr_csr *m, double shift, const double *x, dou
 
 /* No counterpart in the reference: nkpr_csr_pagerank_step for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
- * hipMalloc).  Same arguments and results; its scratch lives in a 12.1 MiB device array of the
+ * hipMalloc).  Same arguments and results; its scratch lives in a 48.1 MiB device array of the
  * library (held by every GPU the library runs on until the process ends) instead of the
  * stream's pool, and it calls no allocating routine.  Every call on one GPU shares that
  * array: calls that overlap on one GPU (other streams, other threads) race and must be
- * ordered by the caller.  A matrix with more than 2^19 long rows or 2^20 pieces of 4096
- * entries (beyond about 1.07e9 entries) is rejected with NKPR_ERR_INVALID before any launch. */
+ * ordered by the caller.  A matrix that could have more than 2^21 rows longer than 32 x the
+ * lanes per row (at most nnz / (32 lanes + 1) of them; 4 lanes up to an average row length of
+ * 6, 8 up to 12, 16 up to 24, 32 up to 48, else 64) or 2^22 pieces of 4096 entries is rejected
+ * with NKPR_ERR_INVALID before any launch: up to about 2.7e8 entries are accepted at 4 lanes,
+ * 1.07e9 at 16 (R-MAT scale 26, 1.06e9 entries, fits) and 4.3e9 at 64. */
 int nkpr_csr_pagerank_step_noalloc(const nkpr_csr *m, double shift, const double *x, double *y,
                                    double *step_sq, void *stream);
 
@@ -356,6 +359,35 @@ void nkpr_nkbg_free(nkpr_nkbg *g);
  * weight_format -1 detects it from the weights as the reference's autoDetect does.  Unlike
  * the reference's writer, deleted slots keep their ids (flag bit 0 clear). */
 int nkpr_nkbg_write(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32_t weight_format);
+
+/* ---- Approximate personalised PageRank (scd) ------------------------------------------------
+ * A handle over a graph's out-lists (Graph::forNeighborsOf order: directed, the out-edges;
+ * undirected, the adjacency with an edge at both endpoints and a self-loop once): ptr[z+1],
+ * dst[m] (int32), w[m] (FP64, NULL = unit weights), host (device_ptrs = 0) or device pointers;
+ * copied to the device with weightedDegree(u, countSelfLoopsTwice = true) per node. */
+typedef struct nkpr_apr nkpr_apr;
+typedef struct {
+    uint64_t rounds;   /* frontier rounds                                                    */
+    uint64_t pushes;   /* node pushes                                                        */
+    uint64_t touched;  /* nodes in the result                                                */
+    uint64_t edge_pushes; /* out-edges of the pushed nodes (one mass each)                    */
+    double ms_total;
+} nkpr_apr_stats;
+
+int nkpr_apr_create(uint64_t z, const uint64_t *ptr, const int32_t *dst, const double *w,
+                    int32_t device_ptrs, nkpr_apr **out);
+int nkpr_apr_destroy(nkpr_apr *a);
+/* Replaces ApproximatePageRank::run(seeds) (networkit/cpp/scd/ApproximatePageRank.cpp:18-61):
+ * residual pushes from the seeds (a set: repeats ignored, each with residual 1/|seeds|) until
+ * every residual is below eps * weightedDegree; alpha is the loop probability.  The pushes run
+ * a frontier at a time on the device (the reference runs a FIFO queue on one thread): the
+ * identity p + pr_alpha(r) = pr_alpha(seeds) and the stopping rule are the reference's, the
+ * values differ from its within that rule.  *count receives the number of touched nodes
+ * (every node that held residual, the reference's result set); ids_out (ascending), p_out and
+ * r_out (nullable, host, at least z entries) their ids, scores and final residuals. */
+int nkpr_apr_run(nkpr_apr *a, double alpha, double eps, const uint64_t *seeds, uint64_t nseeds,
+                 uint64_t *count, int64_t *ids_out, double *p_out, double *r_out,
+                 nkpr_apr_stats *stats);
 
 /* Centrality::ranking: ids of the existing nodes ordered by score descending, ties by id
  * ascending; ids_out and scores_out (nullable) receive *count = number of existing nodes

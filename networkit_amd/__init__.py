@@ -24,7 +24,7 @@ except ImportError as _e:  # pragma: no cover - exercised only when the build is
         "networkit_amd's native extension is not built (run __graft_entry__.build() or "
         "python -m networkit_amd._build): %s" % _e) from _e
 
-from . import algebraic, base, centrality, graph, graphio  # noqa: E402,F401
+from . import algebraic, base, centrality, graph, graphio, scd  # noqa: E402,F401
 from .graph import Graph  # noqa: E402,F401
 
-__all__ = ["Graph", "algebraic", "base", "centrality", "graph", "graphio"]
+__all__ = ["Graph", "algebraic", "base", "centrality", "graph", "graphio", "scd"]

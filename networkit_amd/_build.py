@@ -23,7 +23,7 @@ ROCM_LIB = "/opt/rocm/lib"
 
 HIP_SOURCES = ["nkpr_sweep.hip", "nkpr_graph.hip", "nkpr_krylov.hip", "nkpr_batch.hip",
                "nkpr_adjoint.hip", "nkpr_groups.hip", "nkpr_binned.hip", "nkpr_power.hip",
-               "nkpr_csr.hip", "nkpr_nkbg.hip", "nkpr_capi.hip"]
+               "nkpr_csr.hip", "nkpr_nkbg.hip", "nkpr_apr.hip", "nkpr_capi.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # packed FP32 instructions stay off (some of their forms are unreliable on this pool)
@@ -34,7 +34,7 @@ HIP_FLAGS = [
 LIB_NKPR = os.path.join(HERE, "libnkpr.so")
 LIB_NKGEN = os.path.join(HERE, "libnkgen.so")
 CXX_SOURCES = ["Graph.cpp", "PageRank.cpp", "EigenvectorCentrality.cpp", "AlgebraicPageRank.cpp",
-               "NetworkitBinaryIO.cpp"]
+               "NetworkitBinaryIO.cpp", "SelectiveCommunity.cpp"]
 
 
 def _run(cmd, cwd=None):

@@ -14,6 +14,8 @@ from libcpp cimport bool as bool_t
 from libcpp.vector cimport vector
 from libcpp.utility cimport pair
 from libcpp.string cimport string
+from libcpp.set cimport set as cset
+from cython.operator cimport dereference as deref, preincrement as inc
 
 import numpy as np
 cimport numpy as cnp
@@ -132,6 +134,14 @@ cdef extern from "NetworkitBinaryIO.hpp" namespace "NetworKit":
     cdef cppclass _NetworkitBinaryWriter "NetworKit::NetworkitBinaryWriter":
         _NetworkitBinaryWriter(uint64_t, _NetworkitBinaryWeights)
         void write(_Graph, string) except + nogil
+
+cdef extern from "SelectiveCommunity.hpp" namespace "NetworKit":
+    cdef cppclass _ApproximatePageRank "NetworKit::ApproximatePageRank":
+        _ApproximatePageRank(_Graph, double, double) except +
+        vector[pair[node, double]] run(cset[node]) except + nogil
+    cdef cppclass _PageRankNibble "NetworKit::PageRankNibble":
+        _PageRankNibble(_Graph, double, double) except +
+        cset[node] expandOneCommunity(cset[node]) except + nogil
 
 
 cdef class Graph:
@@ -471,3 +481,78 @@ def _write_nkbg(Graph G, path, uint64_t chunks=32, int weightsType=5):
             w.write(G._this[0], p)
     finally:
         del w
+
+
+cdef cset[node] _node_set(seeds):
+    cdef cset[node] out
+    cdef node x
+    if isinstance(seeds, int):
+        seeds = [seeds]
+    for s in seeds:
+        x = s
+        out.insert(x)
+    return out
+
+
+cdef class ApproximatePageRank:
+    """ApproximatePageRank(G, alpha, epsilon=1e-12)
+
+    Approximate personalised PageRank from seed nodes by residual pushes
+    (networkit.scd.ApproximatePageRank, scd.pyx:93-141; ApproximatePageRank.cpp:18-61), the
+    pushes running a frontier at a time on the MI355X.  run(seeds) returns the [(node, score)]
+    pairs of every node the pushes touched (ids ascending)."""
+    cdef _ApproximatePageRank* _this
+    cdef Graph _G
+
+    def __cinit__(self, Graph G not None, double alpha, double epsilon=1e-12):
+        self._G = G
+        self._this = new _ApproximatePageRank(G._this[0], alpha, epsilon)
+
+    def __dealloc__(self):
+        if self._this != NULL:
+            del self._this
+        self._this = NULL
+
+    def run(self, seeds):
+        cdef cset[node] sd = _node_set(seeds)
+        cdef vector[pair[node, double]] res
+        with nogil:
+            res = self._this.run(sd)
+        return [(p.first, p.second) for p in res]
+
+
+cdef class PageRankNibble:
+    """PageRankNibble(G, alpha, epsilon)
+
+    A community around seed nodes: the approximate personalised PageRank (pushes on the MI355X)
+    and its best sweep set by conductance (networkit.scd.PageRankNibble, scd.pyx:143-156;
+    PageRankNibble.cpp:28-96).  run(seeds) -> {seed: set of nodes}, one community per seed
+    (SelectiveCommunityDetector.cpp:15-23); expandOneCommunity(seeds) -> set of nodes."""
+    cdef _PageRankNibble* _this
+    cdef Graph _G
+
+    def __cinit__(self, Graph G not None, double alpha, double epsilon):
+        self._G = G
+        self._this = new _PageRankNibble(G._this[0], alpha, epsilon)
+
+    def __dealloc__(self):
+        if self._this != NULL:
+            del self._this
+        self._this = NULL
+
+    def expandOneCommunity(self, seeds):
+        cdef cset[node] sd = _node_set(seeds)
+        cdef cset[node] res
+        with nogil:
+            res = self._this.expandOneCommunity(sd)
+        cdef list out = []
+        cdef cset[node].iterator it = res.begin()
+        while it != res.end():
+            out.append(deref(it))
+            inc(it)
+        return set(out)
+
+    def run(self, seeds):
+        if isinstance(seeds, int):
+            seeds = [seeds]
+        return {s: self.expandOneCommunity([s]) for s in sorted(set(seeds))}

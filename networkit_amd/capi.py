@@ -31,7 +31,8 @@ EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch
             "nkpr_ppr_batch", "nkpr_ranking", "nkpr_last_error", "nkpr_abi_version",
             "nkpr_power_params_default", "nkpr_power_run", "nkpr_csr_spmv",
             "nkpr_csr_pagerank_step", "nkpr_algebraic_pagerank", "nkpr_nkbg_read",
-            "nkpr_nkbg_free", "nkpr_nkbg_write", "nkpr_csr_pagerank_step_noalloc"]
+            "nkpr_nkbg_free", "nkpr_nkbg_write", "nkpr_csr_pagerank_step_noalloc",
+            "nkpr_apr_create", "nkpr_apr_destroy", "nkpr_apr_run"]
 ABI_VERSION = 3
 
 
@@ -97,6 +98,12 @@ class Nkbg(ctypes.Structure):
                 ("out_weight", ctypes.c_void_p), ("exists", ctypes.c_void_p)]
 
 
+class AprStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_uint64), ("pushes", ctypes.c_uint64),
+                ("touched", ctypes.c_uint64), ("edge_pushes", ctypes.c_uint64),
+                ("ms_total", ctypes.c_double)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint64), ("sweeps_fp32", ctypes.c_uint64),
                 ("sweeps_fp64", ctypes.c_uint64), ("refinements", ctypes.c_uint64),
@@ -154,6 +161,11 @@ def lib():
     L.nkpr_nkbg_free.restype = None
     L.nkpr_nkbg_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(Nkbg), ctypes.c_uint64,
                                   ctypes.c_int32]
+    L.nkpr_apr_create.argtypes = [ctypes.c_uint64, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(vp)]
+    L.nkpr_apr_destroy.argtypes = [vp]
+    L.nkpr_apr_run.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_uint64,
+                               ctypes.POINTER(ctypes.c_uint64), vp, vp, vp,
+                               ctypes.POINTER(AprStats)]
     L.nkpr_last_error.restype = ctypes.c_char_p
     L.nkpr_abi_version.restype = ctypes.c_int
     for name in EXPORTED:
@@ -511,3 +523,44 @@ def write_nkbg(path, nodes, directed, out_ptr, out_dst, out_weight=None, in_ptr=
              exists=arr(exists, np.uint8))
     check(lib().nkpr_nkbg_write(os.fsencode(path), ctypes.byref(g), int(chunks),
                                 int(weight_format)))
+
+
+class DeviceAPR:
+    """nkpr_apr handle over out-lists (host numpy arrays or, with device=True, torch CUDA
+    tensors): ptr uint64[z+1], dst int32[m], w float64[m] or None."""
+
+    def __init__(self, z, ptr, dst, w=None, device=False):
+        if not device:
+            ptr = np.ascontiguousarray(ptr, dtype=np.uint64)
+            dst = np.ascontiguousarray(dst, dtype=np.int32)
+            w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().nkpr_apr_create(int(z), _addr(ptr), _addr(dst), _addr(w),
+                                    1 if device else 0, ctypes.byref(h)))
+        self.handle = h
+        self.z = int(z)
+
+    def run(self, seeds, alpha, eps=1e-12):
+        """Returns (ids, p, r, stats): the touched nodes ascending, their scores and residuals."""
+        sd = np.ascontiguousarray(np.atleast_1d(np.asarray(seeds)), dtype=np.uint64)
+        ids = np.zeros(self.z, dtype=np.int64)
+        p = np.zeros(self.z, dtype=np.float64)
+        r = np.zeros(self.z, dtype=np.float64)
+        k = ctypes.c_uint64(0)
+        st = AprStats()
+        check(lib().nkpr_apr_run(self.handle, float(alpha), float(eps), _addr(sd), len(sd),
+                                 ctypes.byref(k), _addr(ids), _addr(p), _addr(r),
+                                 ctypes.byref(st)))
+        n = k.value
+        return ids[:n], p[:n], r[:n], {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().nkpr_apr_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

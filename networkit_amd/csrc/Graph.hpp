@@ -60,6 +60,22 @@ public:
             if (exists[v]) handle(v);
     }
 
+    // Graph::forNeighborsOf (Graph.hpp:1906-1915 with every edge used): u's out-list (the
+    // adjacency when undirected) in storage order.  handle(v, weight).
+    template <typename L> void forNeighborsOf(node u, L handle) const {
+        const auto &lst = outEdges[checked(u)];
+        for (size_t i = 0; i < lst.size(); ++i)
+            handle(lst[i], weighted ? outEdgeWeights[u][i] : defaultEdgeWeight);
+    }
+
+    // Graph::totalEdgeWeight (Graph.cpp:860-868): the sum over forEdges, or m unweighted.
+    edgeweight totalEdgeWeight() const noexcept {
+        if (!weighted) return static_cast<edgeweight>(m) * defaultEdgeWeight;
+        edgeweight sum = 0.0;
+        forEdges([&](node, node, edgeweight w) { sum += w; });
+        return sum;
+    }
+
     // Graph::forEdges (Graph.hpp:1941-1945, 1895-1915): u ascending, each out-list in
     // storage order; an undirected edge is visited once, at its endpoint u >= v.
     // handle(u, v, weight).

@@ -682,6 +682,29 @@ int nkpr_nkbg_write(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32
     return guarded([&] { nkbg_write_impl(path, g, chunks, weight_format); });
 }
 
+int nkpr_apr_create(uint64_t z, const uint64_t *ptr, const int32_t *dst, const double *w,
+                    int32_t device_ptrs, nkpr_apr **out) {
+    return guarded([&] { apr_create_impl(z, ptr, dst, w, device_ptrs, out); });
+}
+
+int nkpr_apr_destroy(nkpr_apr *a) {
+    return guarded([&] {
+        if (!a) return;
+        DeviceGuard dg(apr_device(a));
+        apr_destroy_impl(a);
+    });
+}
+
+int nkpr_apr_run(nkpr_apr *a, double alpha, double eps, const uint64_t *seeds, uint64_t nseeds,
+                 uint64_t *count, int64_t *ids_out, double *p_out, double *r_out,
+                 nkpr_apr_stats *stats) {
+    return guarded([&] {
+        NKPR_CHECK(a != nullptr, "handle is NULL");
+        DeviceGuard dg(apr_device(a));
+        apr_run_impl(a, alpha, eps, seeds, nseeds, count, ids_out, p_out, r_out, stats);
+    });
+}
+
 int nkpr_ranking(uint64_t z, const double *scores, const uint8_t *exists, int64_t *ids_out,
                  double *scores_out, uint64_t *count) {
     return guarded([&] {

@@ -46,12 +46,12 @@
 
 #include "nkpr_internal.hpp"
 
-// Scratch of nkpr_csr_pagerank_step_noalloc (12.1 MiB): the long-row list (2^19 rows), the
-// piece sums (2^20), the block counts and offsets, and the step's block sums.  Every GPU the
+// Scratch of nkpr_csr_pagerank_step_noalloc (48.1 MiB): the long-row list (2^21 rows), the
+// piece sums (2^22), the block counts and offsets, and the step's block sums.  Every GPU the
 // library runs on holds it until the process ends.  Host code never uses this name as an
 // address: it takes the array's address per device with hipGetSymbolAddress
 // (csr_noalloc_scratch).
-static __device__ __align__(256) unsigned char scratch[(1u << 19) * 8 + (1u << 20) * 8 + 65536];
+static __device__ __align__(256) unsigned char scratch[(1u << 21) * 8 + (1u << 22) * 8 + 65536];
 
 namespace nkpr {
 namespace {
@@ -66,7 +66,7 @@ constexpr uint32_t kPieceGrid = 4096;
 constexpr int kFixThreads = 256;
 constexpr uint32_t kFixGrid = 256;
 constexpr int kFinalThreads = 256;
-constexpr uint32_t kNoallocList = 1u << 19, kNoallocPieces = 1u << 20;
+constexpr uint32_t kNoallocList = 1u << 21, kNoallocPieces = 1u << 22;
 
 struct CsrView {
     uint64_t rows;
@@ -544,8 +544,10 @@ void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, dou
     uint64_t lc, pc;
     long_caps(A.rows, m->nnz, lc, pc);
     NKPR_CHECK(lc <= kNoallocList && pc <= kNoallocPieces,
-               "matrix too large for the allocation-free step (at most 2^19 long rows and 2^20 "
-               "pieces of 4096 entries: about 1.07e9 entries)");
+               "matrix too large for the allocation-free step: it could have more than 2^21 rows "
+               "longer than 32 x the lanes per row, or 2^22 pieces of 4096 entries (at most "
+               "nnz / (32 lanes + 1) such rows: up to about 2.7e8 entries at an average row "
+               "length of 6 or less, 1.07e9 at 24 or less, 4.3e9 above 48)");
     launch_product<true>(A, m->nnz, x, shift, y, carve(csr_noalloc_scratch(), kNoallocList,
                                                        kNoallocPieces), step_sq, s);
 }

@@ -342,6 +342,15 @@ void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, dou
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                              double *scores, int32_t scores_on_device, uint64_t *iterations);
 
+// Approximate personalised PageRank (nkpr_apr.hip)
+void apr_create_impl(uint64_t z, const uint64_t *ptr, const int32_t *dst, const double *w,
+                     int32_t device_ptrs, nkpr_apr **out);
+void apr_run_impl(nkpr_apr *a, double alpha, double eps, const uint64_t *seeds, uint64_t nseeds,
+                  uint64_t *count, int64_t *ids_out, double *p_out, double *r_out,
+                  nkpr_apr_stats *st);
+void apr_destroy_impl(nkpr_apr *a);
+int apr_device(const nkpr_apr *a);
+
 // NetworKit binary graph files (nkpr_nkbg.hip)
 void nkbg_read_impl(const char *path, int32_t with_out_lists, nkpr_nkbg *out);
 void nkbg_free_impl(nkpr_nkbg *g);

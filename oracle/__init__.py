@@ -7,7 +7,8 @@ host cores).  The product (``networkit_amd``) never imports it and has no CPU fa
 The arithmetic lives in ``pagerank_oracle.c`` (a restatement of
 networkit/cpp/centrality/PageRank.cpp:20-104), ``power_oracle.c`` (EigenvectorCentrality.cpp:
 21-66, KatzCentrality.cpp:28-76) and ``algebraic_oracle.c`` (AlgebraicPageRank.hpp:35-119,
-CSRGeneralMatrix.hpp:151-182,563-578); this module loads it with ctypes and adds the
+CSRGeneralMatrix.hpp:151-182,563-578) and ``scd_oracle.c`` (ApproximatePageRank.cpp:18-61,
+PageRankNibble.cpp:28-89); this module loads it with ctypes and adds the
 NetworKit-semantics edge-list -> in-list conversion (Graph.cpp:549-597, Graph.hpp:1918-1938)
 that the C code consumes.
 
@@ -29,6 +30,9 @@ Parity status of each function is pinned in tests/test_oracle.py:
   algebraic_pagerank  reference known answers (AlgebraicPageRankGTest.cpp:21-94), sparse direct
                   solve of (I - M) x = (1-d)/n 1, equality with pagerank() on graphs without
                   deleted nodes
+  approximate_pagerank  the ACL identity p + pr(r) = pr(seeds) against a sparse direct solve
+                  of the lazy personalised PageRank, the stopping invariant, mass conservation
+  best_sweep_set  brute force over every prefix with set-based cut and volume
 """
 import ctypes
 import os
@@ -90,6 +94,11 @@ def _load():
     lib.oracle_algebraic_pagerank.argtypes = [u64, p, p, p, ctypes.c_double, ctypes.c_double,
                                               u64, p, ctypes.POINTER(u64)]
     lib.oracle_algebraic_pagerank.restype = ctypes.c_int
+    lib.oracle_apr.argtypes = [u64, p, p, p, ctypes.c_double, ctypes.c_double, p, u64, p, p, p]
+    lib.oracle_apr.restype = ctypes.c_int64
+    lib.oracle_sweep.argtypes = [u64, p, p, p, u64, ctypes.c_double, u64, p, p, p,
+                                 ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_sweep.restype = u64
     lib.oracle_num_threads.argtypes = []
     lib.oracle_num_threads.restype = ctypes.c_int
     _lib = lib
@@ -318,6 +327,74 @@ def algebraic_pagerank(mat, damp=0.85, tol=1e-8, max_iterations=2**64 - 1):
     if rc != 0:
         raise RuntimeError("oracle_algebraic_pagerank failed")
     return scores, int(it.value)
+
+
+def out_lists(z, u, v, w=None, directed=True):
+    """NetworKit's out-lists (Graph::forNeighborsOf order) of the Graph built by
+    ``addEdge(u[i], v[i], w[i])`` in order: directed, u's out-edges; undirected, u's
+    adjacency (an edge at both endpoints, a self-loop once), insertion order.  Returns
+    (ptr, dst, w)."""
+    u = np.asarray(u, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64)
+    wt = None if w is None else np.asarray(w, dtype=np.float64)
+    if directed:
+        rows, cols, seq, ws = u, v, np.arange(len(u), dtype=np.int64), wt
+    else:
+        loop = u == v
+        k = len(u)
+        rows = np.concatenate([u, v[~loop]])
+        cols = np.concatenate([v, u[~loop]])
+        seq = np.concatenate([2 * np.arange(k), 2 * np.arange(k)[~loop] + 1])
+        ws = None if wt is None else np.concatenate([wt, wt[~loop]])
+    order = np.lexsort((seq, rows))
+    ptr = np.zeros(z + 1, dtype=np.uint64)
+    ptr[1:] = np.cumsum(np.bincount(rows, minlength=z))
+    return (ptr, np.ascontiguousarray(cols[order], dtype=np.uint64),
+            None if ws is None else np.ascontiguousarray(ws[order]))
+
+
+def approximate_pagerank(z, ptr, dst, w, seeds, alpha, eps=1e-12):
+    """The restated ApproximatePageRank::run (ApproximatePageRank.cpp:18-61) on out-lists.
+    Returns (p, r, touched) as dense arrays of z entries."""
+    lib = _load()
+    seeds = np.unique(np.asarray(list(seeds), dtype=np.uint64))
+    p = np.zeros(z, dtype=np.float64)
+    r = np.zeros(z, dtype=np.float64)
+    touched = np.zeros(z, dtype=np.uint8)
+    ptr = np.ascontiguousarray(ptr, dtype=np.uint64)
+    dst = np.ascontiguousarray(dst, dtype=np.uint64)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    k = lib.oracle_apr(z, _ptr(ptr), _ptr(dst), _ptr(w), alpha, eps, _ptr(seeds), len(seeds),
+                       _ptr(p), _ptr(r), _ptr(touched))
+    if k < 0:
+        raise RuntimeError("oracle_apr failed")
+    return p, r, touched
+
+
+def weighted_degree2(z, ptr, dst, w):
+    """Graph::weightedDegree(u, countSelfLoopsTwice = true) from out-lists."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    rows = np.repeat(np.arange(z), np.diff(ptr))
+    wt = np.ones(len(dst)) if w is None else np.asarray(w, dtype=np.float64)
+    wt = np.where(np.asarray(dst, dtype=np.int64) == rows, 2.0 * wt, wt)
+    return np.bincount(rows, weights=wt, minlength=z)
+
+
+def best_sweep_set(z, ptr, dst, w, n, total_volume, ids, score):
+    """PageRankNibble::bestSweepSet (PageRankNibble.cpp:28-89) on nodes ``ids`` with scores
+    ``score`` (p / weightedDegree(u, true)).  Returns (order, best prefix length, best
+    conductance)."""
+    lib = _load()
+    ids = np.ascontiguousarray(ids, dtype=np.uint64)
+    score = np.ascontiguousarray(score, dtype=np.float64)
+    order = np.zeros(max(len(ids), 1), dtype=np.uint64)
+    best = ctypes.c_double(0.0)
+    ptr = np.ascontiguousarray(ptr, dtype=np.uint64)
+    dst = np.ascontiguousarray(dst, dtype=np.uint64)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    k = lib.oracle_sweep(z, _ptr(ptr), _ptr(dst), _ptr(w), int(n), float(total_volume),
+                         len(ids), _ptr(ids), _ptr(score), _ptr(order), ctypes.byref(best))
+    return order[:len(ids)].astype(np.int64), int(k), float(best.value)
 
 
 def num_threads():

@@ -31,6 +31,41 @@ def test_library_exports_every_declared_symbol():
         assert re.search(r"\bT %s\b" % name, out), name
 
 
+def test_struct_layouts_match_the_compiler():
+    """Every ctypes mirror has the size and field offsets the C compiler gives the struct of
+    nkpr.h (a small program built with gcc prints sizeof and offsetof)."""
+    import shutil
+    import subprocess
+    import tempfile
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"nkpr_csc": capi.Csc, "nkpr_graph_info": capi.GraphInfo,
+               "nkpr_params": capi.Params, "nkpr_stats": capi.Stats,
+               "nkpr_batch_stats": capi.BatchStats, "nkpr_power_params": capi.PowerParams,
+               "nkpr_csr": capi.CsrMatrix, "nkpr_nkbg": capi.Nkbg,
+               "nkpr_apr_stats": capi.AprStats}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", "#include <nkpr.h>", "int main(void) {"]
+    for c, py in mirrors.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (c, c))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (c, f, c, f))
+    lines += ["return 0;", "}"]
+    include = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        with open(src, "w") as fh:
+            fh.write("\n".join(lines))
+        subprocess.run([cc, "-std=c11", "-I", include, "-o", exe, src], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
+    for c, py in mirrors.items():
+        assert int(got[c]) == ctypes.sizeof(py), c
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (c, f)]) == getattr(py, f).offset, (c, f)
+
+
 def test_struct_sizes_match_header_layout():
     # sizes of the C structs (x86-64 SysV): keep ctypes mirrors in sync with nkpr.h
     assert ctypes.sizeof(capi.Csc) == 8 * 7 + 8

@@ -415,3 +415,117 @@ def test_algebraic_pagerank_fixed_point_and_pagerank(directed):
     assert np.abs(s8 - ref).sum() <= 1e-12
     # the cap stops the loop
     assert oracle.algebraic_pagerank(mat, max_iterations=3)[1] == 3
+
+
+# ---- ApproximatePageRank and PageRankNibble's sweep (ApproximatePageRank.cpp:18-61,
+#      PageRankNibble.cpp:28-85) ----
+
+def lazy_ppr(z, ptr, dst, w, s, alpha):
+    """pr_alpha(s): the solution of pr = alpha s + (1 - alpha) W^T pr with the lazy walk
+    W = (I + D^-1 A') / 2 (sparse direct solve), where A' is the adjacency without its
+    diagonal and D the weighted degree with self-loops counted twice: a push sends nothing
+    through a self-loop (the reference's r[u] assignment drops that share), and on graphs
+    without self-loops W is the usual lazy walk (I + D^-1 A) / 2."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    rows = np.repeat(np.arange(z), np.diff(np.asarray(ptr, np.int64)))
+    cols = np.asarray(dst, np.int64)
+    wt = np.ones(len(dst)) if w is None else np.asarray(w, np.float64)
+    off = rows != cols
+    A = sp.csr_matrix((wt[off], (rows[off], cols[off])), shape=(z, z))
+    d = np.bincount(rows, weights=np.where(off, wt, 2.0 * wt), minlength=z)
+    W = 0.5 * (sp.identity(z) + sp.diags(1.0 / d) @ A)
+    M = sp.identity(z, format="csc") - (1 - alpha) * W.T.tocsc()
+    return alpha * spl.spsolve(M, s)
+
+
+@pytest.mark.parametrize("self_loops", [False, True])
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("seeds", [[5], [3, 17, 40]])
+def test_apr_acl_identity_and_invariants(weighted, seeds, self_loops):
+    """p + pr_alpha(r) = pr_alpha(seeds) exactly (any push order keeps it), every residual
+    below eps vol, mass conserved, p >= 0, touched = support of p and r.  With self-loops the
+    reference drops the share a push sends through one: the identity holds for the walk
+    without them (and fails for the walk that keeps them), and mass leaks."""
+    z, u, v, w = random_graph(400, 2400, 9, directed=False, weighted=weighted,
+                              self_loops=self_loops)
+    if self_loops:                           # loops at the seeds and around them
+        extra = np.array([5, 3, 17, 40, 6, 7, 8, 9])
+        u, v = np.concatenate([u, extra]), np.concatenate([v, extra])
+        if w is not None:
+            w = np.concatenate([w, np.full(len(extra), 0.75)])
+    ptr, dst, ww = oracle.out_lists(z, u, v, w, directed=False)
+    vol = oracle.weighted_degree2(z, ptr, dst, ww)
+    assert (vol > 0).all()
+    alpha, eps = 0.1, 1e-7
+    p, r, touched = oracle.approximate_pagerank(z, ptr, dst, ww, seeds, alpha, eps)
+    s = np.zeros(z)
+    s[np.unique(seeds)] = 1.0 / len(np.unique(seeds))
+    np.testing.assert_allclose(p + lazy_ppr(z, ptr, dst, ww, r, alpha),
+                               lazy_ppr(z, ptr, dst, ww, s, alpha), rtol=0, atol=1e-12)
+    assert (r < eps * vol).all()
+    if self_loops:
+        assert p.sum() + r.sum() < 1.0 - 1e-4
+        # the walk that keeps the self-loop share (D^-1 A with A's diagonal) is not the one
+        kept = sp_lazy_keep(z, ptr, dst, ww, alpha)
+        assert np.abs(p + kept(r) - kept(s)).max() > 1e-6
+    else:
+        assert abs(p.sum() + r.sum() - 1.0) <= 1e-12
+    assert (p >= 0).all()
+    assert np.array_equal(touched.astype(bool), (p > 0) | (r > 0))
+
+
+def sp_lazy_keep(z, ptr, dst, w, alpha):
+    """pr_alpha for the lazy walk that keeps a self-loop's share: (I + D^-1 A) / 2, A with its
+    diagonal counted twice like D (a planted alternative the reference does not follow)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    rows = np.repeat(np.arange(z), np.diff(np.asarray(ptr, np.int64)))
+    cols = np.asarray(dst, np.int64)
+    wt = np.ones(len(dst)) if w is None else np.asarray(w, np.float64)
+    wt = np.where(rows == cols, 2.0 * wt, wt)
+    A = sp.csr_matrix((wt, (rows, cols)), shape=(z, z))
+    d = np.asarray(A.sum(axis=1)).ravel()
+    M = sp.identity(z, format="csc") - (1 - alpha) * (0.5 * (sp.identity(z) + sp.diags(1.0 / d) @ A)).T.tocsc()
+    return lambda x: alpha * spl.spsolve(M, x)
+
+
+def test_sweep_matches_brute_force():
+    z, u, v, w = random_graph(300, 1500, 10, directed=False, weighted=True, self_loops=True)
+    ptr, dst, ww = oracle.out_lists(z, u, v, w, directed=False)
+    rng = np.random.default_rng(2)
+    ids = rng.choice(z, 60, replace=False).astype(np.uint64)
+    score = rng.random(60)
+    score[5] = score[6]                      # a tie, ordered by id
+    total = 2 * float(np.asarray(w).sum())   # self-loops once, like totalEdgeWeight
+    order, best, cond = oracle.best_sweep_set(z, ptr, dst, ww, z, total, ids, score)
+    want = ids[np.lexsort((ids, -score))].astype(np.int64)
+    assert np.array_equal(order, want)
+    pi = np.asarray(ptr, np.int64)
+    conds = []
+    for L in range(1, len(order) + 1):
+        S = set(order[:L].tolist())
+        vol = cut = 0.0
+        for x in S:
+            for e in range(pi[x], pi[x + 1]):
+                n = int(dst[e])
+                vol += ww[e]
+                if n not in S or n == x:
+                    cut += ww[e]
+        conds.append(cut / min(vol, total - vol))
+    k = int(np.argmin(conds))
+    assert best == k + 1
+    assert abs(cond - conds[k]) <= 1e-9 * abs(conds[k])
+
+
+def test_apr_reference_inputs():
+    """test_scd.py:12-25 and SelectiveCDGTest.cpp:33-39: non-empty results with non-negative
+    scores of existing nodes, the same for a seed and the one-seed list."""
+    from nkbg_inputs import metis
+    for name, seed, alpha, eps in (("PGPgiantcompo.graph", 0, 0.1, 1e-12),
+                                   ("hep-th.graph", 50, 0.1, 1e-5)):
+        n, u, v = metis(name)
+        ptr, dst, ww = oracle.out_lists(n, u, v, None, directed=False)
+        p, r, touched = oracle.approximate_pagerank(n, ptr, dst, ww, [seed], alpha, eps)
+        assert touched.sum() >= 1 and (p >= 0).all() and p[seed] > 0
+        assert abs(p.sum() + r.sum() - 1.0) <= 1e-9

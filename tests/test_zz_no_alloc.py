@@ -63,13 +63,13 @@ def test_noalloc_step_rejects_bad_arguments():
 
 def test_noalloc_step_rejects_matrices_beyond_its_scratch():
     """A matrix that could need more long-row pieces than the scratch holds is refused before
-    any kernel runs (the struct claims 2^33 entries; no array is read)."""
+    any kernel runs (the struct claims 2^35 entries, more than 2^22 pieces; no array is read)."""
     import ctypes
     import torch
     a = matrix(100, 4, 0, 6)
     d = capi.DeviceCSR(a.rows, a.cols, a.row_idx, a.col_idx, a.values)
     m = d.struct()
-    m.nnz = 1 << 33
+    m.nnz = 1 << 35
     x = torch.ones(a.rows, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     sq = torch.empty(1, dtype=torch.float64, device="cuda")
