@@ -48,7 +48,8 @@ struct nkpr_apr {
     hipStream_t stream = nullptr;
     uint64_t z = 0, m = 0;
     nkpr::DevBuf ptr, dst, w, vol;
-    nkpr::DevBuf p, r, flags, f0, f1, taken, cnt, out_ids, out_p, out_r, out_cnt, seeds;
+    nkpr::DevBuf p, r, flags, f0, f1, taken, chunks, cnt, out_ids, out_p, out_r, out_cnt, seeds;
+    uint64_t chunk_cap = 0;  // entries per heavy chunk list (two lists)
     bool dirty = true;  // p, r, flags hold a run that was not cleared
     ~nkpr_apr() {
         if (stream) {
@@ -63,6 +64,7 @@ namespace {
 
 constexpr uint32_t kTouched = 1, kQueued = 2;
 constexpr int kPushThreads = 256;
+constexpr int kBatch = 8;  // rounds per host read of the counters
 constexpr double kUnit = 4611686018427387904.0;  // 2^62 fixed-point units per unit of mass
 
 // ceil(eps vol 2^62): r < thr  <=>  r 2^-62 < eps vol (the product rounded as the reference's).
@@ -121,7 +123,16 @@ __global__ void apr_reset_kernel(const uint32_t *ids, const uint32_t *count, dou
     }
 }
 
-// the seeds: residual 1/|seeds|, touched, queued, in the first frontier (ids ascending)
+// Counters (cnt, a 64-byte device buffer of uint32).  Round k pushes frontier k & 1 (size
+// cnt[k % 3]) and appends to the other frontier array (size cnt[(k + 1) % 3]); its take kernel
+// clears cnt[(k + 2) % 3], the size round k - 1 pushed, for round k + 1's appends.  The heavy
+// chunk lists alternate the same way (sizes cnt[kCntChunks + (k & 1)]).  [4..5] the out-edges
+// of the pushed nodes, [6..7] the pushes, [8] the rounds with a non-empty frontier.
+constexpr int kCntEdges = 4, kCntPushes = 6, kCntRounds = 8, kCntChunks = 10;
+// A node with more out-edges than kPushChunk is pushed by several waves, kPushChunk edges each.
+constexpr uint32_t kPushChunk = 256;
+
+// the seeds: residual 1/|seeds|, touched, queued, in frontier 0 (ids ascending)
 __global__ void apr_seed_kernel(const uint32_t *seeds, uint32_t k, uint64_t *r, uint32_t *flags,
                                 uint32_t *front, uint32_t *cnt) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
@@ -130,77 +141,125 @@ __global__ void apr_seed_kernel(const uint32_t *seeds, uint32_t k, uint64_t *r, 
         flags[s] = kTouched | kQueued;
         front[i] = s;
     }
+    if (blockIdx.x == 0 && threadIdx.x < 16) cnt[threadIdx.x] = threadIdx.x == 0 ? k : 0u;
+}
+
+// Appends the nodes of the lanes with `append` set at next[*ncnt ...], one counter update per
+// wave (every lane of the wave calls it).
+__device__ __forceinline__ void append_wave(bool append, uint32_t v, uint32_t *next,
+                                            uint32_t *ncnt) {
+    const uint64_t ball = __ballot(append);
+    if (!ball) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(ncnt, (uint32_t)__popcll(ball));
+    base = __shfl(base, 0, 64);
+    if (append) next[base + (uint32_t)__popcll(ball & ((1ull << lane) - 1))] = v;
+}
+
+// Round k, first kernel: a thread per frontier node u takes its residual res (kept in taken[]),
+// adds alpha res to p[u] and leaves (1 - alpha) res / 2 in r[u], appending u to the next
+// frontier when that alone reaches its threshold; a node with more than kPushChunk out-edges
+// lists its chunks for the push kernel.  The push kernel then only adds to residuals.
+__global__ void apr_take_kernel(const uint64_t *ptr, const double *vol, double alpha, double eps,
+                                const uint32_t *front, uint32_t *next, uint32_t *cnt, int k3,
+                                int k2, double *p, uint64_t *r, uint32_t *flags, uint64_t *taken,
+                                uint2 *chunks) {
+    __shared__ unsigned long long bedges;
+    const uint32_t nf = cnt[k3];
+    if (threadIdx.x == 0) bedges = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cnt[0] = k;
-        cnt[1] = 0u;
-        cnt[2] = 0u;  // cnt[2..3]: out-edges of the pushed nodes (64-bit)
-        cnt[3] = 0u;
+        cnt[(k3 + 2) % 3] = 0u;
+        cnt[kCntChunks + (k2 ^ 1)] = 0u;
+        if (nf) {
+            *reinterpret_cast<unsigned long long *>(cnt + kCntPushes) += nf;
+            cnt[kCntRounds] += 1u;
+        }
     }
-}
-
-__device__ __forceinline__ void enqueue(uint32_t v, uint32_t *flags, uint32_t *next,
-                                        uint32_t *ncnt) {
-    const uint32_t old = atomicOr(&flags[v], kQueued | kTouched);
-    if (!(old & kQueued)) next[atomicAdd(ncnt, 1u)] = v;
-}
-
-// Before a round: every frontier node's residual is taken (and its queued bit cleared), so
-// that the round pushes the residuals of its start.
-__global__ void apr_take_kernel(const uint32_t *front, const uint32_t *cnt, uint64_t *r,
-                                uint32_t *flags, uint64_t *taken) {
-    const uint32_t nf = cnt[0];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
-        const uint32_t u = front[i];
-        taken[i] = r[u];
-        r[u] = 0ull;
-        flags[u] &= ~kQueued;
+    __syncthreads();
+    uint32_t *ncnt = cnt + (k3 + 1) % 3;
+    uint32_t *hcnt = cnt + kCntChunks + k2;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // whole waves step together (append_wave is a wave operation)
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < nf; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool again = false;
+        uint32_t u = 0;
+        if (i < nf) {
+            u = front[i];
+            const uint64_t res = r[u];
+            taken[i] = res;
+            const double dres = (double)res;
+            p[u] += alpha * dres / kUnit;
+            const uint64_t keep = __double2ull_rn((1.0 - alpha) * dres / 2);
+            r[u] = keep;
+            again = keep >= threshold(eps, vol[u]);
+            flags[u] = again ? (kTouched | kQueued) : kTouched;
+            const uint64_t deg = ptr[u + 1] - ptr[u];
+            atomicAdd(&bedges, (unsigned long long)deg);
+            if (deg > kPushChunk) {
+                const uint32_t nc = (uint32_t)((deg + kPushChunk - 1) / kPushChunk);
+                const uint32_t at = atomicAdd(hcnt, nc);
+                for (uint32_t c = 0; c < nc; ++c) chunks[at + c] = make_uint2(i, c);
+            }
+        }
+        append_wave(again, u, next, ncnt);
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && bedges)
+        atomicAdd(reinterpret_cast<unsigned long long *>(cnt + kCntEdges), bedges);
 }
 
-// One round: a wave per frontier node.  cnt[0] = this frontier's size, cnt[1] the next one's,
-// cnt[2..3] the out-edges pushed so far (8-byte aligned: cnt is a 64-byte buffer).  At the
-// start of a round every residual outside the frontier lies below its threshold and every
-// frontier residual is 0, and all adds are positive, so the add that crosses a threshold is
-// the one that finds the residual below it and leaves it at or above it.
+// Round k, second kernel: a wave per frontier node with at most kPushChunk out-edges and a wave
+// per chunk of the others sends the masses (1 - alpha) res w / (2 vol).  At the start of the
+// kernel every residual lies below its threshold unless its node is already queued, and all
+// adds are positive, so the add that crosses a threshold is the one that finds the residual
+// below it and leaves it at or above it; that add sets the node's queued bit.
 __global__ __launch_bounds__(kPushThreads) void apr_push_kernel(
     const uint64_t *ptr, const int32_t *dst, const double *w, const double *vol, double alpha,
-    double eps, double *p, uint64_t *r, uint32_t *flags, const uint32_t *front,
-    const uint64_t *taken, uint32_t *next, uint32_t *cnt) {
-    const uint32_t nf = cnt[0];
+    double eps, uint64_t *r, uint32_t *flags, const uint32_t *front, const uint64_t *taken,
+    const uint2 *chunks, uint32_t *next, uint32_t *cnt, int k3, int k2) {
+    const uint32_t nf = cnt[k3], nh = cnt[kCntChunks + k2];
+    uint32_t *ncnt = cnt + (k3 + 1) % 3;
     const int lane = threadIdx.x & 63;
     const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < nf; i += nw) {
-        const uint32_t u = front[i];
-        const double res = (double)taken[i];  // in units
-        const double vu = vol[u];
-        const double share = vu > 0.0 ? (1.0 - alpha) * res / (2.0 * vu) : 0.0;
-        for (uint64_t e = ptr[u] + lane; e < ptr[u + 1]; e += 64) {
-            const uint32_t v = (uint32_t)dst[e];
-            if (v == u) continue;  // the reference overwrites r[u] after the loop: dropped
-            const uint64_t mass = __double2ull_rn(share * (w ? w[e] : 1.0));
-            const uint64_t thr = threshold(eps, vol[v]);
-            const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long *>(&r[v]),
-                                           (unsigned long long)mass);
-            if (old < thr && old + mass >= thr) enqueue(v, flags, next, cnt + 1);
-            else atomicOr(&flags[v], kTouched);
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < nf + nh; t += nw) {
+        uint32_t i;
+        uint64_t b, end;
+        if (t < nf) {
+            i = t;
+            const uint32_t u = front[i];
+            b = ptr[u];
+            end = ptr[u + 1];
+            if (end - b > kPushChunk) continue;  // pushed by its chunks
+        } else {
+            const uint2 c = chunks[t - nf];
+            i = c.x;
+            const uint32_t u = front[i];
+            b = ptr[u] + (uint64_t)c.y * kPushChunk;
+            end = std::min<uint64_t>(ptr[u + 1], b + kPushChunk);
         }
-        if (lane == 0) {
-            p[u] += alpha * res / kUnit;
-            const uint64_t keep = __double2ull_rn((1.0 - alpha) * res / 2);
-            const uint64_t thr = threshold(eps, vu);
-            const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long *>(&r[u]),
-                                           (unsigned long long)keep);
-            if (old < thr && old + keep >= thr) enqueue(u, flags, next, cnt + 1);
-            atomicAdd(reinterpret_cast<unsigned long long *>(cnt + 2),
-                      (unsigned long long)(ptr[u + 1] - ptr[u]));
+        const uint32_t u = front[i];
+        const double vu = vol[u];
+        const double share = vu > 0.0 ? (1.0 - alpha) * (double)taken[i] / (2.0 * vu) : 0.0;
+        for (uint64_t e0 = b; e0 < end; e0 += 64) {
+            const uint64_t e = e0 + lane;
+            bool append = false;
+            uint32_t v = 0;
+            // a self-loop sends nothing: the reference overwrites r[u] after its loop
+            if (e < end && (v = (uint32_t)dst[e]) != u) {
+                const uint64_t mass = __double2ull_rn(share * (w ? w[e] : 1.0));
+                const uint64_t thr = threshold(eps, vol[v]);
+                const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long *>(&r[v]),
+                                               (unsigned long long)mass);
+                if (old < thr && old + mass >= thr)
+                    append = !(atomicOr(&flags[v], kQueued | kTouched) & kQueued);
+                else if (!(flags[v] & kTouched))
+                    atomicOr(&flags[v], kTouched);
+            }
+            append_wave(append, v, next, ncnt);
         }
     }
-}
-
-// cnt[0] = cnt[1] (the next frontier becomes this one), cnt[1] = 0
-__global__ void apr_swap_kernel(uint32_t *cnt) {
-    cnt[0] = cnt[1];
-    cnt[1] = 0u;
 }
 
 // touched nodes in ascending id order: per-block counts, then a one-block scan, then writes
@@ -219,15 +278,29 @@ __global__ void apr_count_kernel(const uint32_t *flags, uint64_t z, uint32_t *bc
     if (threadIdx.x == 0) bc[blockIdx.x] = s[0];
 }
 
+// exclusive sums of the nb (= blockDim.x, a multiple of 64) block counts, one thread each
 __global__ void apr_scan_kernel(uint32_t *bc, uint32_t nb, uint32_t *total) {
-    if (threadIdx.x) return;
-    uint32_t acc = 0;
-    for (uint32_t i = 0; i < nb; ++i) {
-        const uint32_t c = bc[i];
-        bc[i] = acc;
-        acc += c;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t c = t < nb ? bc[t] : 0u;
+    uint32_t x = c;  // inclusive scan inside the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if ((int)lane >= o) x += y;
     }
-    total[0] = acc;
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (uint32_t i = 0; i < blockDim.x / 64; ++i) {
+            const uint32_t v = wsum[i];
+            wsum[i] = acc;
+            acc += v;
+        }
+        total[0] = acc;
+    }
+    __syncthreads();
+    if (t < nb) bc[t] = wsum[wv] + x - c;
 }
 
 __global__ void apr_compact_kernel(const uint32_t *flags, const double *p, const uint64_t *r,
@@ -297,6 +370,9 @@ void apr_create_impl(uint64_t z, const uint64_t *ptr, const int32_t *dst, const 
     a->f0.alloc(4 * z);
     a->f1.alloc(4 * z);
     a->taken.alloc(8 * z);
+    // chunks of the nodes with more than kPushChunk out-edges: at most 2 deg / kPushChunk per node
+    a->chunk_cap = 2 * m / kPushChunk + 64;
+    a->chunks.alloc(2 * 8 * a->chunk_cap);
     a->cnt.alloc(64);
     a->out_cnt.alloc(4 * 1025);
     a->out_ids.alloc(4 * z);
@@ -355,36 +431,40 @@ void apr_run_impl(nkpr_apr *a, double alpha, double eps, const uint64_t *seeds, 
                        (uint32_t)sd.size(), a->r.as<uint64_t>(), a->flags.as<uint32_t>(),
                        a->f0.as<uint32_t>(), a->cnt.as<uint32_t>());
     NKPR_HIP(hipGetLastError());
-    uint32_t *cur = a->f0.as<uint32_t>(), *nxt = a->f1.as<uint32_t>();
-    uint32_t nf = (uint32_t)sd.size();
-    uint64_t rounds = 0, pushes = 0;
+    // Rounds run in batches without a host read; a round whose frontier is empty returns at
+    // once, and the host reads the counters after each batch.
+    uint32_t *front[2] = {a->f0.as<uint32_t>(), a->f1.as<uint32_t>()};
     int cus = 0;
     NKPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, a->device));
-    while (nf > 0) {
-        const unsigned grid = (unsigned)std::max<uint64_t>(
-            1, std::min<uint64_t>(((uint64_t)nf * 64 + kPushThreads - 1) / kPushThreads,
-                                  (uint64_t)std::max(cus, 1) * 32));
-        const unsigned tgrid = (unsigned)std::min<uint64_t>(((uint64_t)nf + 255) / 256, 4096);
-        hipLaunchKernelGGL(apr_take_kernel, dim3(tgrid), dim3(256), 0, s, cur,
-                           a->cnt.as<uint32_t>(), a->r.as<uint64_t>(), a->flags.as<uint32_t>(),
-                           a->taken.as<uint64_t>());
-        hipLaunchKernelGGL(apr_push_kernel, dim3(grid), dim3(kPushThreads), 0, s,
-                           a->ptr.as<uint64_t>(), a->dst.as<int32_t>(), a->w.as<double>(),
-                           a->vol.as<double>(), alpha, eps, a->p.as<double>(),
-                           a->r.as<uint64_t>(), a->flags.as<uint32_t>(), cur,
-                           a->taken.as<uint64_t>(), nxt, a->cnt.as<uint32_t>());
-        hipLaunchKernelGGL(apr_swap_kernel, dim3(1), dim3(1), 0, s, a->cnt.as<uint32_t>());
+    const unsigned grid = (unsigned)std::max(cus, 1) * 8, tgrid = 128;
+    uint32_t host_cnt[16] = {0};
+    uint64_t k_round = 0;
+    for (;;) {
+        for (int b = 0; b < kBatch; ++b, ++k_round) {
+            const int k2 = (int)(k_round & 1), k3 = (int)(k_round % 3);
+            hipLaunchKernelGGL(apr_take_kernel, dim3(tgrid), dim3(256), 0, s,
+                               a->ptr.as<uint64_t>(), a->vol.as<double>(), alpha, eps,
+                               front[k2], front[k2 ^ 1], a->cnt.as<uint32_t>(), k3, k2,
+                               a->p.as<double>(), a->r.as<uint64_t>(), a->flags.as<uint32_t>(),
+                               a->taken.as<uint64_t>(), a->chunks.as<uint2>() + (k2 ? a->chunk_cap : 0));
+            hipLaunchKernelGGL(apr_push_kernel, dim3(grid), dim3(kPushThreads), 0, s,
+                               a->ptr.as<uint64_t>(), a->dst.as<int32_t>(), a->w.as<double>(),
+                               a->vol.as<double>(), alpha, eps, a->r.as<uint64_t>(),
+                               a->flags.as<uint32_t>(), front[k2], a->taken.as<uint64_t>(),
+                               a->chunks.as<uint2>() + (k2 ? a->chunk_cap : 0), front[k2 ^ 1],
+                               a->cnt.as<uint32_t>(), k3, k2);
+        }
         NKPR_HIP(hipGetLastError());
-        pushes += nf;
-        ++rounds;
-        read_back(s, {{&nf, a->cnt.p, 4}});
-        std::swap(cur, nxt);
+        read_back(s, {{host_cnt, a->cnt.p, sizeof(host_cnt)}});
+        if (host_cnt[k_round % 3] == 0) break;  // the frontier of the next round is empty
     }
+    uint64_t rounds = host_cnt[kCntRounds], pushes = 0;
+    std::memcpy(&pushes, host_cnt + kCntPushes, 8);
     // the touched nodes, ascending
     const unsigned nbk = 1024;
     hipLaunchKernelGGL(apr_count_kernel, dim3(nbk), dim3(256), 0, s, a->flags.as<uint32_t>(), z,
                        a->out_cnt.as<uint32_t>());
-    hipLaunchKernelGGL(apr_scan_kernel, dim3(1), dim3(64), 0, s, a->out_cnt.as<uint32_t>(), nbk,
+    hipLaunchKernelGGL(apr_scan_kernel, dim3(1), dim3(nbk), 0, s, a->out_cnt.as<uint32_t>(), nbk,
                        a->out_cnt.as<uint32_t>() + nbk);
     hipLaunchKernelGGL(apr_compact_kernel, dim3(nbk), dim3(256), 0, s, a->flags.as<uint32_t>(),
                        a->p.as<double>(), a->r.as<uint64_t>(), z, a->out_cnt.as<uint32_t>(),
@@ -407,7 +487,7 @@ void apr_run_impl(nkpr_apr *a, double alpha, double eps, const uint64_t *seeds, 
     }
     if (st) {
         uint64_t ep = 0;
-        read_back(s, {{&ep, a->cnt.as<uint32_t>() + 2, 8}});
+        std::memcpy(&ep, host_cnt + kCntEdges, 8);
         st->edge_pushes = ep;
         st->rounds = rounds;
         st->pushes = pushes;
