@@ -364,7 +364,9 @@ int nkpr_nkbg_write(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32
  * A handle over a graph's out-lists (Graph::forNeighborsOf order: directed, the out-edges;
  * undirected, the adjacency with an edge at both endpoints and a self-loop once): ptr[z+1],
  * dst[m] (int32), w[m] (FP64, NULL = unit weights), host (device_ptrs = 0) or device pointers;
- * copied to the device with weightedDegree(u, countSelfLoopsTwice = true) per node. */
+ * copied to the device with weightedDegree(u, countSelfLoopsTwice = true) per node.  Offsets
+ * that do not rise from 0, ids outside [0, z) and negative or NaN weights are NKPR_ERR_INVALID.
+ * The handle holds about 68 z + 12 m bytes of device memory. */
 typedef struct nkpr_apr nkpr_apr;
 typedef struct {
     uint64_t rounds;   /* frontier rounds                                                    */
@@ -382,7 +384,10 @@ int nkpr_apr_destroy(nkpr_apr *a);
  * every residual is below eps * weightedDegree; alpha is the loop probability.  The pushes run
  * a frontier at a time on the device (the reference runs a FIFO queue on one thread): the
  * identity p + pr_alpha(r) = pr_alpha(seeds) and the stopping rule are the reference's, the
- * values differ from its within that rule.  *count receives the number of touched nodes
+ * values differ from its within that rule.  As in the reference, the share a push sends
+ * through a self-loop is dropped and a node without neighbours is pushed while residual is
+ * left.  Residuals are summed exactly (64-bit fixed point, 2^-62 units), so a run repeats bit
+ * for bit; calls on one handle must not overlap.  *count receives the number of touched nodes
  * (every node that held residual, the reference's result set); ids_out (ascending), p_out and
  * r_out (nullable, host, at least z entries) their ids, scores and final residuals. */
 int nkpr_apr_run(nkpr_apr *a, double alpha, double eps, const uint64_t *seeds, uint64_t nseeds,
