@@ -366,7 +366,7 @@ int nkpr_nkbg_write(const char *path, const nkpr_nkbg *g, uint64_t chunks, int32
  * dst[m] (int32), w[m] (FP64, NULL = unit weights), host (device_ptrs = 0) or device pointers;
  * copied to the device with weightedDegree(u, countSelfLoopsTwice = true) per node.  Offsets
  * that do not rise from 0, ids outside [0, z) and negative or NaN weights are NKPR_ERR_INVALID.
- * The handle holds about 68 z + 12 m bytes of device memory. */
+ * The handle holds about 72 z + 12 m bytes of device memory. */
 typedef struct nkpr_apr nkpr_apr;
 typedef struct {
     uint64_t rounds;   /* frontier rounds                                                    */

@@ -1492,9 +1492,11 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
     BinCtl *ctl = L.ctl.as<BinCtl>();
     hipLaunchKernelGGL(bin_clear_kernel, dim3(1), dim3(64), 0, s, ctl);
     const auto scatter = derived ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
+    // the sources at or past sink_begin have no out-edges: pass 1 reads them as 0, so v may hold
+    // anything there (GMRES keeps other data past its head) and the scale ignores it
     hipLaunchKernelGGL(scatter, dim3(std::min<uint32_t>(std::max<uint32_t>(L.np1, 1), L.grid1)),
                        dim3(kPassThreads), 0, s, L.p1.as<uint4>(), L.np1, v,
-                       g->weighted ? nullptr : g->inv32.as<float>(), n, L.src.as<uint16_t>(),
+                       g->weighted ? nullptr : g->inv32.as<float>(), g->sink_begin, L.src.as<uint16_t>(),
                        g->weighted ? L.coef.as<float>() : nullptr, L.pos.as<uint32_t>(),
                        L.gmask.as<unsigned long long>(), L.gfirst.as<uint32_t>(), L.sdelta.as<uint32_t>(),
                        L.nsdelta, (uint32_t)L.mp, L.val.as<float>(), ctl);

@@ -199,6 +199,20 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         auto apply_b = [&](const float *v, float *w) {
             apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32, false);
         };
+        // Without sink handling the sinks (ids [sink_begin, n), no out-edges) are zero columns
+        // of P: the other rows form a closed system, and each sink's correction follows from
+        // them in the products GMRES computes anyway.  The Krylov vectors then span the head
+        // [0, nh) only, nh = sink_begin rounded up to 64 entries (a few sink rows in the head
+        // are harmless: their columns are zero too), which shrinks every Gram-Schmidt pass.
+        // NKPR_SINK_SPLIT=0 keeps full-length vectors.
+        Krylov krh = kr;
+        {
+            const char *se = std::getenv("NKPR_SINK_SPLIT");
+            const uint64_t nh = ((uint64_t)g->sink_begin + 63) / 64 * 64;
+            if (!dist && !(se && se[0] == '0') && nh < n) krh.n = (uint32_t)nh;
+            if (std::getenv("NKPR_DEBUG"))
+                std::fprintf(stderr, "[nkpr] Krylov vectors: %u of %u entries\n", krh.n, n);
+        }
         // Through the binned FP64 copy, the residual sweeps keep each contribution c = coef * x
         // to within kCompact64Rel |c|, so y = d P x + b is off by at most d kCompact64Rel
         // ||x||_1 (the column sums of P are at most 1) with ||x||_1 <= sum(y) + ||r||_1.  That
@@ -275,7 +289,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             const double target = 0.5 * p->tol * c.sum * (1.0 - damp) / (2.0 * damp);
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
             const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
-            const uint64_t kc = gmres_f32(kr, apply_b, kr.V, kr.e, rtol, left, s);
+            const uint64_t kc = gmres_f32(krh, apply_b, krh.V, krh.e, rtol, left, s, n);
             if (std::getenv("NKPR_DEBUG"))
                 std::fprintf(stderr, "[nkpr] refinement %d: residual l1 %.3e, bound %.3e, rtol %.3e, %llu applications\n",
                              round, c.l1, c.bound, rtol, (unsigned long long)kc);

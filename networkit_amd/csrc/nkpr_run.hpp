@@ -193,10 +193,17 @@ bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool dis
 // ends of the range.  `rhs` may be basis slot 0 itself; it is overwritten then.  The Hessenberg
 // least-squares problem is solved on the host with Givens rotations.  Returns the number of
 // applications.
+//
+// Head and tail (n_full > k.n): the Krylov vectors span the first k.n entries only, a closed
+// system when no row below k.n draws from a source at or past k.n (the sources there have no
+// out-edges).  `apply` still writes all n_full rows, and no projection touches the tail, so
+// slot i + 1 keeps (B u_i) on the tail; the tail rows of A e = rhs then follow exactly:
+// e_t = rhs_t + sum_i c_i (B u_i)_t with e_h = sum_i c_i u_i, which is what is written there.
 template <typename Apply>
 uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, double rtol,
-                   uint64_t cap, hipStream_t s) {
+                   uint64_t cap, hipStream_t s, uint32_t n_full = 0) {
     const uint32_t n = k.n;
+    const uint32_t n_tail = n_full > n ? n_full - n : 0;
     const int mres = k.m;
     const uint64_t ld = k.ld;
     float *const V = k.V;
@@ -215,6 +222,7 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
     const double beta = std::sqrt(beta2);
     if (!(beta > 0.0) || cap == 0) {
         launch_axpby(rhs, nullptr, e, n, 0.0f, 0.0f, s);
+        if (n_tail) launch_axpby(rhs + n, nullptr, e + n, n_tail, 1.0f, 0.0f, s);
         return 0;
     }
     std::vector<double> sig(mres + 1, 1.0);
@@ -298,6 +306,14 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
     }
     for (int i = 0; i < kk; ++i) y[i] *= sig[i];  // e = sum_i y_i v_i = sum_i (y_i sig_i) u_i
     launch_multiaxpy(e, V, ld, kk, n, y.data(), true, s);
+    if (n_tail) {
+        // e_t = rhs_t + sum_i c_i (B u_i)_t: rhs is slot 0 (its tail untouched), B u_i slot i + 1
+        NKPR_CHECK(rhs == V, "a tail needs the right-hand side in slot 0");
+        std::vector<double> ct(kk + 1);
+        ct[0] = 1.0;
+        for (int i = 0; i < kk; ++i) ct[i + 1] = y[i];
+        launch_multiaxpy(e + n, V + n, ld, kk + 1, n_tail, ct.data(), true, s);
+    }
     return applies;
 }
 

@@ -443,6 +443,53 @@ def test_mixed_binned_operator(monkeypatch, weighted, sinks, item):
 
 
 @pytest.mark.parametrize("weighted", [True, False])
+@pytest.mark.parametrize("binned", ["0", "1"])
+@pytest.mark.parametrize("teleport", [False, True])
+def test_mixed_krylov_head_without_sinks(monkeypatch, capfd, weighted, binned, teleport):
+    """Without sink handling the sinks are zero columns of P, so GMRES runs on the non-sink
+    head (sink_begin rounded up to 64 entries) and each sink's correction is rebuilt from the
+    products it computed (e_t = r_t + sum_i c_i (B u_i)_t).  Against full-length vectors
+    (NKPR_SINK_SPLIT=0): both certify at tol and match the converged oracle, with the same
+    refinements (within one); a wrong tail would leave the sinks' residual in place and the
+    certificate unmet.  Pull and binned operators (the binned pass 1 reads the sources past
+    sink_begin as 0), with and without a teleport vector."""
+    monkeypatch.setenv("NKPR_BINNED", binned)
+    monkeypatch.setenv("NKPR_BINNED_ITEM", "3000")
+    monkeypatch.setenv("NKPR_DEBUG", "1")
+    z, u, v, w = skewed_graph(90000, 700000, 67, weighted=weighted)
+    g = oracle_graph(z, u, v, w, True)
+    t = None
+    if teleport:
+        t = np.random.default_rng(3).random(z)
+        t /= t.sum()
+    ref, _, _ = oracle.pagerank(g, tol=1e-15, norm=0, max_iterations=5000, sink_mode=0,
+                                teleport=t)
+    w32 = None if g.in_w is None else g.in_w.astype(np.float32)
+    stats = {}
+    for split in ("0", "1"):
+        monkeypatch.setenv("NKPR_SINK_SPLIT", split)
+        dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), w32, weight64=g.in_w)
+        sinks = dg.info()["sinks"]
+        x = np.full(z, 1.0 / z)
+        kw = {} if t is None else {"teleport": t.ctypes.data}
+        capfd.readouterr()
+        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
+                                           stop=capi.NKPR_STOP_L1_ERROR, sink_handling=0, **kw))
+        err = capfd.readouterr().err
+        head = [line for line in err.splitlines() if "Krylov vectors" in line]
+        want = z if split == "0" else min(z, (z - sinks + 63) // 64 * 64)
+        assert head and all(("%d of %d" % (want, z)) in line for line in head), (head, want)
+        assert st["l1_error_bound"] <= 1e-9
+        assert np.abs(x - ref).sum() <= 1e-9
+        stats[split] = st
+    print("sinks %d of %d; refinements %d / %d, applications %d / %d" % (
+        sinks, z, stats["0"]["refinements"], stats["1"]["refinements"],
+        stats["0"]["sweeps_fp32"], stats["1"]["sweeps_fp32"]), flush=True)
+    assert sinks > z // 4
+    assert abs(int(stats["1"]["refinements"]) - int(stats["0"]["refinements"])) <= 1
+
+
+@pytest.mark.parametrize("weighted", [True, False])
 @pytest.mark.parametrize("item", ["0", "3000"])
 def test_binned_derived_places(monkeypatch, capfd, weighted, item):
     """Pass 1 of both binned copies (the FP32 operator and the FP64 sweep) derives every quad's
