@@ -322,6 +322,25 @@ int nkpr_csr_pagerank_step_noalloc(const nkpr_csr *m, double shift, const double
 int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                             double *scores, int32_t scores_on_device, uint64_t *iterations);
 
+/* AlgebraicPageRank as a handle over the matrix, for repeated runs.  nkpr_algebraic_create
+ * checks M (device arrays, as nkpr_algebraic_pagerank takes them; square, rows < 2^31 - 1,
+ * nnz < 2^32 - 1) and builds a PageRank handle whose in-lists are M's rows and whose
+ * coefficients are M's values (the caller may free M's arrays afterwards).
+ * nkpr_algebraic_run then replaces AlgebraicPageRank::run (AlgebraicPageRank.hpp:94-119) like
+ * nkpr_algebraic_pagerank (same start, teleport (1 - damp) / n, loop condition
+ * ||x' - x||_2 > tol, division by the sum; scores in row order, host or device), with the
+ * PageRank sweeps of the handle (relabelled, source-sliced, binned FP64 from 2^26 entries):
+ * each row is still the FP64 sum of its products, added in another order.  stats (nullable)
+ * receives iterations, sweeps_fp64, ms_sweeps_fp64 (HIP events around every sweep on the
+ * handle's stream), score_sum, last_step (the final ||x' - x||_2) and ms_total.  Blocks until
+ * done. */
+typedef struct nkpr_algebraic nkpr_algebraic;
+int nkpr_algebraic_create(const nkpr_csr *m, nkpr_algebraic **out);
+int nkpr_algebraic_run(nkpr_algebraic *h, double damp, double tol, uint64_t max_iterations,
+                       double *scores, int32_t scores_on_device, uint64_t *iterations,
+                       nkpr_stats *stats);
+int nkpr_algebraic_destroy(nkpr_algebraic *h);
+
 /* ---- NetworKit binary graph files (nkbg002) ------------------------------------------------
  * Host-side CSR lists of a graph in NetworKit's binary format (include/networkit/io/
  * NetworkitBinaryGraph.hpp:20-141).  Arrays are host memory owned by the struct (released by

@@ -23,7 +23,7 @@ ROCM_LIB = "/opt/rocm/lib"
 
 HIP_SOURCES = ["nkpr_sweep.hip", "nkpr_graph.hip", "nkpr_krylov.hip", "nkpr_batch.hip",
                "nkpr_adjoint.hip", "nkpr_groups.hip", "nkpr_binned.hip", "nkpr_power.hip",
-               "nkpr_csr.hip", "nkpr_nkbg.hip", "nkpr_apr.hip", "nkpr_capi.hip"]
+               "nkpr_csr.hip", "nkpr_algebraic.hip", "nkpr_nkbg.hip", "nkpr_apr.hip", "nkpr_capi.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # packed FP32 instructions stay off (some of their forms are unreliable on this pool)

@@ -32,7 +32,8 @@ EXPORTED = ["nkpr_params_default", "nkpr_graph_create", "nkpr_graph_create_batch
             "nkpr_power_params_default", "nkpr_power_run", "nkpr_csr_spmv",
             "nkpr_csr_pagerank_step", "nkpr_algebraic_pagerank", "nkpr_nkbg_read",
             "nkpr_nkbg_free", "nkpr_nkbg_write", "nkpr_csr_pagerank_step_noalloc",
-            "nkpr_apr_create", "nkpr_apr_destroy", "nkpr_apr_run"]
+            "nkpr_apr_create", "nkpr_apr_destroy", "nkpr_apr_run", "nkpr_algebraic_create",
+            "nkpr_algebraic_run", "nkpr_algebraic_destroy"]
 ABI_VERSION = 3
 
 
@@ -166,6 +167,11 @@ def lib():
     L.nkpr_apr_run.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_uint64,
                                ctypes.POINTER(ctypes.c_uint64), vp, vp, vp,
                                ctypes.POINTER(AprStats)]
+    L.nkpr_algebraic_create.argtypes = [ctypes.POINTER(CsrMatrix), ctypes.POINTER(vp)]
+    L.nkpr_algebraic_run.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, vp,
+                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.POINTER(Stats)]
+    L.nkpr_algebraic_destroy.argtypes = [vp]
     L.nkpr_last_error.restype = ctypes.c_char_p
     L.nkpr_abi_version.restype = ctypes.c_int
     for name in EXPORTED:
@@ -466,6 +472,46 @@ class DeviceCSR:
         check(lib().nkpr_algebraic_pagerank(ctypes.byref(m), damp, tol, int(max_iterations),
                                             scores.data_ptr(), 1, ctypes.byref(it)))
         return scores, int(it.value)
+
+
+class DeviceAlgebraic:
+    """nkpr_algebraic handle over a DeviceCSR's matrix M (built once, reused by every run)."""
+
+    def __init__(self, csr):
+        m = csr.struct()
+        h = ctypes.c_void_p()
+        check(lib().nkpr_algebraic_create(ctypes.byref(m), ctypes.byref(h)))
+        self.handle = h
+        self.rows = csr.rows
+
+    def run(self, damp=0.85, tol=1e-8, max_iterations=2**64 - 1, scores=None, stats=False):
+        """nkpr_algebraic_run; returns (scores, iterations) like DeviceCSR.algebraic_pagerank,
+        and the stats dict third when stats=True (sweeps timed with HIP events)."""
+        it = ctypes.c_uint64(0)
+        st = Stats()
+        sp = ctypes.byref(st) if stats else None
+        if scores is None:
+            out = np.zeros(self.rows, dtype=np.float64)
+            check(lib().nkpr_algebraic_run(self.handle, damp, tol, int(max_iterations), _addr(out),
+                                           0, ctypes.byref(it), sp))
+        else:
+            out = scores
+            check(lib().nkpr_algebraic_run(self.handle, damp, tol, int(max_iterations),
+                                           scores.data_ptr(), 1, ctypes.byref(it), sp))
+        if stats:
+            return out, int(it.value), {f: getattr(st, f) for f, _ in st._fields_}
+        return out, int(it.value)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().nkpr_algebraic_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _copy(ptr, dtype, count):

@@ -2,8 +2,8 @@
 //
 // The matrix is built on the host exactly as the reference builds it (the triplet constructor
 // is a counting sort that keeps the triplets' order inside a row, CSRGeneralMatrix.hpp:151-182);
-// products and the PageRank loop run on the device through nkpr_csr_spmv and
-// nkpr_algebraic_pagerank.
+// products run on the device through nkpr_csr_spmv, the PageRank loop through a handle built
+// from M once (nkpr_algebraic_create / nkpr_algebraic_run).
 #include "AlgebraicPageRank.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -161,12 +161,20 @@ AlgebraicPageRank<CSRMatrix>::AlgebraicPageRank(const Graph &graph, double damp_
 }
 
 template <> void AlgebraicPageRank<CSRMatrix>::run() {
+    // AlgebraicPageRank.hpp:94-119 on the device: the first run builds a PageRank handle from M
+    // (nkpr_algebraic_create) and every run iterates on it (nkpr_algebraic_run)
     const count n = M.numberOfRows();
     if (n == 0) throw std::runtime_error("AlgebraicPageRank on an empty graph");
-    const nkpr_csr m = M.deviceView();
+    if (!device) {
+        const nkpr_csr m = M.deviceView();
+        nkpr_algebraic *h = nullptr;
+        nkpr_check(nkpr_algebraic_create(&m, &h));
+        device.reset(h, [](nkpr_algebraic *p) { nkpr_algebraic_destroy(p); });
+    }
     scoreData.assign(n, 0.0);
     uint64_t it = 0;
-    nkpr_check(nkpr_algebraic_pagerank(&m, damp, tol, UINT64_MAX, scoreData.data(), 0, &it));
+    nkpr_check(nkpr_algebraic_run(device.get(), damp, tol, UINT64_MAX, scoreData.data(), 0, &it,
+                                  nullptr));
     iterations = it;
     hasRun = true;
 }

@@ -146,6 +146,8 @@ private:
     const double damp;
     const double tol;
     count iterations = 0;
+    // the device operator built from M on the first run and kept for the next ones
+    std::shared_ptr<nkpr_algebraic> device;
 };
 
 template <> AlgebraicPageRank<CSRMatrix>::AlgebraicPageRank(const Graph &graph, double damp,

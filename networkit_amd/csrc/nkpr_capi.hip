@@ -679,6 +679,29 @@ int nkpr_algebraic_pagerank(const nkpr_csr *m, double damp, double tol, uint64_t
     });
 }
 
+int nkpr_algebraic_create(const nkpr_csr *m, nkpr_algebraic **out) {
+    return guarded([&] { algebraic_create_impl(m, out); });
+}
+
+int nkpr_algebraic_run(nkpr_algebraic *h, double damp, double tol, uint64_t max_iterations,
+                       double *scores, int32_t scores_on_device, uint64_t *iterations,
+                       nkpr_stats *stats) {
+    return guarded([&] {
+        NKPR_CHECK(h != nullptr, "handle is NULL");
+        DeviceGuard dg(algebraic_device(h));
+        algebraic_run_impl(h, damp, tol, max_iterations, scores, scores_on_device, iterations,
+                           stats);
+    });
+}
+
+int nkpr_algebraic_destroy(nkpr_algebraic *h) {
+    return guarded([&] {
+        if (!h) return;
+        DeviceGuard dg(algebraic_device(h));
+        algebraic_destroy_impl(h);
+    });
+}
+
 int nkpr_nkbg_read(const char *path, int32_t with_out_lists, nkpr_nkbg *out) {
     return guarded([&] {
         try {

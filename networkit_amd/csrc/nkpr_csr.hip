@@ -552,6 +552,21 @@ void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, dou
                                                        kNoallocPieces), step_sq, s);
 }
 
+void csr_validate(const nkpr_csr *m, hipStream_t s) {
+    const CsrView A = view_of(m);
+    DevBuf flag;
+    flag.alloc(4);
+    NKPR_HIP(hipMemsetAsync(flag.p, 0, 4, s));
+    hipLaunchKernelGGL(csr_check_kernel, dim3(2048), dim3(256), 0, s, A, m->cols, m->nnz,
+                       flag.as<uint32_t>());
+    NKPR_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    uint64_t first = 0, last = 0;
+    read_back(s, {{&bad, flag.p, 4}, {&first, A.row_idx, 8}, {&last, A.row_idx + A.rows, 8}});
+    NKPR_CHECK(bad == 0 && first == 0 && last == m->nnz,
+               "malformed CSR matrix (row_idx must rise from 0 to nnz, column indices < cols)");
+}
+
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                              double *scores, int32_t scores_on_device, uint64_t *iterations) {
     const CsrView A = view_of(m);

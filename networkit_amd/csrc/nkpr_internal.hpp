@@ -335,12 +335,22 @@ void power_run_impl(nkpr_graph *g, const nkpr_power_params *p, double *scores, n
 // CSR products and AlgebraicPageRank (nkpr_csr.hip).  The plain product and the step take
 // their scratch from the stream's memory pool, the allocation-free step from a static array.
 void csr_spmv_impl(const nkpr_csr *a, const double *x, double *y, hipStream_t s);
+// throws NKPR_ERR_INVALID unless row_idx rises from 0 to nnz and every column index is < cols
+void csr_validate(const nkpr_csr *m, hipStream_t s);
 void csr_step_impl(const nkpr_csr *m, double shift, const double *x, double *y, double *step_sq,
                    hipStream_t s);
 void csr_step_noalloc_impl(const nkpr_csr *m, double shift, const double *x, double *y,
                            double *step_sq, hipStream_t s);
 void algebraic_pagerank_impl(const nkpr_csr *m, double damp, double tol, uint64_t max_iterations,
                              double *scores, int32_t scores_on_device, uint64_t *iterations);
+
+// AlgebraicPageRank through a PageRank handle built from the matrix (nkpr_algebraic.hip)
+void algebraic_create_impl(const nkpr_csr *m, nkpr_algebraic **out);
+void algebraic_run_impl(nkpr_algebraic *h, double damp, double tol, uint64_t max_iterations,
+                        double *scores, int32_t scores_on_device, uint64_t *iterations,
+                        nkpr_stats *stats);
+void algebraic_destroy_impl(nkpr_algebraic *h);
+int algebraic_device(const nkpr_algebraic *h);
 
 // Approximate personalised PageRank (nkpr_apr.hip)
 void apr_create_impl(uint64_t z, const uint64_t *ptr, const int32_t *dst, const double *w,

@@ -6,9 +6,10 @@
       entries), with empty rows and duplicate entries, against oracle.csr_spmv (sums in
       another order: 1e-13 relative to the sum of |terms|);
   nkpr_csr_pagerank_step: y and ||y - x||^2 against numpy;
-  nkpr_algebraic_pagerank (C ABI and the C++/Cython drop-in): the oracle's iteration count and
-      L1 <= 1e-12 on the reference's known-answer graphs, graphs with deleted slots and
-      skewed graphs with long rows; results bit-identical between runs.
+  nkpr_algebraic_pagerank and the handle route (nkpr_algebraic_create/run, which the C++/Cython
+      drop-in takes): the oracle's iteration count and L1 <= 1e-12 on the reference's
+      known-answer graphs, graphs with deleted slots and skewed graphs with long rows; results
+      bit-identical between runs.
 """
 import numpy as np
 import pytest
@@ -129,6 +130,45 @@ def test_algebraic_pagerank_capi(case):
         assert top == WIKI_TOP[False][0] and abs(got[top] - WIKI_TOP[False][1]) <= 1e-3
     if name == "skewed_long_rows":
         assert np.diff(mat.row_idx.astype(np.int64)).max() > 2048
+
+
+@pytest.mark.parametrize("case", [c[0] for c in graph_cases()])
+@pytest.mark.parametrize("binned", ["0", "1"])
+def test_algebraic_handle(monkeypatch, case, binned):
+    """nkpr_algebraic_create/run: the PageRank handle built from M (in-lists = M's rows,
+    coefficients = M's values) runs the reference's loop with the oracle's iteration count and
+    L1 <= 1e-12, through the pull sweep and through the binned FP64 copy (forced here, the
+    default from 2^26 entries); a second run on the same handle repeats the bits."""
+    monkeypatch.setenv("NKPR_BINNED64", binned)
+    name, (z, u, v, w), directed = next(c for c in graph_cases() if c[0] == case)
+    eu, ev, ew = oracle.for_edges(z, u, v, w, directed)
+    mat = oracle.algebraic_matrix(z, eu, ev, ew, directed)
+    ref, it = oracle.algebraic_pagerank(mat)
+    d = capi.DeviceCSR(z, z, mat.row_idx, mat.col_idx, mat.values)
+    h = capi.DeviceAlgebraic(d)
+    got, it_dev = h.run()
+    again, it2 = h.run()
+    h.close()
+    e = np.abs(got - ref).sum()
+    print("%s (binned %s): %d iterations (oracle %d), L1 %.3e" % (name, binned, it_dev, it, e),
+          flush=True)
+    assert it_dev == it == it2
+    assert e <= 1e-12
+    assert np.array_equal(got, again)
+    if name == "eight":
+        np.testing.assert_allclose(got, EIGHT_PR, atol=1e-4)
+
+
+def test_algebraic_handle_rejects_malformed_matrix():
+    a = random_csr(100, 100, np.full(100, 4), 22)
+    col = a.col_idx.copy()
+    col[5] = 1000
+    d = capi.DeviceCSR(100, 100, a.row_idx, col, a.values)
+    with pytest.raises(capi.NkprError, match="malformed"):
+        capi.DeviceAlgebraic(d)
+    rect = capi.DeviceCSR(100, 50, a.row_idx, a.col_idx % 50, a.values)
+    with pytest.raises(capi.NkprError, match="square"):
+        capi.DeviceAlgebraic(rect)
 
 
 def test_algebraic_pagerank_drop_in_with_deleted_nodes():
