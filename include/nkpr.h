@@ -404,8 +404,8 @@ int nkpr_apr_destroy(nkpr_apr *a);
  * a frontier at a time on the device (the reference runs a FIFO queue on one thread): the
  * identity p + pr_alpha(r) = pr_alpha(seeds) and the stopping rule are the reference's, the
  * values differ from its within that rule.  As in the reference, the share a push sends
- * through a self-loop is dropped and a node without neighbours is pushed while residual is
- * left.  Residuals are summed exactly (64-bit fixed point, 2^-62 units), so a run repeats bit
+ * through a self-loop is dropped, and a node without out-edges is never queued by a
+ * neighbour's add (its residual stays) but, as a seed, is pushed while residual is left.  Residuals are summed exactly (64-bit fixed point, 2^-62 units), so a run repeats bit
  * for bit; calls on one handle must not overlap.  *count receives the number of touched nodes
  * (every node that held residual, the reference's result set); ids_out (ascending), p_out and
  * r_out (nullable, host, at least z entries) their ids, scores and final residuals. */

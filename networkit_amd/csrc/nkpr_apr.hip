@@ -68,11 +68,12 @@ constexpr int kBatch = 8;  // rounds per host read of the counters
 constexpr double kUnit = 4611686018427387904.0;  // 2^62 fixed-point units per unit of mass
 
 // ceil(eps vol 2^62): r < thr  <=>  r 2^-62 < eps vol (the product rounded as the reference's).
-// A node without neighbours (vol 0) is pushed again while any residual is left: the reference's
-// r / vol >= eps holds for every r > 0 there.
-__device__ __forceinline__ uint64_t threshold(double eps, double vol) {
+// A node without out-edges (vol 0) follows the reference's two tests: a neighbour's add never
+// queues it (r[v] < vol eps is false), and as a pushed node (only a seed can be one) it is
+// pushed again while any residual is left (r / vol >= eps holds for every r > 0).
+__device__ __forceinline__ uint64_t threshold_of(double eps, double vol, bool self) {
     const double t = eps * vol * kUnit;
-    if (!(t > 0.0)) return 1ull;
+    if (!(t > 0.0)) return self ? 1ull : ~0ull;
     return t >= 1.8e19 ? ~0ull : (uint64_t)ceil(t);
 }
 
@@ -193,7 +194,7 @@ __global__ void apr_take_kernel(const uint64_t *ptr, const double *vol, double a
             p[u] += alpha * dres / kUnit;
             const uint64_t keep = __double2ull_rn((1.0 - alpha) * dres / 2);
             r[u] = keep;
-            again = keep >= threshold(eps, vol[u]);
+            again = keep >= threshold_of(eps, vol[u], true);
             flags[u] = again ? (kTouched | kQueued) : kTouched;
             const uint64_t deg = ptr[u + 1] - ptr[u];
             atomicAdd(&bedges, (unsigned long long)deg);
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kPushThreads) void apr_push_kernel(
             // a self-loop sends nothing: the reference overwrites r[u] after its loop
             if (e < end && (v = (uint32_t)dst[e]) != u) {
                 const uint64_t mass = __double2ull_rn(share * (w ? w[e] : 1.0));
-                const uint64_t thr = threshold(eps, vol[v]);
+                const uint64_t thr = threshold_of(eps, vol[v], false);
                 const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long *>(&r[v]),
                                                (unsigned long long)mass);
                 if (old < thr && old + mass >= thr)

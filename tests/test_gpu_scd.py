@@ -78,6 +78,43 @@ def test_apr_against_oracle(weighted, seeds, alpha, eps):
 
 
 @pytest.mark.parametrize("weighted", [False, True])
+def test_apr_directed_with_sinks(weighted):
+    """Directed out-lists with sinks: a sink's residual stays (no neighbour's add queues it), a
+    sink seed is pushed out (losing what it cannot send); the device vector satisfies the
+    identity for the walk whose sink rows are the lazy half alone, loses no other mass, and lies
+    within the two runs' residual mass of the oracle's."""
+    z, u, v, w = random_graph(3000, 9000, 19, directed=True, weighted=weighted, self_loops=False)
+    ptr, dst, ww = oracle.out_lists(z, u, v, w, directed=True)
+    vol = oracle.weighted_degree2(z, ptr, dst, ww)
+    sinks = np.nonzero(vol == 0)[0]
+    assert len(sinks) > 50
+    seeds = [int(np.nonzero(vol > 0)[0][3]), int(sinks[2])]
+    alpha, eps = 0.15, 1e-9
+    dev = capi.DeviceAPR(z, ptr, dst.astype(np.int32), ww)
+    ids, p, r, st = dev.run(seeds, alpha, eps)
+    dev.close()
+    P = np.zeros(z)
+    R = np.zeros(z)
+    P[ids] = p
+    R[ids] = r
+    s = np.zeros(z)
+    s[np.unique(seeds)] = 0.5
+    np.testing.assert_allclose(P + lazy_ppr(z, ptr, dst, ww, R, alpha),
+                               lazy_ppr(z, ptr, dst, ww, s, alpha), rtol=0, atol=1e-12)
+    # every push of the sink seed adds alpha res to p and has nowhere to send (1 - alpha) res / 2:
+    # (1 - alpha) / (2 alpha) p[seed] is lost in all; every other push conserves mass
+    lost = (1 - alpha) / (2 * alpha) * P[seeds[1]]
+    assert lost > 0.3 and abs(P.sum() + R.sum() - (1.0 - lost)) <= 1e-12
+    assert (R[vol > 0] < eps * vol[vol > 0]).all()
+    assert R[seeds[1]] == 0.0 and R[sinks].sum() > 0
+    po, ro, _ = oracle.approximate_pagerank(z, ptr, dst, ww, seeds, alpha, eps)
+    d = np.abs(P - po).sum()
+    print("directed, weighted %s: %d rounds, %d touched, |p - p_oracle|_1 %.2e (bound %.2e)" % (
+        weighted, st["rounds"], st["touched"], d, R.sum() + ro.sum()), flush=True)
+    assert d <= R.sum() + ro.sum() + 1e-12
+
+
+@pytest.mark.parametrize("weighted", [False, True])
 def test_apr_self_loops(weighted):
     """Self-loops: a push sends nothing through one (the reference overwrites r[u] after its
     loop), so the identity holds for the walk without them and mass leaks, on the device as in

@@ -434,7 +434,9 @@ def lazy_ppr(z, ptr, dst, w, s, alpha):
     off = rows != cols
     A = sp.csr_matrix((wt[off], (rows[off], cols[off])), shape=(z, z))
     d = np.bincount(rows, weights=np.where(off, wt, 2.0 * wt), minlength=z)
-    W = 0.5 * (sp.identity(z) + sp.diags(1.0 / d) @ A)
+    # a node without out-edges sends nothing: its row is the lazy half alone
+    inv = np.divide(1.0, d, out=np.zeros(z), where=d > 0)
+    W = 0.5 * (sp.identity(z) + sp.diags(inv) @ A)
     M = sp.identity(z, format="csc") - (1 - alpha) * W.T.tocsc()
     return alpha * spl.spsolve(M, s)
 
@@ -473,6 +475,35 @@ def test_apr_acl_identity_and_invariants(weighted, seeds, self_loops):
         assert abs(p.sum() + r.sum() - 1.0) <= 1e-12
     assert (p >= 0).all()
     assert np.array_equal(touched.astype(bool), (p > 0) | (r > 0))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_apr_directed_with_sinks(weighted):
+    """Directed graphs (forNeighborsOf = out-edges): a node without out-edges is never queued by
+    a neighbour's add, so its residual stays (the reference's r[v] < vol eps is false at vol
+    0), and a seed without out-edges is pushed until its residual is gone (r / 0 >= eps),
+    losing the half of every push it cannot send.  The identity holds with the lazy walk whose
+    rows at such nodes are the lazy half alone; every other push conserves mass; residuals at
+    nodes with out-edges end below eps vol."""
+    z, u, v, w = random_graph(500, 1500, 17, directed=True, weighted=weighted, self_loops=False)
+    ptr, dst, ww = oracle.out_lists(z, u, v, w, directed=True)
+    vol = oracle.weighted_degree2(z, ptr, dst, ww)
+    sinks = np.nonzero(vol == 0)[0]
+    assert len(sinks) > 10
+    seeds = [int(np.nonzero(vol > 0)[0][0]), int(sinks[0])]
+    alpha, eps = 0.15, 1e-8
+    p, r, touched = oracle.approximate_pagerank(z, ptr, dst, ww, seeds, alpha, eps)
+    s = np.zeros(z)
+    s[np.unique(seeds)] = 0.5
+    np.testing.assert_allclose(p + lazy_ppr(z, ptr, dst, ww, r, alpha),
+                               lazy_ppr(z, ptr, dst, ww, s, alpha), rtol=0, atol=1e-12)
+    # every push of the sink seed adds alpha res to p and has nowhere to send (1 - alpha) res / 2:
+    # (1 - alpha) / (2 alpha) p[seed] is lost in all; every other push conserves mass
+    lost = (1 - alpha) / (2 * alpha) * p[seeds[1]]
+    assert lost > 0.3 and abs(p.sum() + r.sum() - (1.0 - lost)) <= 1e-12
+    assert (r[vol > 0] < eps * vol[vol > 0]).all()
+    assert r[seeds[1]] == 0.0
+    assert r[sinks].sum() > 0                 # mass that reached sinks stays there
 
 
 def sp_lazy_keep(z, ptr, dst, w, alpha):
