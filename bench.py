@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
     ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "graphs", "adjoint",
-                                           "algebraic", "apr"],
+                                           "algebraic", "apr", "nkbg"],
                     default="pagerank",
                     help="pagerank: one solve on R-MAT (the benchmark line); rhg: the same on "
                          "an undirected random hyperbolic graph; ppr: batched personalised "
@@ -66,9 +66,12 @@ def main():
                          "Erdos-Renyi graphs of uneven size; adjoint: forward and backward "
                          "of differentiable PageRank on R-MAT scale 20; algebraic: "
                          "AlgebraicPageRank on NetworKit's CSR matrix of the R-MAT graph; apr: "
-                         "ApproximatePageRank pushes on the random hyperbolic graph")
+                         "ApproximatePageRank pushes on the random hyperbolic graph; nkbg: a "
+                         "NetworKit binary file of the R-MAT graph read into device lists")
     ap.add_argument("--rhg-nodes", type=int, default=10_000_000)
     ap.add_argument("--apr-eps", type=float, default=1e-9)
+    ap.add_argument("--nkbg-scale", type=int, default=24)
+    ap.add_argument("--nkbg-dir", default="/tmp")
     ap.add_argument("--ppr-nodes", type=int, default=2_000_000)
     ap.add_argument("--ppr-columns", type=int, default=256)
     ap.add_argument("--graphs", type=int, default=512)
@@ -91,6 +94,9 @@ def main():
         return
     if args.workload == "apr":
         bench_apr(args)
+        return
+    if args.workload == "nkbg":
+        bench_nkbg(args)
         return
 
     import torch
@@ -808,6 +814,129 @@ def bench_apr(args):
                         "frac": ach / HBM_PEAK_GBS, "traffic": None,
                         "kernel": "whole run (push rounds, frontier swaps, compaction)",
                         "bytes_per_sweep": bytes_run, "ms_per_sweep": sec * 1e3},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def write_rmat_nkbg(path, scale, chunks=32):
+    """The benchmark's R-MAT graph (seed 2, FP32 weights) at `scale` written as nkbg002 by the
+    library's writer (out-lists and in-lists in storage order, as NetworKit's writer takes
+    them from a Graph).  Returns (nodes, arcs)."""
+    import torch
+
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+    g = gen.rmat(scale, 16, seed=2, device="cuda")
+    z = g["z"]
+    colptr, src, w = g["colptr"], g["src"], g["weight"]
+    m = src.numel()
+    dst = torch.repeat_interleave(torch.arange(z, device=src.device, dtype=torch.int32),
+                                  (colptr[1:] - colptr[:-1]).to(torch.int64))
+    order = torch.argsort(src, stable=True)
+    out_ptr = torch.zeros(z + 1, dtype=torch.int64, device=src.device)
+    out_ptr[1:] = torch.cumsum(torch.bincount(src.to(torch.int64), minlength=z), 0)
+    host = lambda t: t.cpu().numpy()
+    w64 = w.to(torch.float64)
+    capi.write_nkbg(path, z, True, host(out_ptr).astype(np.uint64), host(dst[order]),
+                    host(w64[order]), host(colptr).astype(np.uint64), host(src), host(w64),
+                    None, chunks=chunks)
+    del g, colptr, src, w, dst, order, out_ptr, w64
+    torch.cuda.synchronize()
+    return z, m
+
+
+def bench_nkbg(args):
+    """NetworKit binary graph files (NetworkitBinaryReader.cpp:19-237) to device lists: the
+    R-MAT graph (scale nkbg_scale, edge factor 16, seed 2, FP32 weights) written by the
+    library's writer with the reference writer's default of 32 chunks, then read straight into
+    the flat pull lists PageRank takes (no per-node vectors) and copied to HBM.  One step =
+    nkpr_nkbg_read (the file memory-mapped and decoded on every host thread, one chunk per
+    thread at a time) + the copy of colptr, src and the weights to device memory.  Metric: arcs
+    per second, file to device."""
+    import ctypes
+    import torch
+
+    from networkit_amd import capi
+
+    torch.cuda.set_device(0)
+    path = os.path.join(args.nkbg_dir, "nkpr_bench_rmat%d.nkbg" % args.nkbg_scale)
+    t0 = time.perf_counter()
+    z, m = write_rmat_nkbg(path, args.nkbg_scale)
+    size = os.path.getsize(path)
+    log("nkbg: R-MAT scale %d (%d arcs) written in %.1f s, %.2f GB" % (
+        args.nkbg_scale, m, time.perf_counter() - t0, size / 1e9))
+    L = capi.lib()
+    dev = torch.device("cuda:0")
+
+    def step():
+        h = capi.Nkbg()
+        t = time.perf_counter()
+        capi.check(L.nkpr_nkbg_read(os.fsencode(path), 0, ctypes.byref(h)))
+        t_read = time.perf_counter() - t
+        try:
+            n = int(h.nodes)
+            mm = int(np.ctypeslib.as_array(ctypes.cast(h.colptr, ctypes.POINTER(ctypes.c_uint64)),
+                                           (n + 1,))[-1])
+            arrays = [(h.colptr, ctypes.c_int64, n + 1), (h.src, ctypes.c_int32, mm)]
+            if h.weighted:
+                arrays.append((h.weight, ctypes.c_double, mm))
+            out = []
+            for ptr, ct, cnt in arrays:
+                a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), (cnt,))
+                out.append(torch.from_numpy(a).to(dev, non_blocking=False))
+            torch.cuda.synchronize()
+        finally:
+            L.nkpr_nkbg_free(ctypes.byref(h))
+        return time.perf_counter() - t, t_read, out
+
+    try:
+        for _ in range(args.warmup):
+            step()
+        tot, rd = [], []
+        for _ in range(args.steps):
+            t_all, t_read, out = step()
+            tot.append(t_all)
+            rd.append(t_read)
+        sec, sec_read = sum(tot) / len(tot), sum(rd) / len(rd)
+        same = bool(int(out[0][-1].item()) == m and int(out[1].numel()) == m)
+        del out
+    finally:
+        os.remove(path)
+    log("nkbg: file to device %.3f s (%.3g arcs/s, %.2f GB/s of file): decode %.3f s on %d host "
+        "threads, copies %.3f s" % (sec, m / sec, size / sec / 1e9, sec_read, os.cpu_count() or 1,
+                                    sec - sec_read))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import nkbg as onkbg
+        small = os.path.join(args.nkbg_dir, "nkpr_bench_rmat16.nkbg")
+        zs, ms = write_rmat_nkbg(small, 16)
+        data = open(small, "rb").read()
+        os.remove(small)
+        t = time.perf_counter()
+        onkbg.read(data)
+        tc = time.perf_counter() - t
+        log("cpu baseline: the oracle's reader on R-MAT scale 16 (%d arcs): %.2f s" % (ms, tc))
+        cpu = {"value": ms / tc, "unit": "arcs/s", "cores": 1, "kind": "port",
+               "sample": "the oracle's restatement of NetworkitBinaryReader::read (Python, one "
+                         "thread, into per-node lists as the reference builds them) on the same "
+                         "generator's scale-16 graph (%d arcs, %.2f s)" % (ms, tc),
+               "host": host_info()}
+    out = {"metric": "NetworKit binary graph file to device pull lists, arcs/s",
+           "value": m / sec, "unit": "arcs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": "nkbg002 file (32 chunks, FP32 weights) of a directed R-MAT "
+                                  "graph, scale %d, edge factor 16, seed 2, read into flat "
+                                  "in-lists and weights and copied to device memory"
+                                  % args.nkbg_scale,
+                      "n": z, "m": m, "file_bytes": size, "decode_s": sec_read,
+                      "copy_s": sec - sec_read, "lists_complete": same},
+           "roofline": {"bound": "hbm", "achieved": size / sec / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": size / sec / 1e9 / HBM_PEAK_GBS,
+                        "traffic": None,
+                        "kernel": "none: the host decode of the file bounds the path (file "
+                                  "bytes per second, file to device)",
+                        "bytes_per_sweep": size, "ms_per_sweep": sec * 1e3},
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

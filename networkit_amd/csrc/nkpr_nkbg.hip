@@ -10,7 +10,8 @@
 //
 // nkpr_nkbg_read restates NetworkitBinaryReader::read (NetworkitBinaryReader.cpp:19-237)
 // without building NetworKit's per-node vectors: the file is memory-mapped and decoded chunk
-// by chunk on host threads, twice (list lengths, then entries), into CSR arrays:
+// by chunk on host threads, once, into per-chunk buffers in final order, which are then moved
+// into CSR arrays at the summed offsets:
 //   pull lists   in-lists (directed: the transposed lists) or the adjacency (undirected: u's
 //                adjacency list followed by its transposed list without self-loops, the order
 //                the reference's reader appends them in), ready for nkpr_graph_create;
@@ -209,78 +210,83 @@ void nkbg_read_impl(const char *path, int32_t with_out_lists, nkpr_nkbg *out) {
         return s;
     };
     const bool want_out = directed && with_out_lists;
+    const bool weighted = fmt != kNone;
+    // one pass: every chunk decodes its nodes' pull lists (and out-lists) into buffers of its
+    // own, in final order (u's adjacency entries, then its transposed entries without the
+    // self-loop when undirected); the offsets are then summed and the buffers moved into place
+    struct ChunkLists {
+        std::vector<int32_t> ids, out_ids;
+        std::vector<double> w, out_w;
+    };
+    std::vector<ChunkLists> cl(chunks);
     std::vector<uint64_t> pull(nodes, 0), outc(want_out ? nodes : 0, 0);
     std::vector<uint64_t> loops(chunks, 0);
-    // pass 1: list lengths (and self-loops)
     for_chunks(chunks, [&](uint64_t c) {
         Stream a = stream(o_adj, 8, c), t = stream(o_tr, 8, c);
+        Stream aw{f.p, 0, 0}, tw{f.p, 0, 0};
+        if (weighted) {
+            aw = stream(o_w, 0, c);
+            tw = stream(o_tw, 0, c);
+        }
+        ChunkLists &L = cl[c];
         for (uint64_t u = first[c]; u < first[c + 1]; ++u) {
             const uint64_t nout = a.varint(), nin = t.varint();
             NKPR_CHECK(nout <= f.size && nin <= f.size, "nkbg list length outside the file");
-            uint64_t own_in = 0;
+            const size_t before = L.ids.size();
             for (uint64_t j = 0; j < nout; ++j) {
                 const uint64_t v = a.varint();
                 NKPR_CHECK(v < nodes, "nkbg neighbour id out of range");
+                const double w = weighted ? aw.weight(fmt) : 1.0;
                 loops[c] += v == u;
+                if (directed) {
+                    if (want_out) {
+                        L.out_ids.push_back((int32_t)v);
+                        if (weighted) L.out_w.push_back(w);
+                    }
+                } else {
+                    L.ids.push_back((int32_t)v);
+                    if (weighted) L.w.push_back(w);
+                }
             }
             for (uint64_t j = 0; j < nin; ++j) {
                 const uint64_t v = t.varint();
                 NKPR_CHECK(v < nodes, "nkbg neighbour id out of range");
-                own_in += v == u;
+                const double w = weighted ? tw.weight(fmt) : 1.0;
+                if (!directed && v == u) continue;
+                L.ids.push_back((int32_t)v);
+                if (weighted) L.w.push_back(w);
             }
-            pull[u] = directed ? nin : nout + nin - own_in;
+            pull[u] = L.ids.size() - before;
             if (want_out) outc[u] = nout;
         }
     });
     for (uint64_t u = 0; u < nodes; ++u) out->colptr[u + 1] = out->colptr[u] + pull[u];
     const uint64_t m = out->colptr[nodes];
     out->src = alloc_array<int32_t>(m);
-    if (fmt != kNone) out->weight = alloc_array<double>(m);
+    if (weighted) out->weight = alloc_array<double>(m);
     if (want_out) {
         out->out_ptr = alloc_array<uint64_t>(nodes + 1);
         out->out_ptr[0] = 0;
         for (uint64_t u = 0; u < nodes; ++u) out->out_ptr[u + 1] = out->out_ptr[u] + outc[u];
         out->out_dst = alloc_array<int32_t>(out->out_ptr[nodes]);
-        if (fmt != kNone) out->out_weight = alloc_array<double>(out->out_ptr[nodes]);
+        if (weighted) out->out_weight = alloc_array<double>(out->out_ptr[nodes]);
     }
-    // pass 2: entries
     for_chunks(chunks, [&](uint64_t c) {
-        Stream a = stream(o_adj, 8, c), t = stream(o_tr, 8, c);
-        Stream aw{f.p, 0, 0}, tw{f.p, 0, 0};
-        if (fmt != kNone) {
-            aw = stream(o_w, 0, c);
-            tw = stream(o_tw, 0, c);
+        ChunkLists &L = cl[c];
+        const uint64_t at = out->colptr[first[c]];
+        NKPR_CHECK(at + L.ids.size() == out->colptr[first[c + 1]], "nkbg chunk lists do not add up");
+        if (!L.ids.empty()) std::memcpy(out->src + at, L.ids.data(), 4 * L.ids.size());
+        if (weighted && !L.w.empty()) std::memcpy(out->weight + at, L.w.data(), 8 * L.w.size());
+        if (want_out) {
+            const uint64_t ao = out->out_ptr[first[c]];
+            if (!L.out_ids.empty()) std::memcpy(out->out_dst + ao, L.out_ids.data(), 4 * L.out_ids.size());
+            if (weighted && !L.out_w.empty())
+                std::memcpy(out->out_weight + ao, L.out_w.data(), 8 * L.out_w.size());
         }
-        for (uint64_t u = first[c]; u < first[c + 1]; ++u) {
-            const uint64_t nout = a.varint(), nin = t.varint();
-            uint64_t k = out->colptr[u];
-            uint64_t ko = want_out ? out->out_ptr[u] : 0;
-            for (uint64_t j = 0; j < nout; ++j) {
-                const uint64_t v = a.varint();
-                const double w = fmt != kNone ? aw.weight(fmt) : 1.0;
-                if (directed) {
-                    if (want_out) {
-                        out->out_dst[ko] = (int32_t)v;
-                        if (fmt != kNone) out->out_weight[ko] = w;
-                        ++ko;
-                    }
-                } else {
-                    NKPR_CHECK(k < out->colptr[u + 1], "nkbg lists changed between passes");
-                    out->src[k] = (int32_t)v;
-                    if (fmt != kNone) out->weight[k] = w;
-                    ++k;
-                }
-            }
-            for (uint64_t j = 0; j < nin; ++j) {
-                const uint64_t v = t.varint();
-                const double w = fmt != kNone ? tw.weight(fmt) : 1.0;
-                if (!directed && v == u) continue;
-                NKPR_CHECK(k < out->colptr[u + 1], "nkbg lists changed between passes");
-                out->src[k] = (int32_t)v;
-                if (fmt != kNone) out->weight[k] = w;
-                ++k;
-            }
-        }
+        std::vector<int32_t>().swap(L.ids);  // released as soon as copied
+        std::vector<double>().swap(L.w);
+        std::vector<int32_t>().swap(L.out_ids);
+        std::vector<double>().swap(L.out_w);
     });
     for (uint64_t c = 0; c < chunks; ++c) out->self_loops += loops[c];
 }
