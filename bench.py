@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="time N plain FP32 and N plain FP64 sweeps only (profiling runs)")
     ap.add_argument("--workload", choices=["pagerank", "rhg", "ppr", "graphs", "adjoint",
-                                           "algebraic", "apr", "nkbg"],
+                                           "algebraic", "apr", "nkbg", "power"],
                     default="pagerank",
                     help="pagerank: one solve on R-MAT (the benchmark line); rhg: the same on "
                          "an undirected random hyperbolic graph; ppr: batched personalised "
@@ -67,7 +67,8 @@ def main():
                          "of differentiable PageRank on R-MAT scale 20; algebraic: "
                          "AlgebraicPageRank on NetworKit's CSR matrix of the R-MAT graph; apr: "
                          "ApproximatePageRank pushes on the random hyperbolic graph; nkbg: a "
-                         "NetworKit binary file of the R-MAT graph read into device lists")
+                         "NetworKit binary file of the R-MAT graph read into device lists; "
+                         "power: Eigenvector and Katz centrality on the R-MAT graph")
     ap.add_argument("--rhg-nodes", type=int, default=10_000_000)
     ap.add_argument("--apr-eps", type=float, default=1e-9)
     ap.add_argument("--nkbg-scale", type=int, default=24)
@@ -97,6 +98,9 @@ def main():
         return
     if args.workload == "nkbg":
         bench_nkbg(args)
+        return
+    if args.workload == "power":
+        bench_power(args)
         return
 
     import torch
@@ -815,6 +819,106 @@ def bench_apr(args):
                         "kernel": "whole run (push rounds, frontier swaps, compaction)",
                         "bytes_per_sweep": bytes_run, "ms_per_sweep": sec * 1e3},
            "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def bench_power(args):
+    """EigenvectorCentrality (EigenvectorCentrality.cpp:21-66, tol 1e-9, its default) and
+    KatzCentrality (KatzCentrality.cpp:28-76: alpha = 1 / (1 + max out-degree), beta 0.1, tol
+    1e-8, in-edges, its defaults) on the benchmark's R-MAT graph, through nkpr_power_run on a
+    handle with unit out-degrees (its coefficients are the weights).  One step = one
+    eigenvector run and one Katz run from the reference's start vectors.  Metric: edge-updates
+    per second over both (m x sweeps / time).  The dominant kernel is the FP64 sweep (binned
+    at this size), timed with HIP events on the library's stream."""
+    import torch
+
+    from networkit_amd import capi
+    from networkit_amd import generators as gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    g = gen.rmat(args.scale, args.edge_factor, seed=2, device="cuda")
+    z = g["z"]
+    colptr, src, w = g["colptr"], g["src"], g["weight"]
+    m = src.numel()
+    ones = torch.ones(z, dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    dg = capi.DeviceGraph(z, colptr, src, w, out_wdeg=ones, device=True)
+    prep = time.perf_counter() - t0
+    outdeg = torch.bincount(src.to(torch.int64), minlength=z)
+    alpha = 1.0 / (1.0 + float(outdeg.max().item()))
+    del outdeg
+    x = torch.empty(z, dtype=torch.float64, device=dev)
+
+    def runs():
+        se = dg.power_run(x, capi.default_power_params(kind=capi.NKPR_EIGENVECTOR, tol=1e-9,
+                                                       scores_on_device=1))
+        sk = dg.power_run(x, capi.default_power_params(kind=capi.NKPR_KATZ, alpha=alpha,
+                                                       beta=0.1, tol=1e-8, scores_on_device=1))
+        return se, sk
+
+    for _ in range(args.warmup):
+        runs()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        stats.append(runs())
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / args.steps
+    se, sk = stats[-1]
+    sweeps = se["sweeps_fp64"] + sk["sweeps_fp64"]
+    ms_sweep = sum(a["ms_sweeps_fp64"] + b["ms_sweeps_fp64"] for a, b in stats) / \
+        sum(a["sweeps_fp64"] + b["sweeps_fp64"] for a, b in stats)
+    bytes_sweep = algorithmic_bytes(m, z, True, "fp64")
+    ach = bytes_sweep / (ms_sweep / 1e3) / 1e9
+    log("power: eigenvector %d sweeps, Katz %d sweeps (alpha %.3g); %.1f ms per step, %.3f ms per "
+        "sweep = %.0f GB/s; handle built in %.2f s" % (se["sweeps_fp64"], sk["sweeps_fp64"], alpha,
+                                                        sec * 1e3, ms_sweep, ach, prep))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        og = host_in_lists(colptr, src, w)
+        threads = int(oracle.num_threads())
+        t = time.perf_counter()
+        oracle.katz(og, alpha=alpha, beta=0.1, tol=0.0, max_iterations=1)
+        t1 = time.perf_counter() - t
+        k = int(max(2, min(40, args.cpu_seconds / max(t1, 1e-3))))
+        t = time.perf_counter()
+        oracle.katz(og, alpha=alpha, beta=0.1, tol=0.0, max_iterations=k)
+        tk = time.perf_counter() - t
+        per = max((tk - t1) / (k - 1), 1e-9)
+        est = max(t1 - per, 0.0) + sweeps * per
+        log("cpu baseline: Katz run() capped at 1 sweep %.2f s, at %d sweeps %.2f s on %d threads; "
+            "%.3f s per sweep" % (t1, k, tk, threads, per))
+        cpu = {"value": m * sweeps / est, "unit": "edge-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": "the oracle's KatzCentrality::run (the restatement of "
+                         "KatzCentrality.cpp:28-76; the eigenvector loop is the same sweep) on "
+                         "the same graph capped at %d sweeps (%.2f s) and at 1 (%.2f s) on %d "
+                         "host threads; value = m x %d sweeps (the GPU step's) / (set-up + %d x "
+                         "the time per sweep)" % (k, tk, t1, threads, sweeps, sweeps),
+               "ms_per_sweep": per * 1e3, "host": host_info()}
+    out = {"metric": "Eigenvector + Katz centrality edge-updates/s (FP64, reference stop rules)",
+           "value": m * sweeps / sec, "unit": "edge-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+           "data": "synthetic",
+           "config": {"workload": "EigenvectorCentrality (tol 1e-9) and KatzCentrality (alpha = "
+                                  "1/(1 + max out-degree), beta 0.1, tol 1e-8) on a directed "
+                                  "R-MAT graph, scale %d, edge factor %d, a/b/c/d=.57/.19/.19/.05,"
+                                  " duplicates and self-loops removed, FP32 weights U(0.5,1.5), "
+                                  "seed 2" % (args.scale, args.edge_factor),
+                      "n": z, "m": m, "eigenvector_sweeps": se["sweeps_fp64"],
+                      "katz_sweeps": sk["sweeps_fp64"], "katz_alpha": alpha},
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "FP64 sweep of the unit-out-degree handle (binned: "
+                                  "bin_scatter64_kernel + bin_gather64_kernel + "
+                                  "bin_reduce64_kernel), 20 m + 40 n bytes",
+                        "bytes_per_sweep": bytes_sweep, "ms_per_sweep": ms_sweep},
+           "cpu_baseline": cpu}
+    dg.close()
     print(json.dumps(out), flush=True)
 
 

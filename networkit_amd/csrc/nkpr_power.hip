@@ -245,9 +245,10 @@ void power_run_impl(nkpr_graph *g, const nkpr_power_params *p, double *scores, n
         };
         start_ones();
         Vecs<double> v{{x0, x1}, {xh0, xh1}};
+        // the sweeps are timed with HIP events (ms_sweeps_fp64), the norm hooks are not
         const uint64_t total = run_phase<double>(rc, a, v, kind, never,
-                                                 std::max<uint64_t>(p->max_iterations, 1), false,
-                                                 &unused, hook);
+                                                 std::max<uint64_t>(p->max_iterations, 1), true,
+                                                 &st->ms_sweeps_fp64, hook);
         PowerState hs{};
         read_back(s, {{&hs, ps, sizeof(PowerState)}});
         NKPR_CHECK(!hs.bad, "the iteration diverged (the norm is not finite): alpha must be "
