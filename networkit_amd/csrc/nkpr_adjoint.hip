@@ -351,6 +351,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
                                               (uint64_t)std::max(cus, 1) * 8);
     const bool timing = p->timing != 0;
     Krylov kr = krylov_setup(a);
+    a->op_pending = false;
     // w = A v (residual updates) and w = B v (the GMRES Krylov operator), A = I - B
     auto apply = [&](const float *v, float *w) {
         apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32, true);
@@ -440,6 +441,7 @@ void adjoint_impl(nkpr_graph *g, const nkpr_params *p, const nkpr_csc *csc,
             launch_add_correction(lam, kr.e, n, s);
         }
     }
+    collect_op_timing(a, &st->ms_sweeps_fp32);
     st->iterations = total;
 
     // ---- mu = P^T lambda (one FP64 sweep), back to slots ----------------------------------

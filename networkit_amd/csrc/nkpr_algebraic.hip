@@ -180,7 +180,7 @@ void algebraic_run_impl(nkpr_algebraic *h, double damp, double tol, uint64_t max
         ensure(g->zbuf, 8 * (size_t)n);
         sz = g->zbuf.as<double>();
     }
-    launch_scatter_result(v.x[total & 1], g->perm.as<int32_t>(), n, sum, sz, s);
+    launch_gather_result(v.x[total & 1], slot_index(g), n, sum, sz, s);
     read_back(s, {{scores, sz, scores_on_device ? 0 : 8 * (size_t)n}});
     if (iterations) *iterations = total;
     if (stats) {

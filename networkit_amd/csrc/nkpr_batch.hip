@@ -515,16 +515,6 @@ __global__ __launch_bounds__(256) void batch_scatter_kernel(const double *x64, c
     }
 }
 
-// slot_u[slot] = compact id of the node in that slot, -1 for slots without one
-__global__ void slot_u_kernel(uint64_t z, int32_t *slot_u) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z; i += (uint64_t)gridDim.x * blockDim.x)
-        slot_u[i] = -1;
-}
-__global__ void slot_u_set_kernel(const int32_t *perm, uint32_t n, int32_t *slot_u) {
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
-        slot_u[perm[u]] = (int32_t)u;
-}
-
 __global__ void f64_to_f32_kernel(const double *in, float *out, uint64_t cnt) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
 }
@@ -1107,17 +1097,10 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
         sz = g->b_zbuf.as<double>();
         NKPR_HIP(hipMemcpyAsync(sz, scores, 8 * (size_t)z * k, hipMemcpyHostToDevice, s));
     }
-    if (!g->b_slot_ready) {
-        ensure_buf(g->b_slot_u, 4 * std::max<uint64_t>(z, 1));
-        hipLaunchKernelGGL(slot_u_kernel, dim3(grid_for(z)), dim3(256), 0, s, z, g->b_slot_u.as<int32_t>());
-        hipLaunchKernelGGL(slot_u_set_kernel, dim3(grid_for(n)), dim3(256), 0, s, g->perm.as<int32_t>(), n,
-                           g->b_slot_u.as<int32_t>());
-        NKPR_HIP(hipGetLastError());
-        g->b_slot_ready = true;
-    }
+    const int32_t *slot_u = slot_index(g);
     const uint64_t gather_tiles = ((z + kTile - 1) / kTile) * ((ld + kTile - 1) / kTile);
     hipLaunchKernelGGL(batch_gather_kernel, dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(gather_tiles, 1), 1u << 16)),
-                       dim3(256), 0, s, (const double *)sz, z, (const int32_t *)g->b_slot_u.as<int32_t>(), k, ld,
+                       dim3(256), 0, s, (const double *)sz, z, slot_u, k, ld,
                        x64[0]);
     NKPR_HIP(hipGetLastError());
 
@@ -1271,7 +1254,7 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
     NKPR_HIP(hipMemcpyAsync(g->b_factor.p, factor.data(), 8 * (size_t)k, hipMemcpyHostToDevice, s));
     const uint64_t scatter_tiles = ((z + kTile - 1) / kTile) * ((k + kTile - 1) / kTile);
     hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(scatter_tiles, 1), 1u << 16)),
-                       dim3(256), 0, s, (const double *)result, (const int32_t *)g->b_slot_u.as<int32_t>(), k, ld, z,
+                       dim3(256), 0, s, (const double *)result, slot_u, k, ld, z,
                        g->b_factor.as<double>(), sz);
     NKPR_HIP(hipGetLastError());
     read_back(s, {{scores, sz, p->scores_on_device ? 0 : 8 * (size_t)z * k}});

@@ -87,12 +87,28 @@ void prepare_run(nkpr_graph *g) {
     }
 }
 
+const int32_t *slot_index(nkpr_graph *g) {
+    if (!g->slot_ready) {
+        g->slot_u.alloc(4 * std::max<uint64_t>(g->z, 1));
+        launch_slot_index(g->perm.as<int32_t>(), g->n, g->z, g->slot_u.as<int32_t>(), g->stream);
+        g->slot_ready = true;
+    }
+    return g->slot_u.as<int32_t>();
+}
+
+bool closed_start_enabled() {
+    const char *e = std::getenv("NKPR_CLOSED_START");
+    return !(e && e[0] == '0');
+}
+
 // One solve on compact vectors.  On entry g->x64a holds the FP64 start vector (and g->x32a its
-// FP32 copy for NKPR_FP32).  base64 / base32 (compact, optional) replace the constant
-// teleport (1-d)/n by a per-node right-hand side.  Returns the unnormalised FP64 result and
-// the FP64 sums of its last sweep.
+// FP32 copy for NKPR_FP32), unless `start_const` is given: then the start is that constant on
+// every node and x64a is not read (mixed precision without distributed sinks, closed-form start
+// enabled).  base64 / base32 (compact, optional) replace the constant teleport (1-d)/n by a
+// per-node right-hand side.  Returns the unnormalised FP64 result and the FP64 sums of its last
+// sweep.
 SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64,
-                       const float *base32, nkpr_stats *st) {
+                       const float *base32, nkpr_stats *st, const double *start_const) {
     const uint32_t n = g->n;
     const hipStream_t s = g->stream;
     const double damp = p->damp;
@@ -196,6 +212,7 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         double *ynew = g->x64b.as<double>();   // FP64 sweep output
         double *xh = g->xh64a.as<double>();
         Krylov kr = krylov_setup(g);
+        g->op_pending = false;
         auto apply_b = [&](const float *v, float *w) {
             apply_operator(rc, v, w, kr, timing, &st->ms_sweeps_fp32, false);
         };
@@ -227,10 +244,11 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
         // with y = d P x + b = d c rowsum + b in closed form, from the cached FP64 row sums of
         // P: an O(n) pass instead of an O(m) sweep.  It is not counted as a sweep, and it never
         // certifies (the next residual is a full FP64 sweep).
-        const char *ccenv = std::getenv("NKPR_CLOSED_START");
-        bool closed = !(ccenv && ccenv[0] == '0') && n > 0;
-        double cval = 0.0;
-        if (closed) {
+        NKPR_CHECK(!start_const || (!dist && n > 0 && closed_start_enabled()),
+                   "a constant start needs the closed-form first residual");
+        bool closed = closed_start_enabled() && n > 0;
+        double cval = start_const ? *start_const : 0.0;
+        if (closed && !start_const) {
             ensure(g->flag_buf, 4);
             uint32_t *flag = g->flag_buf.as<uint32_t>();
             launch_constant_check(xcur, n, flag, s);
@@ -290,13 +308,15 @@ SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64
             const double rtol = std::min(0.5, std::max(1e-7, 0.5 * target / std::max(c.l1, 1e-300)));
             const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;  // keep one FP64 sweep
             const uint64_t kc = gmres_f32(krh, apply_b, krh.V, krh.e, rtol, left, s, n);
+            collect_op_timing(g, &st->ms_sweeps_fp32);
             if (std::getenv("NKPR_DEBUG"))
                 std::fprintf(stderr, "[nkpr] refinement %d: residual l1 %.3e, bound %.3e, rtol %.3e, %llu applications\n",
                              round, c.l1, c.bound, rtol, (unsigned long long)kc);
             st->sweeps_fp32 += kc;
             st->sweeps_fp32_correction += kc;
             total += kc;
-            launch_add_correction(xcur, kr.e, n, s);
+            if (start_const && round == 0) launch_set_correction(xcur, cval, kr.e, n, s);
+            else launch_add_correction(xcur, kr.e, n, s);
         }
     }
     res.total = total;
@@ -326,7 +346,7 @@ Krylov krylov_setup(nkpr_graph *g) {
     const size_t ld = (n + 63) / 64 * 64;  // every vector on a 256-byte boundary
     ensure(g->kbasis, 4 * (size_t)(k.m + 1) * ld);
     ensure(g->kwork, 4 * (size_t)4 * ld);
-    ensure(g->kdot, 8 * ((size_t)kDotPartials + 128));
+    ensure(g->kdot, 8 * ((size_t)kDotPartials + kDotResults));
     k.n = g->n;
     k.ld = ld;
     k.V = g->kbasis.as<float>();
@@ -336,6 +356,8 @@ Krylov krylov_setup(nkpr_graph *g) {
     k.e = k.y + 3 * ld;
     k.dpart = g->kdot.as<double>();
     k.dout = k.dpart + (size_t)kDotPartials;
+    if (!g->kdot_host) NKPR_HIP(hipHostMalloc((void **)&g->kdot_host, 8 * kDotResults, hipHostMallocDefault));
+    k.dhost = g->kdot_host;
     return k;
 }
 
@@ -344,20 +366,26 @@ Krylov krylov_setup(nkpr_graph *g) {
 bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
                   bool minus, bool timing, double *ms_acc) {
     if (!(binned_enabled(g) && binned_layout(g))) return false;
-    while (timing && g->events.size() < 2) {
-        hipEvent_t e;
-        NKPR_HIP(hipEventCreate(&e));
-        g->events.push_back(e);
-    }
-    binned_apply(g, v, w, damp, distribute, minus, timing ? g->events[0] : nullptr,
-                 timing ? g->events[1] : nullptr);
     if (timing) {
-        float ms = 0.f;
-        NKPR_HIP(hipEventSynchronize(g->events[1]));
-        NKPR_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
-        *ms_acc += ms;
+        // the previous application's time first (its events are done by now: the host has
+        // read the dots computed from its output), so that no application waits on the host
+        collect_op_timing(g, ms_acc);
+        for (hipEvent_t &e : g->op_ev)
+            if (!e) NKPR_HIP(hipEventCreate(&e));
     }
+    binned_apply(g, v, w, damp, distribute, minus, timing ? g->op_ev[0] : nullptr,
+                 timing ? g->op_ev[1] : nullptr);
+    g->op_pending = timing;
     return true;
+}
+
+void collect_op_timing(nkpr_graph *g, double *ms_acc) {
+    if (!g->op_pending) return;
+    g->op_pending = false;
+    float ms = 0.f;
+    NKPR_HIP(hipEventSynchronize(g->op_ev[1]));
+    NKPR_HIP(hipEventElapsedTime(&ms, g->op_ev[0], g->op_ev[1]));
+    *ms_acc += ms;
 }
 
 // w = A v = v - B v (minus) or w = B v, B v = d P v: one FP32 sweep with no teleport term
@@ -440,8 +468,25 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
     }
     const double damp = p->damp;
     prepare_run(g);
-    launch_gather_start(sz, g->perm.as<int32_t>(), n, g->x64a.as<double>(),
-                        p->precision == NKPR_FP32 ? g->x32a.as<float>() : nullptr, s);
+    // A mixed run from a constant start (every first run: 1/n) needs no start vector in compact
+    // order: its first residual is taken in closed form and its first correction is written as
+    // c + e.  The check reads the slot-order scores when every slot holds a node (then they are
+    // the values the gather would move), and runs only without distributed sinks (their sums
+    // read the start vector).
+    double cval = 0.0;
+    bool start_const = false;
+    if (p->precision == NKPR_MIXED && z == n && p->sink_handling != NKPR_SINK_DISTRIBUTE &&
+        closed_start_enabled()) {
+        ensure(g->flag_buf, 4);
+        uint32_t *flag = g->flag_buf.as<uint32_t>();
+        launch_constant_check(sz, n, flag, s);
+        uint32_t hf = 1;
+        read_back(s, {{&hf, flag, 4}, {&cval, sz, 8}});
+        start_const = hf == 0;
+    }
+    if (!start_const)
+        launch_gather_start(sz, g->perm.as<int32_t>(), n, g->x64a.as<double>(),
+                            p->precision == NKPR_FP32 ? g->x32a.as<float>() : nullptr, s);
 
     // personalised teleport: b[u] = (1 - d) t[slot of u]
     const double *b64 = nullptr;
@@ -460,13 +505,13 @@ void run_impl(nkpr_graph *g, const nkpr_params *p, double *scores, nkpr_stats *s
         b64 = g->base64.as<double>();
         b32 = g->base32.as<float>();
     }
-    const SolveResult res = solve_core(g, p, b64, b32, st);
+    const SolveResult res = solve_core(g, p, b64, b32, st, start_const ? &cval : nullptr);
 
     // ---- normalisation (PageRank.cpp:73-92) and scatter to slot order -----------------------
     double factor;
     if (!p->normalized) factor = res.sum;
     else factor = (1.0 / (double)n) * ((1.0 - damp) + damp * res.sink);
-    launch_scatter_result(res.x, g->perm.as<int32_t>(), n, factor, sz, s);
+    launch_gather_result(res.x, slot_index(g), z, factor, sz, s);
     double maximum = 1.0;
     if (p->normalized) {
         launch_range_max(res.x, n, g->red_partials.as<double>(), g->red_out.as<double>(), s);

@@ -403,6 +403,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
             double *xcur = g->x64a.as<double>();
             double *ynew = g->x64b.as<double>();
             Krylov kr = krylov_setup(g);
+            g->op_pending = false;
             const int dist = p->sink_handling == NKPR_SINK_DISTRIBUTE;
             // A v = v - d P v - (per graph) d S_q(v) / n_q
             // w = B v = d P v (+ per-graph sink terms), the GMRES Krylov operator (A = I - B)
@@ -492,6 +493,7 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                 const double rtol = std::min(0.5, std::max(1e-7, 0.5 / std::max(worst, 1e-300)));
                 const uint64_t left = (max_it > total + 1) ? max_it - total - 1 : 0;
                 const uint64_t kc = gmres_f32(kr, apply_b, kr.V, kr.e, rtol, left, s);
+                collect_op_timing(g, &st->ms_sweeps_fp32);
                 st->sweeps_fp32 += kc;
                 st->sweeps_fp32_correction += kc;
                 total += kc;

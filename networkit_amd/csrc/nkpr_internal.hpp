@@ -242,6 +242,7 @@ void launch_gather_scaled(const double *in_z, const int32_t *perm, uint32_t n, d
                           double *out64, float *out32, hipStream_t s);
 void launch_to_double(const float *x32, double *x64, uint32_t n, hipStream_t s);
 void launch_add_correction(double *x64, const float *e32, uint32_t n, hipStream_t s);
+void launch_set_correction(double *x64, double c, const float *e32, uint32_t n, hipStream_t s);
 void launch_copy_f32(const float *in, float *out, uint32_t n, hipStream_t s);
 // rowsum[u] = sum of the in-coefficients of u (FP64), from the plain CSC copy
 void launch_rowsum64(const uint32_t *rowptr, uint32_t n, const int32_t *src, const double *coef,
@@ -253,6 +254,11 @@ void launch_constant_check(const double *x, uint32_t n, uint32_t *flag, hipStrea
 unsigned launch_const_resid(const double *rowsum, uint32_t n, double dc, double rhs,
                             const double *base, double c, uint32_t sink_begin, double *ynew,
                             float *r32, double *partials, hipStream_t s);
+// slot_u[slot] = compact id of the node there, -1 for slots without one (z entries)
+void launch_slot_index(const int32_t *perm, uint32_t n, uint64_t z, int32_t *slot_u, hipStream_t s);
+// scores_z[slot] = y[slot_u[slot]] / factor for slots with a node (the others keep their value)
+void launch_gather_result(const double *y, const int32_t *slot_u, uint64_t z, double factor,
+                          double *scores_z, hipStream_t s);
 void launch_scatter_result(const double *y, const int32_t *perm, uint32_t n, double factor,
                            double *scores_z, hipStream_t s);
 // Deterministic two-stage sums / max over x[begin, end).
@@ -318,6 +324,8 @@ void binned_apply(nkpr_graph *g, const float *v, float *w, double damp, bool dis
 // use); binned_sweep64 is one sweep with the epilogue, reductions and stop rule of
 // launch_sweep<double>.
 bool binned64_active(nkpr_graph *g);
+// slot -> compact id of the graph (z entries, -1 for slots without a node), built on first use
+const int32_t *slot_index(nkpr_graph *g);
 void binned_sweep64(nkpr_graph *g, const SweepArgs<double> &a, const FinalizeArgs &f, int kind,
                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 // The finalize step of launch_sweep<double> alone (nkpr_sweep.hip).
@@ -447,9 +455,17 @@ struct nkpr_graph {
     nkpr::DevBuf kbasis;                // FP32 Krylov basis [(m+1) * n]
     nkpr::DevBuf kwork;                 // FP32 scratch [2 * n]
     nkpr::DevBuf kdot;                  // FP64 dot partials and results
+    double *kdot_host = nullptr;        // pinned landing buffer of the dot results
     nkpr::DevBuf ctl;                   // PhaseCtl
     nkpr::PhaseCtl *ctl_host = nullptr; // pinned mirror
     std::vector<hipEvent_t> events;
+    nkpr::DevBuf slot_u;                // int32 [z]: compact id of a slot, -1 without a node
+    bool slot_ready = false;            // (built on first use, see slot_index)
+    // timing of binned operator applications: the last application's events, whose elapsed
+    // time is added by the next application or by collect_op_timing (no host wait per
+    // application)
+    hipEvent_t op_ev[2] = {nullptr, nullptr};
+    bool op_pending = false;
 
     // batched personalised PageRank: plain CSC copy and its row split (built on first use)
     bool b_ready = false, b_split_ready = false, b_coef_ready = false;
@@ -462,8 +478,6 @@ struct nkpr_graph {
     nkpr::DevBuf b_tptr, b_tcol, b_tval32, b_tval64;
     nkpr::DevBuf b_piece32, b_piece64, b_partials, b_col, b_ctl, b_sink_part, b_factor;
     nkpr::DevBuf b_x64a, b_x64b, b_x32a, b_x32b, b_r32, b_zbuf;
-    nkpr::DevBuf b_slot_u;       // int32 [z]: compact id of a slot, -1 without a node
-    bool b_slot_ready = false;
     nkpr::BatchCtl *b_ctl_host = nullptr;
 
     // batch of graphs (nkpr_graph_create_batch): graph q owns compact ids [goff[q], goff[q+1]),
@@ -491,7 +505,10 @@ struct nkpr_graph {
     nkpr_graph &operator=(const nkpr_graph &) = delete;
     ~nkpr_graph() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (hipEvent_t e : op_ev)
+            if (e) (void)hipEventDestroy(e);
         if (ctl_host) (void)hipHostFree(ctl_host);
+        if (kdot_host) (void)hipHostFree(kdot_host);
         if (b_ctl_host) (void)hipHostFree(b_ctl_host);
     }
 };

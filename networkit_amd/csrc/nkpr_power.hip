@@ -260,7 +260,7 @@ void power_run_impl(nkpr_graph *g, const nkpr_power_params *p, double *scores, n
         st->last_step_l1 = st->last_step;
         st->score_sum = hs.len;
         // eigenvector: y is already normalised; Katz: divided by the last norm
-        launch_scatter_result(y, g->perm.as<int32_t>(), n, katz ? hs.len : 1.0, sz, s);
+        launch_gather_result(y, slot_index(g), z, katz ? hs.len : 1.0, sz, s);
         if (!katz) {
             // sign correction on slot 0 (EigenvectorCentrality.cpp:58-63); deleted slots hold 1
             double s0 = 0.0;

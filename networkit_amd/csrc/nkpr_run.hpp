@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <vector>
 
@@ -160,17 +161,24 @@ struct Krylov {
     float *vh, *yh; // prescaled copies (unweighted graphs)
     float *e;       // solution of the last solve
     double *dpart, *dout;
+    double *dhost;  // pinned host copy of dout (kDotResults entries)
 };
+constexpr uint32_t kDotResults = 128;
 
 void prepare_run(nkpr_graph *g);
+bool closed_start_enabled();
 SolveResult solve_core(nkpr_graph *g, const nkpr_params *p, const double *base64,
-                       const float *base32, nkpr_stats *st);
+                       const float *base32, nkpr_stats *st, const double *start_const = nullptr);
 Krylov krylov_setup(nkpr_graph *g);
 void ensure_rowsum64(nkpr_graph *g);
 void apply_operator(RunCtx &rc, const float *v, float *w, const Krylov &kr, bool timing,
                     double *ms_acc, bool minus);
+// With `timing`, a binned application's time reaches *ms_acc when the next application starts
+// or when collect_op_timing runs; every run that times applications calls it at its end (and
+// clears g->op_pending at its start).
 bool apply_binned(nkpr_graph *g, const float *v, float *w, double damp, bool distribute,
                   bool minus, bool timing, double *ms_acc);
+void collect_op_timing(nkpr_graph *g, double *ms_acc);
 
 // GMRES(m) from e = 0 on A e = rhs, A = I - B, until ||A e - rhs||_2 <= rtol ||rhs||_2 or `cap`
 // operator applications.  `apply(v, w)` computes w = B v.  The Krylov spaces of A and B are
@@ -207,14 +215,19 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
     const int mres = k.m;
     const uint64_t ld = k.ld;
     float *const V = k.V;
+    // the results land in pinned memory (a direct copy, no staging through pageable memory)
+    auto fetch = [&](double *host, uint32_t cnt) {
+        read_back(s, {{k.dhost, k.dout, 8 * cnt}});
+        std::memcpy(host, k.dhost, 8 * cnt);
+    };
     auto dots = [&](const float *w, const float *Vb, uint32_t cnt, double *host) {
         launch_multidot(w, Vb, ld, cnt, n, k.dpart, k.dout, s);
-        read_back(s, {{host, k.dout, 8 * cnt}});
+        fetch(host, cnt);
     };
     // w -= V c (cnt vectors), then host[i] = <u_i, w> for i < cnt and host[cnt] = <w, w>
     auto project = [&](float *w, uint32_t cnt, const double *c, double *host) {
         launch_axpy_dot(w, V, ld, cnt, n, c, k.dpart, k.dout, s);
-        read_back(s, {{host, k.dout, 8 * (cnt + 1)}});
+        fetch(host, cnt + 1);
     };
     const auto in_range = [](double nrm) { return nrm >= 0x1p-50 && nrm <= 0x1p50; };
     double beta2 = 0.0;

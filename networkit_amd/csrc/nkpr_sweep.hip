@@ -485,6 +485,12 @@ __global__ void add_correction_kernel(double *x64, const float *e32, uint32_t n)
     if (i < n) x64[i] += (double)e32[i];
 }
 
+// x64 = c + e32: the correction of a constant iterate that was never written out
+__global__ void set_correction_kernel(double *x64, double c, const float *e32, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x64[i] = c + (double)e32[i];
+}
+
 // rowsum[u] = the FP64 sum of row u's in-coefficients, c_vu = w_vu / deg_v (coef) or 1 / deg_v
 // (inv[v]): one wave per row, lanes strided over the row, then a fixed shuffle tree.
 __global__ void rowsum_kernel(const uint32_t *rowptr, uint32_t n, const int32_t *src,
@@ -552,6 +558,28 @@ __global__ void scatter_result_kernel(const double *y, const int32_t *perm, uint
                                       double factor, double *scores_z) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) scores_z[perm[i]] = y[i] / factor;
+}
+
+// slot_u[slot] = compact id of the node in that slot, -1 for slots without one
+__global__ void slot_clear_kernel(uint64_t z, int32_t *slot_u) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        slot_u[i] = -1;
+}
+__global__ void slot_set_kernel(const int32_t *perm, uint32_t n, int32_t *slot_u) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+        slot_u[perm[u]] = (int32_t)u;
+}
+
+// The scatter of scatter_result_kernel as a gather in slot order: coalesced writes, and the
+// reads of y that are not in order (the nodes with out-edges, y's head) stay in the caches.
+// Slots without a node keep their value.
+__global__ void gather_result_kernel(const double *y, const int32_t *slot_u, uint64_t z,
+                                     double factor, double *scores_z) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= z) return;
+    const int32_t i = slot_u[j];
+    if (i >= 0) scores_z[j] = y[i] / factor;
 }
 
 constexpr int kRedBlocks = 1024;
@@ -740,6 +768,12 @@ void launch_add_correction(double *x64, const float *e32, uint32_t n, hipStream_
     NKPR_HIP(hipGetLastError());
 }
 
+void launch_set_correction(double *x64, double c, const float *e32, uint32_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(set_correction_kernel, dim3(grid1d(n)), dim3(256), 0, s, x64, c, e32, n);
+    NKPR_HIP(hipGetLastError());
+}
+
 void launch_rowsum64(const uint32_t *rowptr, uint32_t n, const int32_t *src, const double *coef,
                      const double *inv, double *rowsum, hipStream_t s) {
     if (!n) return;
@@ -769,6 +803,24 @@ unsigned launch_const_resid(const double *rowsum, uint32_t n, double dc, double 
 void launch_copy_f32(const float *in, float *out, uint32_t n, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(copy_f32_kernel, dim3(grid1d(n)), dim3(256), 0, s, in, out, n);
+    NKPR_HIP(hipGetLastError());
+}
+
+void launch_slot_index(const int32_t *perm, uint32_t n, uint64_t z, int32_t *slot_u, hipStream_t s) {
+    if (!z) return;
+    hipLaunchKernelGGL(slot_clear_kernel, dim3((unsigned)std::min<uint64_t>((z + 255) / 256, 1u << 16)),
+                       dim3(256), 0, s, z, slot_u);
+    if (n)
+        hipLaunchKernelGGL(slot_set_kernel, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16)),
+                           dim3(256), 0, s, perm, n, slot_u);
+    NKPR_HIP(hipGetLastError());
+}
+
+void launch_gather_result(const double *y, const int32_t *slot_u, uint64_t z, double factor,
+                          double *scores_z, hipStream_t s) {
+    if (!z) return;
+    hipLaunchKernelGGL(gather_result_kernel, dim3((unsigned)((z + 255) / 256)), dim3(256), 0, s, y,
+                       slot_u, z, factor, scores_z);
     NKPR_HIP(hipGetLastError());
 }
 
