@@ -712,6 +712,44 @@ def test_mixed_closed_form_start(monkeypatch, weighted, sinks, teleport):
     assert warm["sweeps_fp64"] == warm["refinements"] + 1  # every residual was a sweep
 
 
+@pytest.mark.parametrize("on_device", [False, True])
+def test_constant_start_without_gather_is_bit_identical(on_device):
+    """A mixed run from a constant start on a graph whose every slot holds a node checks the
+    start once in slot order and never gathers it (its first correction is written as c + e);
+    the same graph with a deleted slot appended takes the gathered start.  Both runs give the
+    same bits on every node, the deleted slot keeps its value, and the result matches the
+    converged oracle."""
+    import torch
+    z, u, v, w = skewed_graph(20000, 150000, 5, weighted=True)
+    ex = np.ones(z + 1, np.uint8)
+    ex[z] = 0
+    ref, _, _ = oracle.pagerank(oracle.in_lists(z, u, v, w, True), tol=1e-15, norm=oracle.L1,
+                                max_iterations=20000)
+    out = []
+    for zz, exists in ((z, None), (z + 1, ex)):
+        g = oracle.in_lists(zz, u, v, w, True, exists=exists)
+        dg = capi.DeviceGraph(zz, g.in_ptr, g.in_src.astype(np.int32), g.in_w.astype(np.float32),
+                              weight64=g.in_w, exists=exists)
+        p = capi.default_params(precision=capi.NKPR_MIXED, tol=1e-10, stop=capi.NKPR_STOP_L1_ERROR,
+                                scores_on_device=1 if on_device else 0)
+        if on_device:
+            x = torch.full((zz,), 1.0 / z, dtype=torch.float64, device="cuda")
+            st = dg.run(x, p)
+            torch.cuda.synchronize()
+            x = x.cpu().numpy()
+        else:
+            x = np.full(zz, 1.0 / z)
+            st = dg.run(x, p)
+        dg.close()
+        out.append((x, st))
+    (a, sa), (b, sb) = out
+    assert sa["l1_error_bound"] <= 1e-10
+    assert np.abs(a - ref).sum() <= 1e-10
+    assert sa["iterations"] == sb["iterations"] and sa["refinements"] == sb["refinements"]
+    assert np.array_equal(a, b[:z])
+    assert b[z] == 1.0 / z
+
+
 def test_binned_copies_fall_back_when_memory_is_short(monkeypatch):
     """A device without room for the binned copies (forced from the host: the build's first
     allocation is refused) keeps the pull sweep: the FP64 run still has the oracle's iteration
