@@ -52,20 +52,47 @@ __device__ __forceinline__ void block_partials(const double (&acc)[K], double *p
     }
 }
 
+// out[i] = sum over blocks of partials[b][i], i < count, in a fixed order (a strided sum per
+// thread, then the waves in order), so that every run gives the same bits.
+__device__ void final_sums(const double *partials, int blocks, uint32_t count, double *out) {
+    __shared__ double s_red[kThreads / 64];
+    for (uint32_t i = 0; i < count; ++i) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < blocks; b += kThreads) acc += partials[(uint64_t)b * kDotStride + i];
+        acc = wave_sum_d(acc);
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int k = 0; k < kThreads / 64; ++k) r += s_red[k];
+            out[i] = r;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void dot_final_kernel(const double *partials, int blocks,
+                                                             uint32_t count, double *out) {
+    final_sums(partials, blocks, count, out);
+}
+
 // partials[b][i] = sum over this block's elements of w[u] * V[i][u], i < C.  n4 = n / 4 whole
 // quads; block 0 adds the n % 4 tail elements.
-template <int C>
+// SELF: V[C - 1] is w itself, whose square is taken from the loaded w (one pass fewer).
+template <int C, bool SELF>
 __global__ __launch_bounds__(kThreads) void multidot_kernel(const float *__restrict__ w,
                                                             const float *__restrict__ V,
                                                             uint64_t ld, uint32_t n4, uint32_t n,
                                                             double *partials) {
+    constexpr int kRead = SELF ? C - 1 : C;
     double acc[C];
 #pragma unroll
     for (int i = 0; i < C; ++i) acc[i] = 0.0;
     for (uint32_t q = blockIdx.x * kThreads + threadIdx.x; q < n4; q += gridDim.x * kThreads) {
         const f32x4 a = ld4(w, q);
 #pragma unroll
-        for (int i = 0; i < C; ++i) acc[i] += dot4(a, ld4(V + i * ld, q));
+        for (int i = 0; i < kRead; ++i) acc[i] += dot4(a, ld4(V + i * ld, q));
+        if (SELF) acc[C - 1] += dot4(a, a);
     }
     if (blockIdx.x == 0)
         for (uint32_t u = 4 * n4 + threadIdx.x; u < n; u += kThreads)
@@ -110,24 +137,6 @@ __global__ __launch_bounds__(kThreads) void axpy_dot_kernel(float *__restrict__ 
             acc[C] += (double)x * (double)x;
         }
     block_partials<C + 1>(acc, partials);
-}
-
-__global__ __launch_bounds__(kThreads) void dot_final_kernel(const double *partials, int blocks,
-                                                             uint32_t count, double *out) {
-    __shared__ double s_red[kThreads / 64];
-    for (uint32_t i = 0; i < count; ++i) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < blocks; b += kThreads) acc += partials[(uint64_t)b * kDotStride + i];
-        acc = wave_sum_d(acc);
-        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = 0.0;
-            for (int k = 0; k < kThreads / 64; ++k) r += s_red[k];
-            out[i] = r;
-        }
-        __syncthreads();
-    }
 }
 
 // mode 0: w -= sum_i c[i] V[i];  mode 1: w = sum_i ...;  mode 2: w += sum_i ...
@@ -200,9 +209,15 @@ void launch_multidot(const float *w, const float *V, uint64_t ld, uint32_t count
     const unsigned blocks = std::min<unsigned>(kDotBlocks, grid_for(n4));
     for (uint32_t first = 0; first < count; first += kDotGroup) {
         const uint32_t c = std::min<uint32_t>(kDotGroup, count - first);
+        // the group's last vector is w itself (the norm of a new Krylov vector)
+        const bool self = V + (uint64_t)(first + c - 1) * ld == w;
         with_count(c, [&](auto C) {
-            hipLaunchKernelGGL(multidot_kernel<decltype(C)::value>, dim3(blocks), dim3(kThreads), 0, s, w,
-                               V + first * ld, ld, n4, n, partials);
+            if (self)
+                hipLaunchKernelGGL((multidot_kernel<decltype(C)::value, true>), dim3(blocks), dim3(kThreads),
+                                   0, s, w, V + first * ld, ld, n4, n, partials);
+            else
+                hipLaunchKernelGGL((multidot_kernel<decltype(C)::value, false>), dim3(blocks), dim3(kThreads),
+                                   0, s, w, V + first * ld, ld, n4, n, partials);
         });
         NKPR_HIP(hipGetLastError());
         hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, partials,
