@@ -346,11 +346,11 @@ Krylov krylov_setup(nkpr_graph *g) {
     // only pay where convergence needs them; 15 measured best on a slowly mixing graph (random
     // hyperbolic, n = 10M: 30 applications in 25.1 ms against 32 in 27.2 ms with 20), and the
     // refinements of the other workloads end within 15 steps
-    k.m = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 15), 2), 64);
+    k.m = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 15), 2), kMaxKrylov);
     const size_t ld = (n + 63) / 64 * 64;  // every vector on a 256-byte boundary
     ensure(g->kbasis, 4 * (size_t)(k.m + 1) * ld);
     ensure(g->kwork, 4 * (size_t)4 * ld);
-    ensure(g->kdot, 8 * ((size_t)kDotPartials + kDotResults));
+    ensure(g->kdot, 8 * ((size_t)kDotPartials + kDotResults) + 4 * (size_t)kMaxKrylov);
     k.n = g->n;
     k.ld = ld;
     k.V = g->kbasis.as<float>();
@@ -360,8 +360,11 @@ Krylov krylov_setup(nkpr_graph *g) {
     k.e = k.y + 3 * ld;
     k.dpart = g->kdot.as<double>();
     k.dout = k.dpart + (size_t)kDotPartials;
+    k.cdev = reinterpret_cast<float *>(k.dout + kDotResults);
     if (!g->kdot_host) NKPR_HIP(hipHostMalloc((void **)&g->kdot_host, 8 * kDotResults, hipHostMallocDefault));
+    if (!g->kdot_ev) NKPR_HIP(hipEventCreateWithFlags(&g->kdot_ev, hipEventDisableTiming));
     k.dhost = g->kdot_host;
+    k.ev = g->kdot_ev;
     return k;
 }
 

@@ -277,12 +277,20 @@ constexpr int kDotPartials = kDotStride * 1024;
 struct Coefs {
     float c[kDotGroup];
 };
+constexpr int kMaxKrylov = 64;  // largest GMRES restart length
+struct Scales {
+    double s[kMaxKrylov];
+};
 // out[i] = <w, V[i]> (FP64 accumulation) for i < count; partials: kDotPartials doubles
 void launch_multidot(const float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                      double *partials, double *out, hipStream_t s);
 // assign: w = sum_i coef[i] V[i];  otherwise w -= sum_i coef[i] V[i]
 void launch_multiaxpy(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                       const double *coef, bool assign, hipStream_t s);
+// w -= sum_i c_i V[i] with c_i = sig[i] (sig[i] d[i]) formed on the device from the dots d
+// (device memory) and the host's scales sig; cdev: count floats of device scratch
+void launch_gs_project(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
+                       const double *d, const double *sig, float *cdev, hipStream_t s);
 // w -= sum_i coef[i] V[i], then out[i] = <V[i], w> for i < count and out[count] = <w, w>
 void launch_axpy_dot(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                      const double *coef, double *partials, double *out, hipStream_t s);
@@ -456,6 +464,7 @@ struct nkpr_graph {
     nkpr::DevBuf kwork;                 // FP32 scratch [2 * n]
     nkpr::DevBuf kdot;                  // FP64 dot partials and results
     double *kdot_host = nullptr;        // pinned landing buffer of the dot results
+    hipEvent_t kdot_ev = nullptr;       // marks the landing of the dot results
     nkpr::DevBuf ctl;                   // PhaseCtl
     nkpr::PhaseCtl *ctl_host = nullptr; // pinned mirror
     std::vector<hipEvent_t> events;
@@ -509,6 +518,7 @@ struct nkpr_graph {
             if (e) (void)hipEventDestroy(e);
         if (ctl_host) (void)hipHostFree(ctl_host);
         if (kdot_host) (void)hipHostFree(kdot_host);
+        if (kdot_ev) (void)hipEventDestroy(kdot_ev);
         if (b_ctl_host) (void)hipHostFree(b_ctl_host);
     }
 };

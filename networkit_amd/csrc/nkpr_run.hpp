@@ -162,6 +162,8 @@ struct Krylov {
     float *e;       // solution of the last solve
     double *dpart, *dout;
     double *dhost;  // pinned host copy of dout (kDotResults entries)
+    float *cdev;    // projection coefficients formed on the device (kMaxKrylov entries)
+    hipEvent_t ev;  // recorded after a copy of dout to dhost
 };
 constexpr uint32_t kDotResults = 128;
 
@@ -258,8 +260,15 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         float *w = V + (size_t)(j + 1) * ld;
         apply(uj, w);  // w = B u_j, so B v_j = sig[j] w
         ++applies;
-        // <u_i, w> for i <= j and <w, w> (w is slot j + 1)
-        dots(w, V, j + 2, d.data());
+        // <u_i, w> for i <= j and <w, w> (w is slot j + 1).  The projection w -= sum_i c_i u_i
+        // with c_i = sig_i <u_i, w> sig_i is formed on the device and runs while the host reads
+        // the dots, so that the device does not wait for the host between the steps.
+        launch_multidot(w, V, ld, j + 2, n, k.dpart, k.dout, s);
+        NKPR_HIP(hipMemcpyAsync(k.dhost, k.dout, 8 * (size_t)(j + 2), hipMemcpyDeviceToHost, s));
+        NKPR_HIP(hipEventRecord(k.ev, s));
+        launch_gs_project(w, V, ld, j + 1, n, k.dout, sig.data(), k.cdev, s);
+        NKPR_HIP(hipEventSynchronize(k.ev));
+        std::memcpy(d.data(), k.dhost, 8 * (size_t)(j + 2));
         const double ww = d[j + 1];
         double proj2 = 0.0;
         for (int i = 0; i <= j; ++i) {
@@ -269,11 +278,9 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         }
         double hn2 = ww - proj2;
         const bool reorth = !(hn2 > reorth_eta2 * ww);
-        if (!reorth) {
-            launch_multiaxpy(w, V, ld, j + 1, n, c.data(), false, s);
-        } else {
-            // both passes with measured dots and norms
-            project(w, j + 1, c.data(), d.data());
+        if (reorth) {
+            // the first pass (done above) measured, then the second pass
+            dots(w, V, j + 2, d.data());
             for (int i = 0; i <= j; ++i) {
                 const double h2 = sig[i] * d[i];
                 h[i] += h2;
