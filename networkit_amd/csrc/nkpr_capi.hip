@@ -350,7 +350,7 @@ Krylov krylov_setup(nkpr_graph *g) {
     const size_t ld = (n + 63) / 64 * 64;  // every vector on a 256-byte boundary
     ensure(g->kbasis, 4 * (size_t)(k.m + 1) * ld);
     ensure(g->kwork, 4 * (size_t)4 * ld);
-    ensure(g->kdot, 8 * ((size_t)kDotPartials + kDotResults) + 4 * (size_t)kMaxKrylov);
+    ensure(g->kdot, 8 * ((size_t)kDotPartials + kDotResults));
     k.n = g->n;
     k.ld = ld;
     k.V = g->kbasis.as<float>();
@@ -360,7 +360,6 @@ Krylov krylov_setup(nkpr_graph *g) {
     k.e = k.y + 3 * ld;
     k.dpart = g->kdot.as<double>();
     k.dout = k.dpart + (size_t)kDotPartials;
-    k.cdev = reinterpret_cast<float *>(k.dout + kDotResults);
     if (!g->kdot_host) NKPR_HIP(hipHostMalloc((void **)&g->kdot_host, 8 * kDotResults, hipHostMallocDefault));
     if (!g->kdot_ev) NKPR_HIP(hipEventCreateWithFlags(&g->kdot_ev, hipEventDisableTiming));
     k.dhost = g->kdot_host;

@@ -288,9 +288,9 @@ void launch_multidot(const float *w, const float *V, uint64_t ld, uint32_t count
 void launch_multiaxpy(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                       const double *coef, bool assign, hipStream_t s);
 // w -= sum_i c_i V[i] with c_i = sig[i] (sig[i] d[i]) formed on the device from the dots d
-// (device memory) and the host's scales sig; cdev: count floats of device scratch
+// (device memory) and the host's scales sig
 void launch_gs_project(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
-                       const double *d, const double *sig, float *cdev, hipStream_t s);
+                       const double *d, const double *sig, hipStream_t s);
 // w -= sum_i coef[i] V[i], then out[i] = <V[i], w> for i < count and out[count] = <w, w>
 void launch_axpy_dot(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
                      const double *coef, double *partials, double *out, hipStream_t s);

@@ -165,23 +165,22 @@ __global__ __launch_bounds__(kThreads) void multiaxpy_kernel(float *__restrict__
         }
 }
 
-// The projection coefficients of a Gram-Schmidt step on the device: c[i] = sig_i (sig_i d[i])
-// for i < count (the same FP64 operations as on the host, rounded to FP32 as the host rounds
-// them), so that the projection needs no host read of the dots.
-__global__ void gs_coefs_kernel(const double *d, Scales sig, uint32_t count, float *c) {
-    const uint32_t i = threadIdx.x;
-    if (i < count) c[i] = (float)(sig.s[i] * (sig.s[i] * d[i]));
-}
-
-// w -= sum_i c[i] V[i] with the coefficients read from device memory
+// w -= sum_i c_i V[i] with the projection coefficients of a Gram-Schmidt step formed on the
+// device: c_i = sig_i (sig_i d[i]) from the dots d in device memory (the host's FP64 operations,
+// rounded to FP32 as the host rounds them), so that the projection needs no host read of the
+// dots.  sig holds the scales of the whole basis; this launch covers vectors first .. first + C.
 template <int C>
 __global__ __launch_bounds__(kThreads) void multiaxpy_dev_kernel(float *__restrict__ w,
                                                                  const float *__restrict__ V,
                                                                  uint64_t ld, uint32_t n4, uint32_t n,
-                                                                 const float *__restrict__ cdev) {
+                                                                 const double *__restrict__ d,
+                                                                 Scales sig, uint32_t first) {
     float c[C];
 #pragma unroll
-    for (int i = 0; i < C; ++i) c[i] = cdev[i];
+    for (int i = 0; i < C; ++i) {
+        const double si = sig.s[first + i];
+        c[i] = (float)(si * (si * d[first + i]));
+    }
     for (uint32_t q = blockIdx.x * kThreads + threadIdx.x; q < n4; q += gridDim.x * kThreads) {
         f32x4 s = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -278,20 +277,18 @@ void launch_multiaxpy(float *w, const float *V, uint64_t ld, uint32_t count, uin
 }
 
 void launch_gs_project(float *w, const float *V, uint64_t ld, uint32_t count, uint32_t n,
-                       const double *d, const double *sig, float *cdev, hipStream_t s) {
+                       const double *d, const double *sig, hipStream_t s) {
     NKPR_CHECK(count >= 1 && count <= (uint32_t)kMaxKrylov, "Gram-Schmidt step size out of range");
     check_aligned(w, ld);
     check_aligned(V, ld);
     Scales sc{};
     for (uint32_t i = 0; i < count; ++i) sc.s[i] = sig[i];
-    hipLaunchKernelGGL(gs_coefs_kernel, dim3(1), dim3(kMaxKrylov), 0, s, d, sc, count, cdev);
-    NKPR_HIP(hipGetLastError());
     const uint32_t n4 = n / 4;
     for (uint32_t first = 0; first < count; first += kDotGroup) {
         const uint32_t c = std::min<uint32_t>(kDotGroup, count - first);
         with_count(c, [&](auto C) {
             hipLaunchKernelGGL(multiaxpy_dev_kernel<decltype(C)::value>, dim3(grid_for(n4)), dim3(kThreads),
-                               0, s, w, V + first * ld, ld, n4, n, (const float *)(cdev + first));
+                               0, s, w, V + first * ld, ld, n4, n, d, sc, first);
         });
         NKPR_HIP(hipGetLastError());
     }

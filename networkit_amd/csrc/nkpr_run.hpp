@@ -162,7 +162,6 @@ struct Krylov {
     float *e;       // solution of the last solve
     double *dpart, *dout;
     double *dhost;  // pinned host copy of dout (kDotResults entries)
-    float *cdev;    // projection coefficients formed on the device (kMaxKrylov entries)
     hipEvent_t ev;  // recorded after a copy of dout to dhost
 };
 constexpr uint32_t kDotResults = 128;
@@ -266,7 +265,7 @@ uint64_t gmres_f32(const Krylov &k, Apply &&apply, const float *rhs, float *e, d
         launch_multidot(w, V, ld, j + 2, n, k.dpart, k.dout, s);
         NKPR_HIP(hipMemcpyAsync(k.dhost, k.dout, 8 * (size_t)(j + 2), hipMemcpyDeviceToHost, s));
         NKPR_HIP(hipEventRecord(k.ev, s));
-        launch_gs_project(w, V, ld, j + 1, n, k.dout, sig.data(), k.cdev, s);
+        launch_gs_project(w, V, ld, j + 1, n, k.dout, sig.data(), s);
         NKPR_HIP(hipEventSynchronize(k.ev));
         std::memcpy(d.data(), k.dhost, 8 * (size_t)(j + 2));
         const double ww = d[j + 1];
