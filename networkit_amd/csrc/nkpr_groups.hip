@@ -166,6 +166,13 @@ __global__ void group_scatter_kernel(const Chunk *ch, const double *x, const int
     }
 }
 
+// first[k] = the score in graph k's first slot (0 for a graph without slots)
+__global__ void group_first_kernel(const double *scores, const uint64_t *gslot, uint32_t groups,
+                                   double *first) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < groups) first[k] = gslot[k] < gslot[k + 1] ? scores[gslot[k]] : 0.0;
+}
+
 __global__ void group_init_kernel(GroupCtl *gc, uint32_t groups, uint32_t *flag) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < groups) {
@@ -289,12 +296,14 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
         uint32_t *tail = (uint32_t *)((char *)g->q_chunks.p + sizeof(Chunk) * ch.size());
         NKPR_HIP(hipMemcpyAsync(tail, cfirst.data(), 4 * ((size_t)G + 1), hipMemcpyHostToDevice, s));
         NKPR_HIP(hipMemcpyAsync(tail + G + 1, gn.data(), 4 * (size_t)G, hipMemcpyHostToDevice, s));
+        g->q_gslot.alloc(8 * ((size_t)G + 1));
+        NKPR_HIP(hipMemcpyAsync(g->q_gslot.p, g->gslot.data(), 8 * ((size_t)G + 1), hipMemcpyHostToDevice, s));
         NKPR_HIP(hipStreamSynchronize(s));
     }
     ensure(g->q_part, 8 * 4 * std::max<size_t>(g->q_nchunks, 1));
     ensure(g->q_ctl, sizeof(GroupCtl) * G);
     ensure(g->q_add, 8 * (size_t)G);
-    ensure(g->q_factor, 8 * (size_t)G + 8 * std::max<size_t>(g->q_nchunks, 1));
+    ensure(g->q_factor, 16 * (size_t)G + 8 * std::max<size_t>(g->q_nchunks, 1));
     ensure(g->q_flag, 16);
     GroupRun q{};
     q.g = g;
@@ -523,18 +532,19 @@ void run_batch_impl(nkpr_graph *g, const nkpr_params *p, double *scores, uint64_
                            result, g->perm.as<int32_t>(), (const double *)fdev, sz, mdev);
         NKPR_HIP(hipGetLastError());
         std::vector<double> cmax(q.nch);
-        read_back(s, {{cmax.data(), mdev, 8 * (size_t)q.nch}});
         if (maximum) {
             // maximum(): 1 when not normalised; otherwise the largest score, seeded with the
-            // graph's first slot (PageRank.cpp:95-102)
+            // graph's first slot (PageRank.cpp:95-102); the first slots are gathered on the
+            // device and read with the chunk maxima in one copy
+            double *first_dev = mdev + std::max<size_t>(q.nch, 1);
+            hipLaunchKernelGGL(group_first_kernel, dim3((G + 255) / 256), dim3(256), 0, s, (const double *)sz,
+                               (const uint64_t *)g->q_gslot.as<uint64_t>(), G, first_dev);
+            NKPR_HIP(hipGetLastError());
             std::vector<Chunk> ch(q.nch);
-            read_back(s, {{ch.data(), q.chunks, sizeof(Chunk) * q.nch}});
             std::vector<double> first(G, 0.0);
-            std::vector<HostCopy> reads;
-            for (uint32_t k = 0; k < G; ++k) {
-                if (g->gslot[k] < g->gslot[k + 1]) reads.push_back({&first[k], sz + g->gslot[k], 8});
-            }
-            read_back(s, reads.data(), reads.size());
+            read_back(s, {{cmax.data(), mdev, 8 * (size_t)q.nch},
+                          {ch.data(), q.chunks, sizeof(Chunk) * q.nch},
+                          {first.data(), first_dev, 8 * (size_t)G}});
             for (uint32_t k = 0; k < G; ++k) maximum[k] = p->normalized ? first[k] : 1.0;
             if (p->normalized)
                 for (uint32_t c = 0; c < q.nch; ++c)

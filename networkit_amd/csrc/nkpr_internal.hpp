@@ -495,6 +495,7 @@ struct nkpr_graph {
     std::vector<uint32_t> goff, gsink;
     std::vector<uint64_t> gslot;
     nkpr::DevBuf q_chunks, q_part, q_ctl, q_add, q_factor, q_flag;
+    nkpr::DevBuf q_gslot;           // uint64 [n_groups + 1]: gslot on the device
     uint32_t q_nchunks = 0;
 
     // binned copies of the edges (nkpr_binned.hip, built on first use): the FP32 operator of
