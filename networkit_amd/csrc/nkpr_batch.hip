@@ -515,6 +515,22 @@ __global__ __launch_bounds__(256) void batch_scatter_kernel(const double *x64, c
     }
 }
 
+// tptr[r] = first teleport entry of row r, r <= n: the offset of the first listed row >= r
+// (urow ascending, nu rows; uoff[nu] = the entry count)
+__global__ void tele_rowptr_kernel(const uint32_t *urow, const uint32_t *uoff, uint32_t nu,
+                                   uint32_t n, uint32_t *tptr) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= n;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nu;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (urow[mid] < r) lo = mid + 1;
+            else hi = mid;
+        }
+        tptr[r] = uoff[lo];
+    }
+}
+
 __global__ void f64_to_f32_kernel(const double *in, float *out, uint64_t cnt) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
 }
@@ -721,6 +737,8 @@ void batch_layout(nkpr_graph *g) {
     g->b_perm_host.resize(n);
     read_back(s, {{g->b_rowptr_host.data(), g->b_rowptr.p, 4 * ((size_t)n + 1)},
                   {g->b_perm_host.data(), g->perm.p, 4 * (size_t)n}});
+    g->b_iperm_host.assign(g->z, -1);
+    for (uint32_t u = 0; u < n; ++u) g->b_iperm_host[(uint32_t)g->b_perm_host[u]] = (int32_t)u;
     g->b_ready = true;
     g->b_coef_ready = true;
 }
@@ -1018,8 +1036,11 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
     batch_layout(g);
 
     // ---- row-sparse teleport in relabelled ids ----------------------------------------------
-    std::vector<int32_t> iperm(z, -1);
-    for (uint32_t u = 0; u < n; ++u) iperm[(uint32_t)g->b_perm_host[u]] = (int32_t)u;
+    // Entries (row, column, weight) in column order, sorted stably by row (so columns ascend
+    // within a row and repeated seeds of a column stay in seed order), repeated (row, column)
+    // pairs merged in that order; the rows with entries and their offsets go to the device,
+    // which expands them into the row pointer.
+    const std::vector<int32_t> &iperm = g->b_iperm_host;
     struct Ent {
         uint32_t row, col;
         double w;
@@ -1041,33 +1062,54 @@ void ppr_batch_impl(nkpr_graph *g, const nkpr_params *p, uint32_t k, const uint6
         for (uint64_t i = seed_off[j]; i < seed_off[j + 1]; ++i)
             ent.push_back({(uint32_t)iperm[seeds[i]], j, (1.0 - d) * (seed_w ? seed_w[i] : 1.0) / wsum});
     }
-    std::sort(ent.begin(), ent.end(), [](const Ent &a, const Ent &b) {
-        return a.row != b.row ? a.row < b.row : a.col < b.col;
-    });
-    // merge repeated (row, col) pairs in a fixed order
-    std::vector<uint32_t> tptr((size_t)n + 1, 0), tcol;
+    {
+        // stable LSD radix sort by row, 11 bits per pass
+        std::vector<Ent> tmp(ent.size());
+        for (int shift = 0; shift < 32; shift += 11) {
+            std::vector<uint32_t> cnt(2049, 0);
+            for (const Ent &e : ent) ++cnt[((e.row >> shift) & 2047u) + 1];
+            if (cnt[((ent.empty() ? 0u : (ent[0].row >> shift) & 2047u)) + 1] == ent.size()) continue;
+            for (int b = 0; b < 2048; ++b) cnt[b + 1] += cnt[b];
+            for (const Ent &e : ent) tmp[cnt[(e.row >> shift) & 2047u]++] = e;
+            ent.swap(tmp);
+        }
+    }
+    std::vector<uint32_t> urow, uoff, tcol;
     std::vector<double> tv64;
-    std::vector<float> tv32;
     for (size_t i = 0; i < ent.size(); ++i) {
-        if (!tcol.empty() && i > 0 && ent[i].row == ent[i - 1].row && ent[i].col == ent[i - 1].col) {
+        if (i > 0 && ent[i].row == ent[i - 1].row && ent[i].col == ent[i - 1].col) {
             tv64.back() += ent[i].w;
             continue;
         }
+        if (urow.empty() || urow.back() != ent[i].row) {
+            urow.push_back(ent[i].row);
+            uoff.push_back((uint32_t)tcol.size());
+        }
         tcol.push_back(ent[i].col);
         tv64.push_back(ent[i].w);
-        tptr[ent[i].row + 1]++;
     }
-    for (uint32_t u = 0; u < n; ++u) tptr[u + 1] += tptr[u];
-    tv32.resize(tv64.size());
+    uoff.push_back((uint32_t)tcol.size());
+    std::vector<float> tv32(tv64.size());
     for (size_t i = 0; i < tv64.size(); ++i) tv32[i] = (float)tv64[i];
     auto up = [&](DevBuf &b, const void *h, size_t bytes) {
         b.alloc(std::max<size_t>(bytes, 16));
         if (bytes) NKPR_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, s));
     };
-    up(g->b_tptr, tptr.data(), 4 * tptr.size());
     up(g->b_tcol, tcol.data(), 4 * tcol.size());
     up(g->b_tval32, tv32.data(), 4 * tv32.size());
     up(g->b_tval64, tv64.data(), 8 * tv64.size());
+    {
+        DevBuf urow_d, uoff_d;
+        up(urow_d, urow.data(), 4 * urow.size());
+        up(uoff_d, uoff.data(), 4 * uoff.size());
+        g->b_tptr.alloc(4 * ((size_t)n + 1));
+        hipLaunchKernelGGL(tele_rowptr_kernel, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s,
+                           (const uint32_t *)urow_d.as<uint32_t>(), (const uint32_t *)uoff_d.as<uint32_t>(),
+                           (uint32_t)urow.size(), n, g->b_tptr.as<uint32_t>());
+        NKPR_HIP(hipGetLastError());
+        // the staging buffers are freed when this block ends: the expansion must be done first
+        NKPR_HIP(hipStreamSynchronize(s));
+    }
 
     // ---- buffers -------------------------------------------------------------------------------
     const uint32_t ld = (k + 3u) & ~3u;  // 16 bytes of FP32 columns; FP64 uses 2 per lane

@@ -481,6 +481,7 @@ struct nkpr_graph {
     nkpr::DevBuf b_rowptr, b_src, b_coef32, b_coef64;
     std::vector<uint32_t> b_rowptr_host;
     std::vector<int32_t> b_perm_host;
+    std::vector<int32_t> b_iperm_host;  // [z]: compact id of a slot, -1 without a node
     uint32_t b_heavy = 0, b_n_light = 0, b_n_heavy = 0, b_n_pieces = 0;
     nkpr::DevBuf b_light, b_hrow, b_hpiece, b_pieces;
     std::vector<nkpr::BatchTune> b_tuned;
