@@ -102,6 +102,30 @@ struct Stream {
         pos += (size_t)n + 1;
         return v;
     }
+    // steps over n varints without assembling them (the first byte gives each one's length)
+    void skip_varints(uint64_t n) {
+        for (uint64_t i = 0; i < n; ++i) {
+            NKPR_CHECK(pos < end, "nkbg file truncated (varint)");
+            const uint8_t head = p[pos];
+            const size_t len = (size_t)(head ? __builtin_ctz(head) : 8) + 1;
+            NKPR_CHECK(end - pos >= len, "nkbg file truncated (varint)");
+            pos += len;
+        }
+    }
+    void skip_weights(int fmt, uint64_t n) {
+        switch (fmt) {
+        case kVarint:
+        case kSignedVarint: skip_varints(n); break;
+        case kDouble:
+        case kFloat: {
+            const uint64_t size = fmt == kDouble ? 8 : 4;
+            NKPR_CHECK(pos <= end && n <= (end - pos) / size, "nkbg file truncated (weights)");
+            pos += (size_t)(n * size);
+            break;
+        }
+        default: break;
+        }
+    }
     double weight(int fmt) {
         switch (fmt) {
         case kVarint: return (double)varint();
@@ -229,11 +253,18 @@ void nkbg_read_impl(const char *path, int32_t with_out_lists, nkpr_nkbg *out) {
             tw = stream(o_tw, 0, c);
         }
         ChunkLists &L = cl[c];
+        // a directed graph read without out-lists needs only the in-lists: its out-lists are
+        // stepped over (their self-loops are counted in the in-lists, where each appears too)
+        const bool skip_out = directed && !want_out;
         for (uint64_t u = first[c]; u < first[c + 1]; ++u) {
             const uint64_t nout = a.varint(), nin = t.varint();
             NKPR_CHECK(nout <= f.size && nin <= f.size, "nkbg list length outside the file");
             const size_t before = L.ids.size();
-            for (uint64_t j = 0; j < nout; ++j) {
+            if (skip_out) {
+                a.skip_varints(nout);
+                if (weighted) aw.skip_weights(fmt, nout);
+            }
+            for (uint64_t j = 0; j < (skip_out ? 0 : nout); ++j) {
                 const uint64_t v = a.varint();
                 NKPR_CHECK(v < nodes, "nkbg neighbour id out of range");
                 const double w = weighted ? aw.weight(fmt) : 1.0;
@@ -252,7 +283,10 @@ void nkbg_read_impl(const char *path, int32_t with_out_lists, nkpr_nkbg *out) {
                 const uint64_t v = t.varint();
                 NKPR_CHECK(v < nodes, "nkbg neighbour id out of range");
                 const double w = weighted ? tw.weight(fmt) : 1.0;
-                if (!directed && v == u) continue;
+                if (v == u) {
+                    if (!directed) continue;
+                    if (skip_out) ++loops[c];
+                }
                 L.ids.push_back((int32_t)v);
                 if (weighted) L.w.push_back(w);
             }
