@@ -343,10 +343,11 @@ Krylov krylov_setup(nkpr_graph *g) {
     ensure(g->r32, 4 * n);
     Krylov k{};
     // restart length: the Gram-Schmidt cost of a cycle grows with its square, so long cycles
-    // only pay where convergence needs them; 15 measured best on a slowly mixing graph (random
-    // hyperbolic, n = 10M: 30 applications in 25.1 ms against 32 in 27.2 ms with 20), and the
-    // refinements of the other workloads end within 15 steps
-    k.m = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 15), 2), kMaxKrylov);
+    // only pay where convergence needs them.  Measured in alternating processes: a slowly mixing
+    // graph (random hyperbolic, n = 10M) 24.6 ms per solve with 10, 24.7-24.8 with 15, 26.2-26.4
+    // with 12 and 27.2 with 20; a batch of 512 Erdos-Renyi graphs 47.4-47.5 ms with 10 and
+    // 48.4-48.7 with 12 or 15; the R-MAT benchmark's refinements end within 10 steps
+    k.m = (int)std::min<uint32_t>(std::max<uint32_t>(env_u32("NKPR_GMRES_RESTART", 10), 2), kMaxKrylov);
     const size_t ld = (n + 63) / 64 * 64;  // every vector on a 256-byte boundary
     ensure(g->kbasis, 4 * (size_t)(k.m + 1) * ld);
     ensure(g->kwork, 4 * (size_t)4 * ld);
