@@ -712,6 +712,34 @@ def test_mixed_closed_form_start(monkeypatch, weighted, sinks, teleport):
     assert warm["sweeps_fp64"] == warm["refinements"] + 1  # every residual was a sweep
 
 
+@pytest.mark.parametrize("binned", ["0", "1"])
+def test_mixed_solve_timing_stats(monkeypatch, binned):
+    """With timing on, a mixed solve reports the device time of its operator applications and
+    FP64 sweeps (HIP events; a binned application's events are read when the next one starts or
+    when the run ends): both positive, together within the run's wall time, and close per
+    application on a second run of the same handle, so that no application's time moves to
+    another run; the scores equal those of an untimed run bit for bit."""
+    monkeypatch.setenv("NKPR_BINNED", binned)
+    z, u, v, w = skewed_graph(60000, 500000, 17, weighted=True)
+    g = oracle_graph(z, u, v, w, True)
+    dg = capi.DeviceGraph(z, g.in_ptr, g.in_src.astype(np.int32), g.in_w.astype(np.float32),
+                          weight64=g.in_w)
+    runs = []
+    for timing in (1, 1, 0):
+        x = np.full(z, 1.0 / z)
+        st = dg.run(x, capi.default_params(precision=capi.NKPR_MIXED, tol=1e-9,
+                                           stop=capi.NKPR_STOP_L1_ERROR, timing=timing))
+        runs.append((x, st))
+    dg.close()
+    for x, st in runs[:2]:
+        assert st["sweeps_fp32"] > 0 and st["ms_sweeps_fp32"] > 0 and st["ms_sweeps_fp64"] > 0
+        assert st["ms_sweeps_fp32"] + st["ms_sweeps_fp64"] <= st["ms_total"]
+    per = [st["ms_sweeps_fp32"] / st["sweeps_fp32"] for _, st in runs[:2]]
+    assert 0.5 < per[0] / per[1] < 2.0, per
+    assert runs[2][1]["ms_sweeps_fp32"] == 0.0
+    assert np.array_equal(runs[0][0], runs[2][0]) and np.array_equal(runs[1][0], runs[2][0])
+
+
 @pytest.mark.parametrize("on_device", [False, True])
 def test_constant_start_without_gather_is_bit_identical(on_device):
     """A mixed run from a constant start on a graph whose every slot holds a node checks the
