@@ -190,7 +190,7 @@ __global__ void inv_degree_kernel(const int32_t *perm, uint32_t n, const uint32_
 }
 
 // flag = 1 when some FP64 weight is not the FP32 weight given beside it.  Each thread keeps its
-// own verdict over a grid-stride loop; one atomic per block at most (the grid is capped by the
+// own result over a grid-stride loop; one atomic per block at most (the grid is capped by the
 // launcher), so a graph whose weights all differ does not serialise on one address.
 __global__ void fp32_exact_kernel(const float *w, const double *w64, uint64_t m, uint32_t *flag) {
     int diff = 0;
